@@ -1,0 +1,1163 @@
+// engine.hip -- MI355X (gfx950) He trajectory engine: kernels + C ABI.
+//
+// Hot path (one GPU):
+//   k_sample  Philox4x32-10 rejection sampler of monteTraj::setUpTraj
+//             (src/monteTraj.cpp:174-278); one thread per trajectory; hits
+//             are compacted into a work list.
+//   k_traj    persistent velocity-Verlet kernel of diffEqSolver::verletVel
+//             (src/diffEqSolver.cpp:81-211): one trajectory per lane, lanes
+//             refilled from the work list with one atomic per wavefront;
+//             every iteration evaluates forceAccel::getReducedVec at each
+//             lane's position.  A lane that reaches the ellipsoid writes an
+//             exit record and takes new work.
+//   k_check   energy-conservation test of an exit (getPotential at the start
+//             and exit points, FP64): accept, or queue a retry with dt/(k+1)
+//             for the next k_traj round (at most three rounds).
+//   k_reduce  per-CCS-integral sums of x = const1 const2 (1 - cos chi)
+//             (src/monteTraj.cpp:456-477), FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "cosims_engine.h"
+#include "kernels.hpp"
+#include "prep.hpp"
+
+using namespace cosims;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_CHECK(expr)                                                                            \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(COSIMS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+enum : int8_t { ST_MISS = 0, ST_OK = 1, ST_FAIL = 2, ST_PENDING = 3, ST_RETRY = 4 };
+
+constexpr double kPiD = 3.14159265358979323846;
+constexpr int32_t kMaxStepsPerAttempt = 200000;
+constexpr double kPi2D = 1.57079632679489661923;
+
+// ------------------------------------------------------------- Philox ----
+__device__ __forceinline__ void philox4x32(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+        uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+}
+
+__device__ __forceinline__ double u01(uint32_t hi, uint32_t lo)
+{
+    uint64_t x = ((uint64_t)hi << 32) | lo;
+    return (double)(x >> 11) * (1.0 / 9007199254740992.0);
+}
+
+struct SampleArgs {
+    uint32_t key0, key1, job, iteration0;
+    int64_t first, n, per_iter;     // per_iter > 0: iteration = j / per_iter
+    double gmin, gmax, max_val, vd_a, vd_norm, bmax, maxdist, ea, eb, ec;
+    int projection;
+    double *p0, *v0, *g;
+    int8_t *status;                 // ST_MISS or ST_PENDING(=hit, to run)
+    int32_t *hit;                   // optional
+    int32_t *items;                 // optional compacted hit list
+    int32_t *n_items;
+};
+
+// setUpTraj: rejection sampling of (g, phi, theta, gamma, b) and projection
+// of the start point onto the ellipsoid.
+__global__ void __launch_bounds__(256) k_sample(SampleArgs A)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool hit = false;
+    if (j < A.n) {
+        int64_t tj = A.first + j;
+        uint32_t it = A.iteration0, traj = (uint32_t)tj;
+        if (A.per_iter > 0) {
+            it = (uint32_t)(tj / A.per_iter);
+            traj = (uint32_t)(tj % A.per_iter);
+        }
+        double g = 0, phi = 0, theta = 0, gamma = 0, b = 0;
+        const double bMax = A.bmax;
+        bool acc = false;
+        for (uint32_t t = 0; t < 4096 && !acc; t++) {
+            uint32_t w[16];
+            for (int blk = 0; blk < 4; blk++) {
+                uint32_t c[4] = {traj, it, A.job, 4 * t + blk};
+                philox4x32(c, A.key0, A.key1);
+                w[4 * blk] = c[0]; w[4 * blk + 1] = c[1]; w[4 * blk + 2] = c[2]; w[4 * blk + 3] = c[3];
+            }
+            double u0 = u01(w[0], w[1]), u1 = u01(w[2], w[3]), u2 = u01(w[4], w[5]), u3 = u01(w[6], w[7]);
+            double u4 = u01(w[8], w[9]), u5 = u01(w[10], w[11]), u6 = u01(w[12], w[13]), u7 = u01(w[14], w[15]);
+            g = u0 * (A.gmax - A.gmin) + A.gmin;
+            double gCheck = u1 * A.max_val;
+            phi = u2 * 2.0 * kPiD;
+            theta = u3 * kPiD;
+            double thetaCheck = 0.5 * u4;
+            gamma = u5 * 2.0 * kPiD;
+            b = u6 * bMax;
+            double bCheck = u7 * (2 / bMax);
+            if (thetaCheck <= sin(theta) / 2 && bCheck <= 2 * b / (bMax * bMax) &&
+                gCheck <= pow(g, 5.0) * exp(-A.vd_a * g * g) * A.vd_norm)
+                acc = true;
+        }
+        const double md = A.maxdist;
+        double P0 = b * cos(gamma) * cos(phi) * cos(theta) - b * sin(gamma) * sin(phi) + md * cos(phi) * sin(theta);
+        double P1 = b * cos(gamma) * sin(phi) * cos(theta) + b * sin(gamma) * cos(phi) + md * sin(phi) * sin(theta);
+        double P2 = md * cos(theta) - b * cos(gamma) * sin(theta);
+        double V0 = -g * cos(phi) * sin(theta), V1 = -g * sin(phi) * sin(theta), V2 = -g * cos(theta);
+        hit = acc;
+        if (acc && A.projection) {
+            double v0 = V0 / A.ea, v1 = V1 / A.eb, v2 = V2 / A.ec;
+            double p0 = P0 / A.ea, p1 = P1 / A.eb, p2 = P2 / A.ec;
+            double c0 = (p1 * v2) - (p2 * v1), c1 = (p2 * v0) - (p0 * v2), c2 = (p0 * v1) - (p1 * v0);
+            double vv = (v0 * v0) + (v1 * v1) + (v2 * v2);
+            double disc = -((c0 * c0) + (c1 * c1) + (c2 * c2)) + vv;
+            if (disc < 0) {
+                hit = false;
+            } else {
+                double pv = (p0 * v0) + (p1 * v1) + (p2 * v2);
+                double pp = -(pv / vv) - sqrt(disc) / vv;
+                P0 = P0 + V0 * pp; P1 = P1 + V1 * pp; P2 = P2 + V2 * pp;
+            }
+        }
+        A.p0[3 * j] = P0; A.p0[3 * j + 1] = P1; A.p0[3 * j + 2] = P2;
+        A.v0[3 * j] = V0; A.v0[3 * j + 1] = V1; A.v0[3 * j + 2] = V2;
+        A.g[j] = g;
+        if (A.status) A.status[j] = hit ? ST_PENDING : ST_MISS;
+        if (A.hit) A.hit[j] = hit ? 1 : 0;
+    }
+    if (A.items) {  // wave-aggregated compaction of hits
+        const bool h = (j < A.n) && hit;
+        const unsigned long long mask = __ballot(h);
+        if (mask) {
+            const int lane = threadIdx.x & 63;
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(A.n_items, __popcll(mask));
+            base = __shfl(base, leader);
+            if (h) A.items[base + __popcll(mask & ((1ull << lane) - 1))] = (int32_t)j;
+        }
+    }
+}
+
+// ------------------------------------------------------ trajectory ---
+struct TrajArgs {
+    MolDev m;
+    const int32_t *items;
+    int32_t n_items;
+    int32_t *counter;
+    const double *p0, *v0;
+    int32_t *tries, *maxtries;
+    int8_t *status;
+    double *chi;
+    int32_t *steps, *totsteps;
+    double *exit_pos, *exit_v2;
+    double dt0;
+    int32_t max_steps;          // per-attempt step cap (the reference has none)
+    unsigned long long *evals;  // [0] integration steps, [1] force evaluations of this launch
+};
+
+template <int PREC, int FT>
+__global__ void __launch_bounds__(256) k_traj(TrajArgs A)
+{
+    const MolDev &m = A.m;
+    const int lane = threadIdx.x & 63;
+    int t = 0, state = 0, tries = 0, maxtries = 8, orbits = 0, steps = 0, tot = 0;
+    double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0, ax = 0, ay = 0, az = 0, dt = 0, rdot = 0;
+    bool exhausted = false;  // wave-uniform
+    unsigned n_steps = 0, n_evals = 0;
+    for (;;) {
+        // refill idle lanes: one atomic per wavefront
+        if (!exhausted) {
+            const bool need = (state == 0);
+            const unsigned long long mask = __ballot(need);
+            if (mask) {
+                const int cnt = __popcll(mask);
+                const int leader = __ffsll((unsigned long long)mask) - 1;
+                int base = 0;
+                if (lane == leader) base = atomicAdd(A.counter, cnt);
+                base = __shfl(base, leader);
+                if (base + cnt >= A.n_items) exhausted = true;
+                if (need) {
+                    const int idx = base + __popcll(mask & ((1ull << lane) - 1));
+                    if (idx < A.n_items) {
+                        t = A.items[idx];
+                        tries = A.tries[t];
+                        maxtries = A.maxtries[t];
+                        dt = A.dt0 / (double)(tries + 1);
+                        px = A.p0[3 * t]; py = A.p0[3 * t + 1]; pz = A.p0[3 * t + 2];
+                        tot = 0;
+                        state = 1;
+                    }
+                }
+            }
+        }
+        if (!__any(state != 0)) break;
+
+        double ex, ey, ez, hx = 0, hy = 0, hz = 0;
+        if (state == 2) {  // src/diffEqSolver.cpp:131-132
+            ex = px + vx * dt + ax * dt * dt * 0.5;
+            ey = py + vy * dt + ay * dt * dt * 0.5;
+            ez = pz + vz * dt + az * dt * dt * 0.5;
+            hx = vx + ax * dt * 0.5;
+            hy = vy + ay * dt * 0.5;
+            hz = vz + az * dt * 0.5;
+        } else if (state == 1) {
+            ex = px; ey = py; ez = pz;
+        } else {
+            ex = ey = ez = 1.0e4;  // idle lane: any finite far point
+        }
+        double fx, fy, fz;
+        accel<PREC, FT>(m, ex, ey, ez, fx, fy, fz);
+        n_evals += (state != 0);
+        n_steps += (state == 2);
+
+        if (state == 1) {  // start of an attempt, src/diffEqSolver.cpp:112-127
+            ax = fx; ay = fy; az = fz;
+            vx = A.v0[3 * t]; vy = A.v0[3 * t + 1]; vz = A.v0[3 * t + 2];
+            orbits = 0;
+            steps = 0;
+            rdot = px * vx + py * vy + pz * vz;
+            state = 2;
+        } else if (state == 2) {  // src/diffEqSolver.cpp:133-205
+            px = ex; py = ey; pz = ez;
+            ax = fx; ay = fy; az = fz;
+            vx = hx + ax * dt * 0.5;
+            vy = hy + ay * dt * 0.5;
+            vz = hz + az * dt * 0.5;
+            steps++;
+            tot++;
+            const double rold = rdot;
+            rdot = px * vx + py * vy + pz * vz;
+            const double prod = rdot * rold;
+            bool track = true;
+            if (steps > 2 && prod < 0 && !(rdot > rold)) {
+                orbits++;
+                if (orbits > 8) track = false;
+            }
+            // A non-finite position or a runaway attempt can never reach the
+            // exit test; it ends as a failed trajectory (status 2).
+            const bool runaway = !(fabs(px) + fabs(py) + fabs(pz) < 1.0e30) || steps > A.max_steps;
+            if (runaway) {
+                A.status[t] = ST_FAIL;
+                A.chi[t] = kPi2D;
+                A.steps[t] = steps;
+                A.totsteps[t] += tot;
+                A.tries[t] = tries;
+                state = 0;
+            } else if (!track) {
+                tries++;
+                dt = A.dt0 / (double)(tries + 1);
+                if (!(tries < maxtries && maxtries < 10)) {  // loop exit: failed trajectory
+                    A.status[t] = ST_FAIL;
+                    A.chi[t] = kPi2D;
+                    A.steps[t] = steps;
+                    A.totsteps[t] += tot;
+                    A.tries[t] = tries;
+                    state = 0;
+                }
+            } else {
+                const double tx = px / m.a, ty = py / m.b, tz = pz / m.c;
+                const double tm2 = tx * tx + ty * ty + tz * tz;
+                if (((tm2 > 1 && m.projection) || (tm2 > m.maxrad && !m.projection)) && steps > 10) {
+                    // exit: scattering angle (vector3D::Angle, src/vector3D.h:134-143)
+                    const double w0 = A.v0[3 * t], w1 = A.v0[3 * t + 1], w2 = A.v0[3 * t + 2];
+                    const double t1 = w0 * vx + w1 * vy + w2 * vz;
+                    const double t2 = sqrt(w0 * w0 + w1 * w1 + w2 * w2) * sqrt(vx * vx + vy * vy + vz * vz);
+                    const double t3 = t1 / t2;
+                    A.chi[t] = (t3 > 1) ? 0.0 : ((t3 < -1) ? kPiD : acos(t1 / t2));
+                    A.exit_pos[3 * t] = px; A.exit_pos[3 * t + 1] = py; A.exit_pos[3 * t + 2] = pz;
+                    A.exit_v2[t] = vx * vx + vy * vy + vz * vz;
+                    A.steps[t] = steps;
+                    A.totsteps[t] += tot;
+                    A.tries[t] = tries;
+                    A.maxtries[t] = maxtries;
+                    A.status[t] = ST_PENDING;
+                    state = 0;
+                }
+            }
+        }
+    }
+    // per-launch counters: wave sum, one atomic per wavefront
+    unsigned long long s = n_steps, e = n_evals;
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off);
+        e += __shfl_xor(e, off);
+    }
+    if (lane == 0) {
+        atomicAdd(A.evals, s);
+        atomicAdd(A.evals + 1, e);
+    }
+}
+
+// ------------------------------------------------------ energy check ---
+struct CheckArgs {
+    MolDev m;
+    const int32_t *items;
+    int32_t n_items;
+    int first_round;
+    const double *p0, *v0, *exit_pos, *exit_v2;
+    double *e0, *chi, *ergy;
+    int8_t *status;
+    int32_t *tries, *maxtries, *steps;
+    int32_t *next_items, *next_count;
+};
+
+// src/diffEqSolver.cpp:108, 178-203
+__global__ void __launch_bounds__(256) k_check(CheckArgs A)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool retry = false;
+    int t = 0;
+    if (i < A.n_items) {
+        t = A.items[i];
+        if (A.status[t] == ST_PENDING) {
+            const MolDev &m = A.m;
+            const double hm = 0.5 * m.mu;
+            double e0;
+            if (A.first_round) {
+                const double *p = A.p0 + 3 * t, *v = A.v0 + 3 * t;
+                e0 = potential(m, p[0], p[1], p[2]) + hm * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                A.e0[t] = e0;
+            } else {
+                e0 = A.e0[t];
+            }
+            const double *q = A.exit_pos + 3 * t;
+            const double e1 = potential(m, q[0], q[1], q[2]) + hm * A.exit_v2[t];
+            const double pct = 100 * fabs(e0 - e1) / e0;
+            if (pct > 0.5) {
+                const int tr = A.tries[t] + 1, mt = A.maxtries[t] + 1;
+                A.tries[t] = tr;
+                A.maxtries[t] = mt;
+                if (tr < mt && mt < 10) {
+                    A.status[t] = ST_RETRY;
+                    retry = true;
+                } else {
+                    A.status[t] = ST_FAIL;
+                    A.chi[t] = kPi2D;
+                    A.steps[t] = 0;
+                }
+            } else {
+                A.status[t] = ST_OK;
+                A.ergy[t] = pct;
+            }
+        }
+    }
+    const unsigned long long mask = __ballot(retry);
+    if (mask) {
+        const int lane = threadIdx.x & 63;
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(A.next_count, __popcll(mask));
+        base = __shfl(base, leader);
+        if (retry) A.next_items[base + __popcll(mask & ((1ull << lane) - 1))] = t;
+    }
+}
+
+// ------------------------------------------------------------ reduce ---
+struct ReduceArgs {
+    const int8_t *status;
+    const double *chi, *ergy;
+    const int32_t *totsteps;
+    int64_t per_iter;
+    double ccs_const;
+    double *out;  // per iteration: total, total2, max_ergy, steps, hits, fails
+};
+
+// runThreads x = const1*const2*(1 - cos|chi|) (src/monteTraj.cpp:471-475);
+// one workgroup per CCS integral, fixed-order tree.
+__global__ void __launch_bounds__(256) k_reduce(ReduceArgs A)
+{
+    const int it = blockIdx.x;
+    const int64_t base = (int64_t)it * A.per_iter;
+    double s = 0, s2 = 0, mx = 0, st = 0, nh = 0, nf = 0;
+    for (int64_t j = threadIdx.x; j < A.per_iter; j += blockDim.x) {
+        const int8_t stt = A.status[base + j];
+        const double c = (stt == ST_MISS) ? 0.0 : A.chi[base + j];
+        const double x = A.ccs_const * (1 - cos(fabs(c)));
+        s += x;
+        s2 += x * x;
+        if (stt == ST_OK) mx = fmax(mx, A.ergy[base + j]);
+        if (stt != ST_MISS) {
+            nh += 1;
+            st += A.totsteps[base + j];
+        }
+        if (stt == ST_FAIL) nf += 1;
+    }
+    __shared__ double red[6][256];
+    red[0][threadIdx.x] = s; red[1][threadIdx.x] = s2; red[2][threadIdx.x] = mx;
+    red[3][threadIdx.x] = st; red[4][threadIdx.x] = nh; red[5][threadIdx.x] = nf;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            for (int k = 0; k < 6; k++) {
+                if (k == 2) red[k][threadIdx.x] = fmax(red[k][threadIdx.x], red[k][threadIdx.x + w]);
+                else red[k][threadIdx.x] += red[k][threadIdx.x + w];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) A.out[6 * it + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// --------------------------------------------------- batch evaluators ---
+template <int PREC, int FT>
+__global__ void __launch_bounds__(256) k_accel(MolDev m, int64_t n, const double *pos, double *acc)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double x = 1.0e4, y = 1.0e4, z = 1.0e4;
+    if (i < n) { x = pos[3 * i]; y = pos[3 * i + 1]; z = pos[3 * i + 2]; }
+    double ox, oy, oz;
+    accel<PREC, FT>(m, x, y, z, ox, oy, oz);
+    if (i < n) { acc[3 * i] = ox; acc[3 * i + 1] = oy; acc[3 * i + 2] = oz; }
+}
+
+__global__ void __launch_bounds__(256) k_potential(MolDev m, int64_t n, const double *pos, double *u)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) u[i] = potential(m, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+}
+
+// ------------------------------------------------- kernel selection ---
+using TrajFn = void (*)(TrajArgs);
+using AccFn = void (*)(MolDev, int64_t, const double *, double *);
+
+template <int P>
+TrajFn traj_fn(int ft)
+{
+    switch (ft) {
+    case 1: return k_traj<P, 1>;
+    case 2: return k_traj<P, 2>;
+    case 3: return k_traj<P, 3>;
+    case 4: return k_traj<P, 4>;
+    case 5: return k_traj<P, 5>;
+    default: return k_traj<P, 6>;
+    }
+}
+TrajFn traj_kernel(int prec, int ft) { return prec == COSIMS_PREC_FP64 ? traj_fn<0>(ft) : traj_fn<1>(ft); }
+
+template <int P>
+AccFn acc_fn(int ft)
+{
+    switch (ft) {
+    case 1: return k_accel<P, 1>;
+    case 2: return k_accel<P, 2>;
+    case 3: return k_accel<P, 3>;
+    case 4: return k_accel<P, 4>;
+    case 5: return k_accel<P, 5>;
+    default: return k_accel<P, 6>;
+    }
+}
+AccFn accel_kernel(int prec, int ft) { return prec == COSIMS_PREC_FP64 ? acc_fn<0>(ft) : acc_fn<1>(ft); }
+
+// ------------------------------------------------------ device state ---
+struct Device {
+    int id = -1, cus = 0;
+    char name[256] = {0};
+};
+
+int device_init(Device &d)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(COSIMS_E_NODEVICE, "no HIP device");
+    HIP_CHECK(hipGetDevice(&d.id));
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, d.id));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(COSIMS_E_NODEVICE, std::string("engine is built for gfx950, device is ") + prop.gcnArchName);
+    d.cus = prop.multiProcessorCount;
+    std::snprintf(d.name, sizeof(d.name), "%s", prop.name);
+    return 0;
+}
+
+// RAII device buffer
+struct DBuf {
+    void *p = nullptr;
+    ~DBuf() { if (p) (void)hipFree(p); }
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 16); }
+    template <typename T> T *as() const { return (T *)p; }
+};
+
+struct Stream {
+    hipStream_t s = nullptr;
+    ~Stream() { if (s) (void)hipStreamDestroy(s); }
+};
+
+}  // namespace
+
+// -------------------------------------------------------------- handle ---
+struct cosims_mol {
+    Params params;
+    Prepared prep;
+    double setup_ms = 0;
+    // device side, created on first GPU use (std::call_once)
+    mutable std::once_flag up_once;
+    mutable int up_rc = 0;
+    mutable std::string up_err;
+    mutable Device dev;
+    mutable MolDev md{};
+    mutable void *dmem = nullptr;
+    ~cosims_mol() { if (dmem) (void)hipFree(dmem); }
+};
+
+namespace {
+
+Params to_params(const cosims_params *p)
+{
+    Params q;
+    q.temp = p->temp; q.dt = p->dt; q.traj = p->traj; q.iter = p->iter; q.charge = p->charge != 0;
+    q.seed = p->seed; q.multipole = p->multipole != 0; q.dispersion_cutoff = p->dispersion_cutoff != 0;
+    q.dispersion_radius = p->dispersion_radius; q.multipole_radius = p->multipole_radius;
+    q.multipole_order = p->multipole_order; q.max_cluster_size = p->max_cluster_size;
+    q.projection = p->projection != 0; q.print_rate = p->print_rate; q.total_charge = p->total_charge;
+    q.threads = p->threads;
+    return q;
+}
+
+template <typename T>
+void fill_atom(T &a, double x, double y, double z, double q, double s6, double e24, double e4)
+{
+    a.x = x; a.y = y; a.z = z; a.q = q; a.s6 = s6; a.e24 = e24; a.e4 = e4; a.pad = 0;
+}
+
+template <typename T>
+void fill_head(ClusterT<T> &h, const Prepared &p, int c)
+{
+    h.px = p.cl_pos[3 * c]; h.py = p.cl_pos[3 * c + 1]; h.pz = p.cl_pos[3 * c + 2];
+    double mr = p.multipole_radius + p.cl_maxdist[c];
+    mr *= mr;
+    double dr = p.dispersion_radius + p.cl_maxdist[c];
+    dr *= dr;
+    h.mpr = mr; h.dpr = dr; h.q = p.cl_q[c];
+    h.d0 = p.cl_dip[3 * c]; h.d1 = p.cl_dip[3 * c + 1]; h.d2 = p.cl_dip[3 * c + 2];
+    for (int k = 0; k < 9; k++) h.Q[k] = p.cl_quad[9 * c + k];
+    // the band branch reads data[n] with n the cluster index (src/forceAccel.cpp:353-355)
+    const int qi = c < p.n ? c : 0;
+    h.qs6 = p.sig6[qi]; h.qe24 = p.eps24m[qi];
+    h.start = p.cl_start[c]; h.end = p.cl_start[c + 1]; h.rep = p.cl_replace[c]; h.pad = 0;
+}
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int upload(const cosims_mol *M)
+{
+    const Prepared &p = M->prep;
+    const int n = p.n, ncl = p.ncl;
+    size_t o64 = 0, o32 = align_up(o64 + sizeof(AtomF64) * n), oc64 = align_up(o32 + sizeof(AtomF32) * n),
+           oc32 = align_up(oc64 + sizeof(AtomF64) * n), oh64 = align_up(oc32 + sizeof(AtomF32) * n),
+           oh32 = align_up(oh64 + sizeof(ClusterF64) * ncl), total = align_up(oh32 + sizeof(ClusterF32) * ncl);
+    std::vector<char> host(total, 0);
+    auto *a64 = (AtomF64 *)(host.data() + o64);
+    auto *a32 = (AtomF32 *)(host.data() + o32);
+    auto *c64 = (AtomF64 *)(host.data() + oc64);
+    auto *c32 = (AtomF32 *)(host.data() + oc32);
+    auto *h64 = (ClusterF64 *)(host.data() + oh64);
+    auto *h32 = (ClusterF32 *)(host.data() + oh32);
+    for (int i = 0; i < n; i++) {
+        const double *x = &p.xyz[3 * i];
+        fill_atom(a64[i], x[0], x[1], x[2], p.q[i], p.sig6[i], p.eps24m[i], p.eps4[i]);
+        fill_atom(a32[i], x[0], x[1], x[2], p.q[i], p.sig6[i], p.eps24m[i], p.eps4[i]);
+    }
+    for (int k = 0; k < (ncl > 0 ? n : 0); k++) {
+        const int i = p.cl_atoms[k];
+        const double *x = &p.xyz[3 * i];
+        fill_atom(c64[k], x[0], x[1], x[2], p.q[i], p.sig6[i], p.eps24m[i], p.eps4[i]);
+        fill_atom(c32[k], x[0], x[1], x[2], p.q[i], p.sig6[i], p.eps24m[i], p.eps4[i]);
+    }
+    for (int c = 0; c < ncl; c++) {
+        fill_head(h64[c], p, c);
+        fill_head(h32[c], p, c);
+    }
+    HIP_CHECK(hipMalloc(&M->dmem, total));
+    HIP_CHECK(hipMemcpy(M->dmem, host.data(), total, hipMemcpyHostToDevice));
+    char *d = (char *)M->dmem;
+    MolDev &m = M->md;
+    m.n = n; m.ncl = ncl; m.force_type = p.force_type; m.charged = p.charged ? 1 : 0;
+    m.order = p.multipole_order; m.projection = p.projection ? 1 : 0;
+    m.mu = p.mu; m.dipole_scalar = p.dipole_scalar; m.a = p.a; m.b = p.b; m.c = p.c;
+    m.maxrad = std::max(p.a, std::max(p.b, p.c));
+    m.a64 = (const AtomF64 *)(d + o64); m.a32 = (const AtomF32 *)(d + o32);
+    m.c64 = (const AtomF64 *)(d + oc64); m.c32 = (const AtomF32 *)(d + oc32);
+    m.h64 = (const ClusterF64 *)(d + oh64); m.h32 = (const ClusterF32 *)(d + oh32);
+    return 0;
+}
+
+// Device upload on first GPU use; every later call sees the same result.
+int ensure_device(const cosims_mol *M)
+{
+    std::call_once(M->up_once, [M]() {
+        M->up_rc = device_init(M->dev);
+        if (M->up_rc == 0) M->up_rc = upload(M);
+        if (M->up_rc) M->up_err = g_err;
+    });
+    if (M->up_rc) g_err = M->up_err;
+    return M->up_rc;
+}
+
+int finish_create(cosims_mol *M, cosims_mol **out, std::chrono::steady_clock::time_point t0)
+{
+    M->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *out = M;
+    return 0;
+}
+
+int grid_for(const void *fn, int wg, int blocks_per_cu, int cus, int64_t items)
+{
+    int occ = 0;
+    if (blocks_per_cu <= 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, wg, 0) != hipSuccess || occ <= 0) occ = 2;
+    } else {
+        occ = blocks_per_cu;
+    }
+    int64_t g = (int64_t)occ * cus;
+    int64_t need = (items + wg - 1) / wg;
+    return (int)std::max<int64_t>(1, std::min(g, need));
+}
+
+}  // namespace
+
+// ================================================================ C ABI ===
+extern "C" {
+
+const char *cosims_strerror(int code)
+{
+    switch (code) {
+    case COSIMS_OK: return "ok";
+    case COSIMS_E_INVALID: return "invalid argument";
+    case COSIMS_E_HIP: return "HIP runtime error";
+    case COSIMS_E_NOMEM: return "out of memory";
+    case COSIMS_E_SETUP: return "molecule setup failed";
+    case COSIMS_E_IO: return "file could not be read";
+    case COSIMS_E_UNSUPPORTED: return "input outside the defined behaviour of the reference";
+    case COSIMS_E_NODEVICE: return "no usable gfx950 device";
+    default: return "unknown error";
+    }
+}
+
+const char *cosims_last_error(void) { return g_err.c_str(); }
+
+int cosims_device_sync(void)
+{
+    HIP_CHECK(hipDeviceSynchronize());
+    return 0;
+}
+
+int cosims_device_info(char *name, size_t cap, int32_t *cus)
+{
+    Device d;
+    int rc = device_init(d);
+    if (rc) return rc;
+    if (name && cap) std::snprintf(name, cap, "%s", d.name);
+    if (cus) *cus = d.cus;
+    return 0;
+}
+
+void cosims_params_default(cosims_params *p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->temp = 298; p->dt = 0.01; p->traj = 50000; p->iter = 10; p->charge = 0; p->seed = 0;
+    p->multipole = 1; p->dispersion_cutoff = 1; p->dispersion_radius = 0; p->multipole_radius = 0;
+    p->multipole_order = 1; p->max_cluster_size = 4; p->projection = 1; p->print_rate = 20000;
+    p->total_charge = 0; p->threads = 0;
+}
+
+int cosims_mol_create(const cosims_params *p, const cosims_atoms *a, cosims_mol **out)
+{
+    if (!p || !a || !out || a->n <= 0) return fail(COSIMS_E_INVALID, "cosims_mol_create: null argument or empty molecule");
+    auto t0 = std::chrono::steady_clock::now();
+    std::unique_ptr<cosims_mol> M(new cosims_mol());
+    M->params = to_params(p);
+    std::string err;
+    int rc = prepare(M->params, a->n, a->xyz, a->mass, a->sigma, a->epsilon, a->charge, a->sigma_max, M->prep, err);
+    if (rc) return fail(rc, err);
+    return finish_create(M.release(), out, t0);
+}
+
+int cosims_mol_create_prepared(const cosims_params *p, const cosims_prepared *s, cosims_mol **out)
+{
+    if (!p || !s || !out || s->n <= 0 || !s->xyz || !s->sig6 || !s->eps24m || !s->eps4)
+        return fail(COSIMS_E_INVALID, "cosims_mol_create_prepared: missing arrays");
+    auto t0 = std::chrono::steady_clock::now();
+    std::unique_ptr<cosims_mol> M(new cosims_mol());
+    M->params = to_params(p);
+    Prepared &m = M->prep;
+    const int n = s->n;
+    m.n = n;
+    m.xyz.assign(s->xyz, s->xyz + 3 * n);
+    m.q.assign(n, 0.0);
+    if (s->charge) m.q.assign(s->charge, s->charge + n);
+    m.sig6.assign(s->sig6, s->sig6 + n);
+    m.eps24m.assign(s->eps24m, s->eps24m + n);
+    m.eps4.assign(s->eps4, s->eps4 + n);
+    m.mu = s->mu;
+    m.a = s->a; m.b = s->b; m.c = s->c;
+    m.dispersion_radius = s->dispersion_radius;
+    m.multipole_radius = s->multipole_radius;
+    m.ncl = s->ncl;
+    if (m.ncl > 0) {
+        if (!s->cl_start || !s->cl_atoms || !s->cl_center || !s->cl_maxdist)
+            return fail(COSIMS_E_INVALID, "cosims_mol_create_prepared: cluster arrays missing");
+        m.cl_start.assign(s->cl_start, s->cl_start + m.ncl + 1);
+        if (m.cl_start[0] != 0 || m.cl_start[m.ncl] != n)
+            return fail(COSIMS_E_INVALID, "cluster offsets must cover all atoms");
+        for (int c = 0; c < m.ncl; c++)
+            if (m.cl_start[c + 1] < m.cl_start[c]) return fail(COSIMS_E_INVALID, "cluster offsets not monotone");
+        m.cl_atoms.assign(s->cl_atoms, s->cl_atoms + n);
+        for (int32_t i : m.cl_atoms)
+            if (i < 0 || i >= n) return fail(COSIMS_E_INVALID, "cluster member index out of range");
+        m.cl_center.assign(s->cl_center, s->cl_center + 3 * m.ncl);
+        m.cl_maxdist.assign(s->cl_maxdist, s->cl_maxdist + m.ncl);
+    }
+    int ft = force_type_of(p->charge != 0, p->multipole != 0, p->dispersion_cutoff != 0);
+    if (ft != 4 && ft != 6 && m.ncl == 0) return fail(COSIMS_E_INVALID, "cut-off force types need clusters");
+    std::string err;
+    int rc = complete_prepared(M->params, m, err);
+    if (rc) return fail(rc, err);
+    return finish_create(M.release(), out, t0);
+}
+
+void cosims_mol_destroy(cosims_mol *m) { delete m; }
+
+int cosims_mol_upload(const cosims_mol *m)
+{
+    if (!m) return fail(COSIMS_E_INVALID, "null handle");
+    return ensure_device(m);
+}
+
+int cosims_mol_info_get(const cosims_mol *M, cosims_mol_info *o)
+{
+    if (!M || !o) return fail(COSIMS_E_INVALID, "null argument");
+    const Prepared &p = M->prep;
+    std::memset(o, 0, sizeof(*o));
+    o->n = p.n; o->ncl = p.ncl; o->force_type = p.force_type; o->charged = p.charged;
+    o->projection = p.projection; o->multipole_order = p.multipole_order;
+    o->mu = p.mu; o->total_mass = p.total_mass; o->dipole_scalar = p.dipole_scalar;
+    o->a = p.a; o->b = p.b; o->c = p.c; o->bmax = p.bmax; o->enlarge = p.enlarge;
+    o->gmin = p.vd.gmin; o->gmax = p.vd.gmax; o->gmean = p.vd.mean; o->gmode = p.vd.mode;
+    o->gsigma = p.vd.sigma; o->vd_a = p.vd.a; o->vd_norm = p.vd.norm; o->max_val = p.vd.max_val;
+    o->cdf_frac = p.vd.cdf_frac; o->ccs_const = p.ccs_const; o->min_kin = p.vd.min_kin;
+    o->dispersion_radius = p.dispersion_radius; o->multipole_radius = p.multipole_radius;
+    std::memcpy(o->rot, p.rot, sizeof(o->rot));
+    std::memcpy(o->max_xyz, p.max_xyz, sizeof(o->max_xyz));
+    o->fit_abc[0] = p.fit_a; o->fit_abc[1] = p.fit_b; o->fit_abc[2] = p.fit_c; o->fit_rmsd = p.fit_rmsd;
+    o->n_boundary = p.n_boundary;
+    o->setup_ms = M->setup_ms;
+    return 0;
+}
+
+int cosims_mol_atoms(const cosims_mol *M, double *xyz, double *q, double *s6, double *e24, double *e4)
+{
+    if (!M) return fail(COSIMS_E_INVALID, "null handle");
+    const Prepared &p = M->prep;
+    if (xyz) std::memcpy(xyz, p.xyz.data(), sizeof(double) * 3 * p.n);
+    if (q) std::memcpy(q, p.q.data(), sizeof(double) * p.n);
+    if (s6) std::memcpy(s6, p.sig6.data(), sizeof(double) * p.n);
+    if (e24) std::memcpy(e24, p.eps24m.data(), sizeof(double) * p.n);
+    if (e4) std::memcpy(e4, p.eps4.data(), sizeof(double) * p.n);
+    return 0;
+}
+
+int cosims_mol_clusters(const cosims_mol *M, int32_t *start, int32_t *atoms, double *center, double *maxdist,
+                        double *pos, double *q, double *dip, double *quad, int32_t *rep)
+{
+    if (!M) return fail(COSIMS_E_INVALID, "null handle");
+    const Prepared &p = M->prep;
+    const int c = p.ncl;
+    if (c == 0) return 0;
+    if (start) std::memcpy(start, p.cl_start.data(), sizeof(int32_t) * (c + 1));
+    if (atoms) std::memcpy(atoms, p.cl_atoms.data(), sizeof(int32_t) * p.n);
+    if (center) std::memcpy(center, p.cl_center.data(), sizeof(double) * 3 * c);
+    if (maxdist) std::memcpy(maxdist, p.cl_maxdist.data(), sizeof(double) * c);
+    if (pos) std::memcpy(pos, p.cl_pos.data(), sizeof(double) * 3 * c);
+    if (q) std::memcpy(q, p.cl_q.data(), sizeof(double) * c);
+    if (dip) std::memcpy(dip, p.cl_dip.data(), sizeof(double) * 3 * c);
+    if (quad) std::memcpy(quad, p.cl_quad.data(), sizeof(double) * 9 * c);
+    if (rep) std::memcpy(rep, p.cl_replace.data(), sizeof(int32_t) * c);
+    return 0;
+}
+
+int cosims_accel(const cosims_mol *M, int32_t prec, int64_t n, const double *pos, double *acc)
+{
+    if (!M || (n > 0 && (!pos || !acc)) || n < 0) return fail(COSIMS_E_INVALID, "cosims_accel: bad arguments");
+    if (prec != COSIMS_PREC_FP64 && prec != COSIMS_PREC_MIXED) return fail(COSIMS_E_INVALID, "unknown precision");
+    if (n == 0) return 0;
+    if (int rc = ensure_device(M)) return rc;
+    Stream st;
+    HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+    DBuf dp, da;
+    HIP_CHECK(dp.alloc(sizeof(double) * 3 * n));
+    HIP_CHECK(da.alloc(sizeof(double) * 3 * n));
+    HIP_CHECK(hipMemcpyAsync(dp.p, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st.s));
+    AccFn fn = accel_kernel(prec, M->prep.force_type);
+    hipLaunchKernelGGL(fn, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st.s, M->md, n, dp.as<double>(),
+                       da.as<double>());
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(acc, da.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st.s));
+    HIP_CHECK(hipStreamSynchronize(st.s));
+    return 0;
+}
+
+int cosims_potential(const cosims_mol *M, int64_t n, const double *pos, double *u)
+{
+    if (!M || n < 0 || (n > 0 && (!pos || !u))) return fail(COSIMS_E_INVALID, "cosims_potential: bad arguments");
+    if (n == 0) return 0;
+    if (int rc = ensure_device(M)) return rc;
+    Stream st;
+    HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+    DBuf dp, du;
+    HIP_CHECK(dp.alloc(sizeof(double) * 3 * n));
+    HIP_CHECK(du.alloc(sizeof(double) * n));
+    HIP_CHECK(hipMemcpyAsync(dp.p, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st.s));
+    hipLaunchKernelGGL(k_potential, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st.s, M->md, n,
+                       dp.as<double>(), du.as<double>());
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(u, du.p, sizeof(double) * n, hipMemcpyDeviceToHost, st.s));
+    HIP_CHECK(hipStreamSynchronize(st.s));
+    return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+SampleArgs sample_args(const cosims_mol *M, uint64_t seed, uint32_t job)
+{
+    const Prepared &p = M->prep;
+    SampleArgs A{};
+    A.key0 = (uint32_t)seed; A.key1 = (uint32_t)(seed >> 32); A.job = job;
+    A.gmin = p.vd.gmin; A.gmax = p.vd.gmax; A.max_val = p.vd.max_val; A.vd_a = p.vd.a; A.vd_norm = p.vd.norm;
+    A.bmax = p.bmax;
+    A.maxdist = (double)(float)p.bmax;  // monteTraj::maxDist is a float (src/monteTraj.h:67, :87)
+    A.ea = p.a; A.eb = p.b; A.ec = p.c; A.projection = p.projection ? 1 : 0;
+    return A;
+}
+
+// Work buffers of one batch of trajectories (device).
+struct TrajBufs {
+    DBuf p0, v0, g, status, chi, steps, tot, tries, maxtries, exitp, exitv2, e0, ergy, items[2], counters, evals;
+    int alloc(int64_t n)
+    {
+        HIP_CHECK(p0.alloc(sizeof(double) * 3 * n));
+        HIP_CHECK(v0.alloc(sizeof(double) * 3 * n));
+        HIP_CHECK(g.alloc(sizeof(double) * n));
+        HIP_CHECK(status.alloc(n));
+        HIP_CHECK(chi.alloc(sizeof(double) * n));
+        HIP_CHECK(steps.alloc(sizeof(int32_t) * n));
+        HIP_CHECK(tot.alloc(sizeof(int32_t) * n));
+        HIP_CHECK(tries.alloc(sizeof(int32_t) * n));
+        HIP_CHECK(maxtries.alloc(sizeof(int32_t) * n));
+        HIP_CHECK(exitp.alloc(sizeof(double) * 3 * n));
+        HIP_CHECK(exitv2.alloc(sizeof(double) * n));
+        HIP_CHECK(e0.alloc(sizeof(double) * n));
+        HIP_CHECK(ergy.alloc(sizeof(double) * n));
+        HIP_CHECK(items[0].alloc(sizeof(int32_t) * n));
+        HIP_CHECK(items[1].alloc(sizeof(int32_t) * n));
+        HIP_CHECK(counters.alloc(sizeof(int32_t) * 64));
+        HIP_CHECK(evals.alloc(sizeof(unsigned long long) * 2 * COSIMS_MAX_ROUNDS));
+        return 0;
+    }
+};
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Runs the attempt rounds over the hit list in B.items[0] (count in
+// counters[0]).  Returns per-round stats.
+struct RoundStats {
+    int rounds = 0;
+    double traj_ms[COSIMS_MAX_ROUNDS] = {0};
+    int64_t items[COSIMS_MAX_ROUNDS] = {0};
+    int64_t steps[COSIMS_MAX_ROUNDS] = {0}, evals[COSIMS_MAX_ROUNDS] = {0};
+    double check_ms = 0;
+    int64_t retries = 0;
+};
+
+int run_rounds(const cosims_mol *M, int prec, int64_t n, TrajBufs &B, hipStream_t s, int wg, int bpc,
+               RoundStats &R)
+{
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, s, B.tries.as<int32_t>(), n, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, s, B.maxtries.as<int32_t>(), n, 8);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, s, B.tot.as<int32_t>(), n, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, B.evals.as<int32_t>(), 4 * COSIMS_MAX_ROUNDS, 0);
+    HIP_CHECK(hipGetLastError());
+    int32_t *ctr = B.counters.as<int32_t>();
+    int32_t count = 0;
+    HIP_CHECK(hipMemcpyAsync(&count, ctr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    TrajFn fn = traj_kernel(prec, M->prep.force_type);
+    hipEvent_t e[4];
+    for (auto &x : e) HIP_CHECK(hipEventCreate(&x));
+    int cur = 0;
+    int rc = 0;
+    for (int r = 0; r < COSIMS_MAX_ROUNDS && count > 0; r++) {
+        // counters: [1] = work counter of this round, [2] = retry count
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ctr + 1, 2, 0);
+        TrajArgs T{};
+        T.m = M->md;
+        T.items = B.items[cur].as<int32_t>();
+        T.n_items = count;
+        T.counter = ctr + 1;
+        T.p0 = B.p0.as<double>(); T.v0 = B.v0.as<double>();
+        T.tries = B.tries.as<int32_t>(); T.maxtries = B.maxtries.as<int32_t>();
+        T.status = B.status.as<int8_t>(); T.chi = B.chi.as<double>();
+        T.steps = B.steps.as<int32_t>(); T.totsteps = B.tot.as<int32_t>();
+        T.exit_pos = B.exitp.as<double>(); T.exit_v2 = B.exitv2.as<double>();
+        T.dt0 = M->params.dt;
+        T.max_steps = kMaxStepsPerAttempt;
+        T.evals = B.evals.as<unsigned long long>() + 2 * r;
+        const int grid = grid_for((const void *)fn, wg, bpc, M->dev.cus, count);
+        HIP_CHECK(hipEventRecord(e[0], s));
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(wg), 0, s, T);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipEventRecord(e[1], s));
+        CheckArgs C{};
+        C.m = M->md; C.items = T.items; C.n_items = count; C.first_round = (r == 0);
+        C.p0 = T.p0; C.v0 = T.v0; C.exit_pos = T.exit_pos; C.exit_v2 = T.exit_v2;
+        C.e0 = B.e0.as<double>(); C.chi = T.chi; C.ergy = B.ergy.as<double>(); C.status = T.status;
+        C.tries = T.tries; C.maxtries = T.maxtries; C.steps = T.steps;
+        C.next_items = B.items[cur ^ 1].as<int32_t>(); C.next_count = ctr + 2;
+        hipLaunchKernelGGL(k_check, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, C);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipEventRecord(e[2], s));
+        int32_t next = 0;
+        HIP_CHECK(hipMemcpyAsync(&next, ctr + 2, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        float ms = 0, ms2 = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, e[0], e[1]));
+        HIP_CHECK(hipEventElapsedTime(&ms2, e[1], e[2]));
+        R.traj_ms[r] = ms;
+        R.items[r] = count;
+        R.check_ms += ms2;
+        R.rounds = r + 1;
+        R.retries += next;
+        count = next;
+        cur ^= 1;
+    }
+    for (auto &x : e) (void)hipEventDestroy(x);
+    unsigned long long ev[2 * COSIMS_MAX_ROUNDS];
+    HIP_CHECK(hipMemcpy(ev, B.evals.p, sizeof(ev), hipMemcpyDeviceToHost));
+    for (int r = 0; r < R.rounds; r++) {
+        R.steps[r] = (int64_t)ev[2 * r];
+        R.evals[r] = (int64_t)ev[2 * r + 1];
+    }
+    if (count > 0) rc = fail(COSIMS_E_HIP, "attempt rounds exceeded the reference's retry bound");
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cosims_sample(const cosims_mol *M, uint64_t seed, uint32_t job, uint32_t iteration, int64_t first, int64_t n,
+                  double *p0, double *v0, double *g, int32_t *hit)
+{
+    if (!M || n < 0 || first < 0 || first + n > (int64_t)UINT32_MAX) return fail(COSIMS_E_INVALID, "cosims_sample: bad range");
+    if (n == 0) return 0;
+    if (int rc = ensure_device(M)) return rc;
+    Stream st;
+    HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+    DBuf dp, dv, dg, dh;
+    HIP_CHECK(dp.alloc(sizeof(double) * 3 * n));
+    HIP_CHECK(dv.alloc(sizeof(double) * 3 * n));
+    HIP_CHECK(dg.alloc(sizeof(double) * n));
+    HIP_CHECK(dh.alloc(sizeof(int32_t) * n));
+    SampleArgs A = sample_args(M, seed, job);
+    A.iteration0 = iteration; A.first = first; A.n = n; A.per_iter = 0;
+    A.p0 = dp.as<double>(); A.v0 = dv.as<double>(); A.g = dg.as<double>(); A.hit = dh.as<int32_t>();
+    hipLaunchKernelGGL(k_sample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st.s, A);
+    HIP_CHECK(hipGetLastError());
+    if (p0) HIP_CHECK(hipMemcpyAsync(p0, dp.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st.s));
+    if (v0) HIP_CHECK(hipMemcpyAsync(v0, dv.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st.s));
+    if (g) HIP_CHECK(hipMemcpyAsync(g, dg.p, sizeof(double) * n, hipMemcpyDeviceToHost, st.s));
+    if (hit) HIP_CHECK(hipMemcpyAsync(hit, dh.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st.s));
+    HIP_CHECK(hipStreamSynchronize(st.s));
+    return 0;
+}
+
+int cosims_verlet(const cosims_mol *M, int32_t prec, int64_t n, const double *p0, const double *v0, double *chi,
+                  int32_t *ok, int32_t *steps, double *ergy, int64_t *total_steps)
+{
+    if (!M || n < 0 || (n > 0 && (!p0 || !v0 || !chi || !ok || !steps || !ergy)))
+        return fail(COSIMS_E_INVALID, "cosims_verlet: bad arguments");
+    if (prec != COSIMS_PREC_FP64 && prec != COSIMS_PREC_MIXED) return fail(COSIMS_E_INVALID, "unknown precision");
+    if (n > INT32_MAX / 4) return fail(COSIMS_E_INVALID, "cosims_verlet: batch too large");
+    if (n == 0) return 0;
+    if (int rc = ensure_device(M)) return rc;
+    Stream st;
+    HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+    TrajBufs B;
+    int rc = B.alloc(n);
+    if (rc) return rc;
+    HIP_CHECK(hipMemcpyAsync(B.p0.p, p0, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st.s));
+    HIP_CHECK(hipMemcpyAsync(B.v0.p, v0, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st.s));
+    std::vector<int32_t> idx(n);
+    for (int64_t i = 0; i < n; i++) idx[i] = (int32_t)i;
+    HIP_CHECK(hipMemcpyAsync(B.items[0].p, idx.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st.s));
+    int32_t cnt[3] = {(int32_t)n, 0, 0};
+    HIP_CHECK(hipMemcpyAsync(B.counters.p, cnt, sizeof(cnt), hipMemcpyHostToDevice, st.s));
+    std::vector<int8_t> s8(n, ST_PENDING);
+    HIP_CHECK(hipMemcpyAsync(B.status.p, s8.data(), n, hipMemcpyHostToDevice, st.s));
+    RoundStats R;
+    rc = run_rounds(M, prec, n, B, st.s, 256, 0, R);
+    if (rc) return rc;
+    std::vector<int32_t> tot(n);
+    HIP_CHECK(hipMemcpyAsync(chi, B.chi.p, sizeof(double) * n, hipMemcpyDeviceToHost, st.s));
+    HIP_CHECK(hipMemcpyAsync(s8.data(), B.status.p, n, hipMemcpyDeviceToHost, st.s));
+    HIP_CHECK(hipMemcpyAsync(steps, B.steps.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st.s));
+    HIP_CHECK(hipMemcpyAsync(ergy, B.ergy.p, sizeof(double) * n, hipMemcpyDeviceToHost, st.s));
+    HIP_CHECK(hipMemcpyAsync(tot.data(), B.tot.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st.s));
+    HIP_CHECK(hipStreamSynchronize(st.s));
+    for (int64_t i = 0; i < n; i++) {
+        ok[i] = s8[i] == ST_OK ? 1 : 0;
+        if (!ok[i]) ergy[i] = 0;
+        if (total_steps) total_steps[i] = tot[i];
+    }
+    return 0;
+}
+
+int cosims_run(const cosims_mol *M, const cosims_params *P, const cosims_run_cfg *cfg, double *omega,
+               double *omega_err, cosims_ccs *out)
+{
+    if (!M || !P || !omega || !omega_err || !out) return fail(COSIMS_E_INVALID, "cosims_run: null argument");
+    cosims_run_cfg c{};
+    if (cfg) c = *cfg;
+    const int prec = c.prec;
+    if (prec != COSIMS_PREC_FP64 && prec != COSIMS_PREC_MIXED) return fail(COSIMS_E_INVALID, "unknown precision");
+    const int iters = c.iter > 0 ? c.iter : P->iter;
+    const int64_t T = c.traj > 0 ? c.traj : P->traj;
+    if (iters <= 0 || T <= 0) return fail(COSIMS_E_INVALID, "cosims_run: no trajectories");
+    const int64_t n = (int64_t)iters * T;
+    if (n > INT32_MAX / 4) return fail(COSIMS_E_INVALID, "cosims_run: too many trajectories for one batch");
+    const int wg = c.wg_size > 0 ? c.wg_size : 256;
+    if (wg % 64 || wg > 256) return fail(COSIMS_E_INVALID, "wg_size must be a multiple of 64, at most 256");
+    std::memset(out, 0, sizeof(*out));
+    if (int rc = ensure_device(M)) return rc;
+    auto t0 = std::chrono::steady_clock::now();
+    Stream st;
+    HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+    TrajBufs B;
+    int rc = B.alloc(n);
+    if (rc) return rc;
+    HIP_CHECK(hipMemsetAsync(B.counters.p, 0, sizeof(int32_t) * 64, st.s));
+    SampleArgs A = sample_args(M, P->seed, c.job);
+    A.first = 0; A.n = n; A.per_iter = T;
+    A.p0 = B.p0.as<double>(); A.v0 = B.v0.as<double>(); A.g = B.g.as<double>();
+    A.status = B.status.as<int8_t>(); A.items = B.items[0].as<int32_t>(); A.n_items = B.counters.as<int32_t>();
+    hipEvent_t es0, es1;
+    HIP_CHECK(hipEventCreate(&es0));
+    HIP_CHECK(hipEventCreate(&es1));
+    HIP_CHECK(hipEventRecord(es0, st.s));
+    hipLaunchKernelGGL(k_sample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st.s, A);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(es1, st.s));
+    RoundStats R;
+    rc = run_rounds(M, prec, n, B, st.s, wg, c.blocks_per_cu, R);
+    if (rc) return rc;
+    float sms = 0;
+    HIP_CHECK(hipEventElapsedTime(&sms, es0, es1));
+    (void)hipEventDestroy(es0);
+    (void)hipEventDestroy(es1);
+    DBuf red;
+    HIP_CHECK(red.alloc(sizeof(double) * 6 * iters));
+    ReduceArgs RA{};
+    RA.status = B.status.as<int8_t>(); RA.chi = B.chi.as<double>(); RA.ergy = B.ergy.as<double>();
+    RA.totsteps = B.tot.as<int32_t>(); RA.per_iter = T; RA.ccs_const = M->prep.ccs_const; RA.out = red.as<double>();
+    hipLaunchKernelGGL(k_reduce, dim3(iters), dim3(256), 0, st.s, RA);
+    HIP_CHECK(hipGetLastError());
+    std::vector<double> sums(6 * iters);
+    std::vector<int8_t> status(n);
+    HIP_CHECK(hipMemcpyAsync(sums.data(), red.p, sizeof(double) * 6 * iters, hipMemcpyDeviceToHost, st.s));
+    HIP_CHECK(hipMemcpyAsync(status.data(), B.status.p, n, hipMemcpyDeviceToHost, st.s));
+    if (c.chi) HIP_CHECK(hipMemcpyAsync(c.chi, B.chi.p, sizeof(double) * n, hipMemcpyDeviceToHost, st.s));
+    if (c.g) HIP_CHECK(hipMemcpyAsync(c.g, B.g.p, sizeof(double) * n, hipMemcpyDeviceToHost, st.s));
+    HIP_CHECK(hipStreamSynchronize(st.s));
+    // Denominator currSuccTraj of one OpenMP thread (src/monteTraj.cpp:456-521):
+    // failures (a miss repeats the previous success flag) counted since the
+    // last progress update every ceil(print_rate) trajectories.
+    const int64_t upd = P->print_rate > 0 ? P->print_rate : std::max<int64_t>(1, (int64_t)(T / 5.0));
+    double mx = 0;
+    for (int k = 0; k < iters; k++) {
+        const int8_t *s = status.data() + (int64_t)k * T;
+        bool succ = false;
+        int64_t orbit = 0;
+        for (int64_t i = 0; i < T; i++) {
+            if (s[i] != ST_MISS) succ = (s[i] == ST_OK);
+            if (!succ) orbit++;
+            if ((i + 1) % upd == 0) orbit = 0;
+        }
+        const double d = (double)(T - orbit);
+        const double tot = sums[6 * k], tot2 = sums[6 * k + 1];
+        omega[k] = tot / d;
+        omega_err[k] = std::sqrt((tot2 / d - (tot / d) * (tot / d)) / d);
+        mx = std::max(mx, sums[6 * k + 2]);
+        out->total_steps += (int64_t)sums[6 * k + 3];
+        out->n_hit += (int64_t)sums[6 * k + 4];
+        out->n_fail += (int64_t)sums[6 * k + 5];
+    }
+    if (c.status)
+        for (int64_t i = 0; i < n; i++)
+            c.status[i] = status[i] == ST_OK ? 1 : (status[i] == ST_MISS ? 0 : 2);
+    // final statistics (src/monteTraj.cpp:412-425)
+    double avg = 0, sd = 0, we = 0, wm = 0;
+    for (int k = 0; k < iters; k++) avg += omega[k];
+    avg = avg / (double)iters;
+    for (int k = 0; k < iters; k++) {
+        sd += (omega[k] - avg) * (omega[k] - avg);
+        we += 1.0 / (omega_err[k] * omega_err[k]);
+        wm += omega[k] / (omega_err[k] * omega_err[k]);
+    }
+    wm /= we;
+    out->mean = avg;
+    out->sd = std::sqrt(sd / (double)iters);
+    out->wmean = wm;
+    out->werr = 1.0 / std::sqrt(we);
+    out->max_ergy = mx;
+    out->n_traj = n;
+    out->n_retry = R.retries;
+    out->n_rounds = R.rounds;
+    for (int r = 0; r < R.rounds; r++) {
+        out->traj_kernel_ms[r] = R.traj_ms[r];
+        out->round_items[r] = R.items[r];
+        out->round_steps[r] = R.steps[r];
+        out->round_evals[r] = R.evals[r];
+    }
+    out->check_kernel_ms = R.check_ms;
+    out->sample_kernel_ms = sms;
+    const int ft = M->prep.force_type;
+    const int64_t npair = (ft == 4 || ft == 6) ? M->prep.n : 0;
+    out->pair_evals = npair * (out->total_steps + out->n_hit + out->n_retry);
+    out->pot_evals = (int64_t)M->prep.n * (2 * out->n_hit + out->n_retry);
+    out->run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+// prep.hpp -- host-side molecule setup of the trajectory engine.
+//
+// Everything the persistent trajectory kernel reads is produced here, in
+// IEEE double with the reference's operand order (the file is compiled with
+// -ffp-contract=off), so that a prepared molecule is bit-identical to what
+// the reference's molecule2 / impactParam / PCM_Cluster / forceAccel leave
+// in memory.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace cosims {
+
+constexpr double kPi = 3.14159265358979323846;    // src/vector3D.h:250
+constexpr double kPi2 = 1.57079632679489661923;   // src/vector3D.h:251
+constexpr double kBoltz = 0.831446214563;         // src/maxBoltz.h:29
+constexpr double kHeMass = 4.002602;              // src/molecule2.h:72
+constexpr double kAlphaKe = 28276.01363;          // src/forceAccel.cpp:35
+
+struct TypeRow {
+    std::string name, symbol;
+    double mass, sigma, eps;
+    int intmass;
+};
+
+std::vector<TypeRow> default_types();                                    // assignDefaultFF
+bool apply_atoms_file(std::vector<TypeRow> &t, const std::string &path); // importFile
+
+struct MolFile {
+    std::vector<double> xyz, charge;
+    std::vector<std::string> symbols;
+    bool charged = false;
+    double total_charge = 0;
+    std::string name;
+};
+bool read_mfj(const std::string &path, const std::vector<TypeRow> &t, MolFile &out, std::string &err);
+bool read_pdb(const std::string &path, MolFile &out, std::string &err);
+
+struct Params {
+    double temp = 298, dt = 0.01;
+    int64_t traj = 50000;
+    int iter = 10;
+    bool charge = false;
+    uint64_t seed = 0;
+    bool multipole = true, dispersion_cutoff = true;
+    double dispersion_radius = 0, multipole_radius = 0;
+    int multipole_order = 1, max_cluster_size = 4;
+    bool projection = true;
+    int print_rate = 20000;
+    double total_charge = 0;
+    int threads = 0;
+};
+
+struct VelDist {
+    double a = 0, mean = 0, mode = 0, sigma = 0, gmin = 0, gmax = 0, norm = 0, max_val = 0,
+           cdf_frac = 0, const1 = 0, min_kin = 0;
+};
+VelDist velocity_distribution(double mu, double temp);
+double veldist_pdf(const VelDist &v, double g);
+
+// Read-only data of the force evaluator (forceAccel after setMol/setClusters).
+struct Prepared {
+    int n = 0;
+    std::vector<double> xyz, q, sig6, eps24m, eps4, mass;
+    double mu = 0, total_mass = 0, dipole_scalar = 0;
+    bool charged = false;
+    int force_type = 6, multipole_order = 1;
+    bool projection = true;
+    double dispersion_radius = 0, multipole_radius = 0;
+    double rot[9] = {0}, max_xyz[3] = {0};
+    double a = 0, b = 0, c = 0, fit_a = 0, fit_b = 0, fit_c = 0, fit_rmsd = 0, enlarge = 0;
+    int n_boundary = 0;
+    VelDist vd;
+    double bmax = 0, ccs_const = 0;
+    // clusters (reference order)
+    int ncl = 0;
+    std::vector<int32_t> cl_start, cl_atoms, cl_replace;
+    std::vector<double> cl_center, cl_maxdist, cl_pos, cl_q, cl_dip, cl_quad;
+};
+
+int force_type_of(bool charged, bool multipole, bool cutoff);
+
+// Full setup from raw atoms.  Returns 0, or a negative engine error code with
+// a message in err.
+int prepare(const Params &p, int n, const double *xyz, const double *mass, const double *sigma,
+            const double *eps, const double *charge, double sigma_max, Prepared &out, std::string &err);
+
+// Completes an injected prepared molecule (cluster moments, velocity window,
+// CCS constant).
+int complete_prepared(const Params &p, Prepared &m, std::string &err);
+
+// getPotential on the host (boundary search).
+double potential_host(const Prepared &m, const double pos[3]);
+
+}  // namespace cosims

@@ -1,0 +1,324 @@
+"""ctypes binding of libcosims_engine.so (the C ABI in include/cosims_engine.h).
+
+There is no fallback: if the HIP library is missing or no gfx950 device is
+present the calls raise.  Arrays are numpy float64 / int32.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcosims_engine.so")
+_LIB = None
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+MAX_ROUNDS = 4
+
+EXPORTS = [
+    "cosims_params_default", "cosims_params_read", "cosims_molfile_read", "cosims_molfile_free",
+    "cosims_mol_create", "cosims_mol_create_prepared", "cosims_mol_destroy", "cosims_mol_upload", "cosims_mol_info_get",
+    "cosims_mol_atoms", "cosims_mol_clusters", "cosims_accel", "cosims_potential", "cosims_sample",
+    "cosims_verlet", "cosims_run", "cosims_strerror", "cosims_last_error", "cosims_device_info",
+    "cosims_device_sync",
+]
+
+
+class CosimsError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [("temp", C.c_double), ("dt", C.c_double), ("traj", C.c_int64), ("iter", C.c_int32),
+                ("charge", C.c_int32), ("seed", C.c_uint64), ("multipole", C.c_int32),
+                ("dispersion_cutoff", C.c_int32), ("dispersion_radius", C.c_double),
+                ("multipole_radius", C.c_double), ("multipole_order", C.c_int32),
+                ("max_cluster_size", C.c_int32), ("projection", C.c_int32), ("print_rate", C.c_int32),
+                ("total_charge", C.c_double), ("threads", C.c_int32), ("name", C.c_char * 128)]
+
+
+class MolFile(C.Structure):
+    _fields_ = [("n", C.c_int32), ("xyz", C.POINTER(C.c_double)), ("mass", C.POINTER(C.c_double)),
+                ("sigma", C.POINTER(C.c_double)), ("epsilon", C.POINTER(C.c_double)),
+                ("charge", C.POINTER(C.c_double)), ("charged", C.c_int32), ("total_charge", C.c_double),
+                ("sigma_max", C.c_double), ("name", C.c_char * 128)]
+
+
+class Atoms(C.Structure):
+    _fields_ = [("n", C.c_int32), ("xyz", C.c_void_p), ("mass", C.c_void_p), ("sigma", C.c_void_p),
+                ("epsilon", C.c_void_p), ("charge", C.c_void_p), ("sigma_max", C.c_double)]
+
+
+class Prepared(C.Structure):
+    _fields_ = [("n", C.c_int32), ("xyz", C.c_void_p), ("charge", C.c_void_p), ("sig6", C.c_void_p),
+                ("eps24m", C.c_void_p), ("eps4", C.c_void_p), ("mu", C.c_double), ("a", C.c_double),
+                ("b", C.c_double), ("c", C.c_double), ("ncl", C.c_int32), ("cl_start", C.c_void_p),
+                ("cl_atoms", C.c_void_p), ("cl_center", C.c_void_p), ("cl_maxdist", C.c_void_p),
+                ("dispersion_radius", C.c_double), ("multipole_radius", C.c_double)]
+
+
+class MolInfo(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in ("n", "ncl", "force_type", "charged", "projection",
+                                          "multipole_order")] + \
+               [(k, C.c_double) for k in ("mu", "total_mass", "dipole_scalar", "a", "b", "c", "bmax",
+                                           "enlarge", "gmin", "gmax", "gmean", "gmode", "gsigma", "vd_a",
+                                           "vd_norm", "max_val", "cdf_frac", "ccs_const", "min_kin",
+                                           "dispersion_radius", "multipole_radius")] + \
+               [("rot", C.c_double * 9), ("max_xyz", C.c_double * 3), ("fit_abc", C.c_double * 3),
+                ("fit_rmsd", C.c_double), ("n_boundary", C.c_int32), ("setup_ms", C.c_double)]
+
+
+class RunCfg(C.Structure):
+    _fields_ = [("prec", C.c_int32), ("job", C.c_uint32), ("iter", C.c_int32), ("traj", C.c_int64),
+                ("wg_size", C.c_int32), ("blocks_per_cu", C.c_int32), ("chi", C.c_void_p),
+                ("status", C.c_void_p), ("g", C.c_void_p)]
+
+
+class Ccs(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("mean", "sd", "wmean", "werr", "max_ergy")] + \
+               [(k, C.c_int64) for k in ("n_traj", "n_hit", "n_fail", "n_retry", "total_steps",
+                                          "pair_evals", "pot_evals")] + \
+               [("run_ms", C.c_double), ("n_rounds", C.c_int32),
+                ("traj_kernel_ms", C.c_double * MAX_ROUNDS), ("round_steps", C.c_int64 * MAX_ROUNDS),
+                ("round_evals", C.c_int64 * MAX_ROUNDS), ("round_items", C.c_int64 * MAX_ROUNDS), ("check_kernel_ms", C.c_double),
+                ("sample_kernel_ms", C.c_double)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise CosimsError(f"{LIB_PATH} is not built; run `python -m cosims_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.cosims_params_default.argtypes = [C.POINTER(Params)]
+        L.cosims_params_read.argtypes = [C.c_char_p, C.POINTER(Params)]
+        L.cosims_molfile_read.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.POINTER(MolFile))]
+        L.cosims_molfile_free.argtypes = [C.POINTER(MolFile)]
+        L.cosims_mol_create.argtypes = [C.POINTER(Params), C.POINTER(Atoms), C.POINTER(vp)]
+        L.cosims_mol_create_prepared.argtypes = [C.POINTER(Params), C.POINTER(Prepared), C.POINTER(vp)]
+        L.cosims_mol_destroy.argtypes = [vp]
+        L.cosims_mol_destroy.restype = None
+        L.cosims_mol_upload.argtypes = [vp]
+        L.cosims_mol_info_get.argtypes = [vp, C.POINTER(MolInfo)]
+        L.cosims_mol_atoms.argtypes = [vp] + [vp] * 5
+        L.cosims_mol_clusters.argtypes = [vp] + [vp] * 9
+        L.cosims_accel.argtypes = [vp, C.c_int32, C.c_int64, vp, vp]
+        L.cosims_potential.argtypes = [vp, C.c_int64, vp, vp]
+        L.cosims_sample.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, C.c_int64,
+                                    vp, vp, vp, vp]
+        L.cosims_verlet.argtypes = [vp, C.c_int32, C.c_int64, vp, vp, vp, vp, vp, vp, vp]
+        L.cosims_run.argtypes = [vp, C.POINTER(Params), C.POINTER(RunCfg), vp, vp, C.POINTER(Ccs)]
+        L.cosims_strerror.restype = C.c_char_p
+        L.cosims_strerror.argtypes = [C.c_int]
+        L.cosims_last_error.restype = C.c_char_p
+        L.cosims_device_info.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_int32)]
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != 0:
+        L = lib()
+        raise CosimsError(f"{L.cosims_strerror(rc).decode()} ({rc}): {L.cosims_last_error().decode()}")
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().cosims_params_default(C.byref(p))
+    for k, v in kw.items():
+        if k == "name":
+            v = v.encode()
+        setattr(p, k, v)
+    return p
+
+
+def read_params(path, base: Params | None = None) -> Params:
+    p = base if base is not None else default_params()
+    _check(lib().cosims_params_read(path.encode(), C.byref(p)))
+    return p
+
+
+def read_molfile(mol_path, atoms_path=None) -> dict:
+    ptr = C.POINTER(MolFile)()
+    _check(lib().cosims_molfile_read(mol_path.encode(), atoms_path.encode() if atoms_path else None,
+                                     C.byref(ptr)))
+    f = ptr.contents
+    n = f.n
+    out = dict(n=n,
+               xyz=np.ctypeslib.as_array(f.xyz, (3 * n,)).copy().reshape(n, 3),
+               mass=np.ctypeslib.as_array(f.mass, (n,)).copy(),
+               sigma=np.ctypeslib.as_array(f.sigma, (n,)).copy(),
+               epsilon=np.ctypeslib.as_array(f.epsilon, (n,)).copy(),
+               charge=np.ctypeslib.as_array(f.charge, (n,)).copy(),
+               charged=bool(f.charged), total_charge=f.total_charge, sigma_max=f.sigma_max,
+               name=f.name.decode())
+    lib().cosims_molfile_free(ptr)
+    return out
+
+
+def device_sync():
+    _check(lib().cosims_device_sync())
+
+
+def device_info():
+    buf = C.create_string_buffer(256)
+    cus = C.c_int32()
+    _check(lib().cosims_device_info(buf, 256, C.byref(cus)))
+    return buf.value.decode(), cus.value
+
+
+class Molecule:
+    """A prepared molecule resident on the GPU (one cosims_mol handle)."""
+
+    def __init__(self, handle, params: Params):
+        self._h = handle
+        self.params = params
+
+    @classmethod
+    def create(cls, params: Params, xyz, mass, sigma, epsilon, charge=None, sigma_max=3.5):
+        xyz, mass, sigma, epsilon = (_f64(a) for a in (xyz, mass, sigma, epsilon))
+        charge = None if charge is None else _f64(charge)
+        a = Atoms(n=mass.size, xyz=xyz.ctypes.data, mass=mass.ctypes.data, sigma=sigma.ctypes.data,
+                  epsilon=epsilon.ctypes.data, charge=_p(charge), sigma_max=sigma_max)
+        h = C.c_void_p()
+        _check(lib().cosims_mol_create(C.byref(params), C.byref(a), C.byref(h)))
+        return cls(h, params)
+
+    @classmethod
+    def from_file(cls, mol_path, atoms_path=None, params: Params | None = None, **kw):
+        f = read_molfile(mol_path, atoms_path)
+        if params is None:
+            params = default_params()
+        params.charge = int(f["charged"])
+        params.total_charge = f["total_charge"]
+        for k, v in kw.items():
+            setattr(params, k, v)
+        return cls.create(params, f["xyz"], f["mass"], f["sigma"], f["epsilon"],
+                          f["charge"] if f["charged"] else None, f["sigma_max"])
+
+    @classmethod
+    def from_prepared(cls, params: Params, xyz, charge, sig6, eps24m, eps4, mu, abc, clusters=None,
+                      dispersion_radius=0.0, multipole_radius=0.0):
+        arrs = [_f64(a) for a in (xyz, charge, sig6, eps24m, eps4)]
+        s = Prepared(n=arrs[1].size, xyz=arrs[0].ctypes.data, charge=arrs[1].ctypes.data,
+                     sig6=arrs[2].ctypes.data, eps24m=arrs[3].ctypes.data, eps4=arrs[4].ctypes.data,
+                     mu=mu, a=abc[0], b=abc[1], c=abc[2], dispersion_radius=dispersion_radius,
+                     multipole_radius=multipole_radius)
+        keep = [arrs]
+        if clusters is not None:
+            st = np.ascontiguousarray(clusters["start"], dtype=np.int32)
+            at = np.ascontiguousarray(clusters["atoms"], dtype=np.int32)
+            ce = _f64(clusters["center"]); md = _f64(clusters["maxdist"])
+            keep.append((st, at, ce, md))
+            s.ncl = md.size
+            s.cl_start, s.cl_atoms, s.cl_center, s.cl_maxdist = (st.ctypes.data, at.ctypes.data,
+                                                                 ce.ctypes.data, md.ctypes.data)
+        h = C.c_void_p()
+        _check(lib().cosims_mol_create_prepared(C.byref(params), C.byref(s), C.byref(h)))
+        return cls(h, params)
+
+    def upload(self):
+        _check(lib().cosims_mol_upload(self._h))
+
+    def close(self):
+        if self._h:
+            lib().cosims_mol_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def info(self) -> dict:
+        i = MolInfo()
+        _check(lib().cosims_mol_info_get(self._h, C.byref(i)))
+        out = {}
+        for k, _ in MolInfo._fields_:
+            v = getattr(i, k)
+            out[k] = list(v) if isinstance(v, C.Array) else v
+        return out
+
+    def atoms(self) -> dict:
+        n = self.info()["n"]
+        xyz = np.empty((n, 3)); q, s6, e24, e4 = (np.empty(n) for _ in range(4))
+        _check(lib().cosims_mol_atoms(self._h, _p(xyz), _p(q), _p(s6), _p(e24), _p(e4)))
+        return dict(xyz=xyz, q=q, sig6=s6, eps24m=e24, eps4=e4)
+
+    def clusters(self) -> dict | None:
+        inf = self.info()
+        n, c = inf["n"], inf["ncl"]
+        if c == 0:
+            return None
+        st = np.empty(c + 1, np.int32); at = np.empty(n, np.int32); ce = np.empty(3 * c)
+        md = np.empty(c); pos = np.empty(3 * c); q = np.empty(c); dip = np.empty(3 * c)
+        quad = np.empty(9 * c); rep = np.empty(c, np.int32)
+        _check(lib().cosims_mol_clusters(self._h, *[_p(a) for a in (st, at, ce, md, pos, q, dip, quad, rep)]))
+        return dict(start=st, atoms=at, center=ce, maxdist=md, pos=pos, q=q, dip=dip, quad=quad,
+                    replace=rep)
+
+    def accel(self, pos, prec=PREC_FP64):
+        pos = _f64(pos).reshape(-1, 3)
+        out = np.empty_like(pos)
+        _check(lib().cosims_accel(self._h, prec, pos.shape[0], _p(pos), _p(out)))
+        return out
+
+    def potential(self, pos):
+        pos = _f64(pos).reshape(-1, 3)
+        out = np.empty(pos.shape[0])
+        _check(lib().cosims_potential(self._h, pos.shape[0], _p(pos), _p(out)))
+        return out
+
+    def sample(self, seed, job, iteration, first, n):
+        p0 = np.empty((n, 3)); v0 = np.empty((n, 3)); g = np.empty(n); hit = np.empty(n, np.int32)
+        _check(lib().cosims_sample(self._h, seed, job, iteration, first, n, _p(p0), _p(v0), _p(g), _p(hit)))
+        return dict(p0=p0, v0=v0, g=g, hit=hit)
+
+    def verlet(self, p0, v0, prec=PREC_FP64):
+        p0 = _f64(p0).reshape(-1, 3); v0 = _f64(v0).reshape(-1, 3)
+        n = p0.shape[0]
+        chi = np.empty(n); ergy = np.empty(n); ok = np.empty(n, np.int32); steps = np.empty(n, np.int32)
+        tot = np.empty(n, np.int64)
+        _check(lib().cosims_verlet(self._h, prec, n, _p(p0), _p(v0), _p(chi), _p(ok), _p(steps), _p(ergy),
+                                   _p(tot)))
+        return dict(chi=chi, ok=ok, steps=steps, ergy=ergy, total_steps=tot)
+
+    def run(self, prec=PREC_MIXED, job=0, iters=0, traj=0, keep=False, params: Params | None = None,
+            wg_size=0, blocks_per_cu=0):
+        p = params if params is not None else self.params
+        it = iters or p.iter
+        T = traj or p.traj
+        cfg = RunCfg(prec=prec, job=job, iter=it, traj=T, wg_size=wg_size, blocks_per_cu=blocks_per_cu)
+        chi = status = g = None
+        if keep:
+            chi = np.empty(it * T); status = np.empty(it * T, np.int32); g = np.empty(it * T)
+            cfg.chi, cfg.status, cfg.g = _p(chi), _p(status), _p(g)
+        om = np.empty(it); err = np.empty(it)
+        res = Ccs()
+        _check(lib().cosims_run(self._h, C.byref(p), C.byref(cfg), _p(om), _p(err), C.byref(res)))
+        out = {k: getattr(res, k) for k, _ in Ccs._fields_}
+        for k in ("traj_kernel_ms", "round_steps", "round_evals", "round_items"):
+            out[k] = list(out[k])[: res.n_rounds]
+        out.update(omega=om, omega_err=err)
+        if keep:
+            out.update(chi=chi, status=status, g=g)
+        return out

@@ -1,0 +1,216 @@
+/*
+ * cosims_engine.h -- C ABI of the MI355X (gfx950) He trajectory engine.
+ *
+ * Drop-in boundary for the CoSIMS trajectory path (ChristopherAMyers/CoSIMS).
+ * The reference has no FFI; its boundary is the in-process C++ surface
+ * listed beside each entry point below.  Conventions:
+ *   - plain pointers and sizes; caller-owned inputs are copied, outputs are
+ *     written to caller arrays; the engine owns its device buffers;
+ *   - every call returns 0 on success and a negative COSIMS_E_* code on
+ *     failure; cosims_strerror() gives text, cosims_last_error() details;
+ *   - a cosims_mol handle is immutable after creation; calls on one handle
+ *     may run concurrently (each call uses its own HIP stream);
+ *   - units are the reference's: A, ps, u, e; energies in u A^2 ps^-2.
+ */
+#ifndef COSIMS_ENGINE_H
+#define COSIMS_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define COSIMS_OK 0
+#define COSIMS_E_INVALID (-1)     /* bad argument */
+#define COSIMS_E_HIP (-2)         /* HIP runtime / device error */
+#define COSIMS_E_NOMEM (-3)       /* host or device allocation failed */
+#define COSIMS_E_SETUP (-4)       /* molecule setup failed (see last_error) */
+#define COSIMS_E_IO (-5)          /* file could not be read */
+#define COSIMS_E_UNSUPPORTED (-6) /* input the reference handles only by undefined behaviour */
+#define COSIMS_E_NODEVICE (-7)    /* no usable gfx950 device */
+
+#define COSIMS_PREC_FP64 0  /* all-FP64 force sum, reference expression order */
+#define COSIMS_PREC_MIXED 1 /* FP32 force sum, FP64 Verlet / energy check / CCS */
+
+/* Run parameters: the keys of the reference input file
+ * (fileParams::setDefaultParams / importParams, src/fileParams.cpp:203-510;
+ * globals src/myStructs.h:28-74).  `traj` is a raw count here (the input
+ * file gives it in thousands). */
+typedef struct cosims_params {
+    double temp;              /* temp_ext (K), default 298 */
+    double dt;                /* dt_ext (ps), default 0.01 */
+    int64_t traj;             /* trajNum_ext, default 50000 */
+    int32_t iter;             /* iterations_ext, default 10 */
+    int32_t charge;           /* charge_ext: polarisation terms on */
+    uint64_t seed;            /* seed_ext */
+    int32_t multipole;        /* multipole_ext, default 1 */
+    int32_t dispersion_cutoff;/* dispersionCutoff_ext, default 1 */
+    double dispersion_radius; /* 0 => sigma_max * 300^(1/3) (src/molecule2.cpp:881-887) */
+    double multipole_radius;  /* 0 => 20 A (src/molecule2.cpp:888-895) */
+    int32_t multipole_order;  /* multipole_order_ext, default 1 */
+    int32_t max_cluster_size; /* maxClusterSize_ext (int), default 4 */
+    int32_t projection;       /* projection_ext, default 1 */
+    int32_t print_rate;       /* printRate_ext; 0 => traj/5 as with an input file */
+    double total_charge;      /* totalCharge_ext (informational) */
+    int32_t threads;          /* host threads for setup; 0 => all */
+    char name[128];           /* name_ext */
+} cosims_params;
+
+/* fileParams::setDefaultParams  src/fileParams.cpp:203-259 */
+void cosims_params_default(cosims_params *p);
+/* fileParams::importParams  src/fileParams.cpp:261-510: applies an input
+ * file on top of *p (unknown keys ignored, last value wins, `traj` x1000,
+ * `charge` switches charge_ext).  print_rate defaults to traj/5. */
+int cosims_params_read(const char *input_path, cosims_params *p);
+
+/* A molecule file read through the reference's readers and type table:
+ * molecule2::assignDefaultFF/importFile/importMFJ/importPDB
+ * (src/molecule2.cpp:43-192, 442-768), types resolved as in
+ * setMolProperties (:782-839). */
+typedef struct cosims_molfile {
+    int32_t n;
+    double *xyz;          /* 3n, file order */
+    double *mass;         /* n */
+    double *sigma;        /* n, A */
+    double *epsilon;      /* n, kcal/mol */
+    double *charge;       /* n */
+    int32_t charged;      /* charge_ext as the reader sets it */
+    double total_charge;  /* totalCharge_ext as the reader sets it */
+    double sigma_max;     /* largest sigma of the whole type table */
+    char name[128];       /* MFJ title line (empty for PDB) */
+} cosims_molfile;
+int cosims_molfile_read(const char *mol_path, const char *atoms_dat_path, cosims_molfile **out);
+void cosims_molfile_free(cosims_molfile *f);
+
+/* Raw atoms for setup (what setMolProperties receives). */
+typedef struct cosims_atoms {
+    int32_t n;
+    const double *xyz;     /* 3n, file order, A */
+    const double *mass;    /* n */
+    const double *sigma;   /* n, A */
+    const double *epsilon; /* n, kcal/mol */
+    const double *charge;  /* n, or NULL for a neutral molecule */
+    double sigma_max;      /* largest sigma of the type table */
+} cosims_atoms;
+
+/* A prepared molecule as the reference leaves it after setup, for injecting
+ * setup results computed elsewhere (parity harnesses). */
+typedef struct cosims_prepared {
+    int32_t n;
+    const double *xyz;     /* 3n rotated, centred coordinates */
+    const double *charge;  /* n */
+    const double *sig6;    /* n, sigma^6 */
+    const double *eps24m;  /* n, 24 eps'/mu (atom 0 unscaled, as the reference) */
+    const double *eps4;    /* n, 4 eps' */
+    double mu;             /* reduced mass */
+    double a, b, c;        /* ellipsoid axes after enlargement */
+    int32_t ncl;           /* PCM clusters (0 when neither cut-off is on) */
+    const int32_t *cl_start;  /* ncl+1 */
+    const int32_t *cl_atoms;  /* n */
+    const double *cl_center;  /* 3 ncl geometric centres */
+    const double *cl_maxdist; /* ncl */
+    double dispersion_radius;
+    double multipole_radius;
+} cosims_prepared;
+
+typedef struct cosims_mol cosims_mol;
+
+/* Setup = molecule2::setMolProperties + setInertia(true), impactParam
+ * setBoundry/findEllipsoidBound/enlargeEllipBoundry, VelocityDistribution,
+ * forceAccel::setClusters (PCM_Cluster + multipoles); replaces
+ * `new monteTraj(mol, shared)` (src/main.cpp:119-138, src/monteTraj.cpp:26-142). */
+int cosims_mol_create(const cosims_params *p, const cosims_atoms *atoms, cosims_mol **out);
+/* Same, from injected setup results. */
+int cosims_mol_create_prepared(const cosims_params *p, const cosims_prepared *prep, cosims_mol **out);
+void cosims_mol_destroy(cosims_mol *m);
+/* Creation is host-only; device buffers are made on the first GPU call on
+ * the handle (thread-safe, once).  This call does it eagerly. */
+int cosims_mol_upload(const cosims_mol *m);
+
+typedef struct cosims_mol_info {
+    int32_t n, ncl, force_type, charged, projection, multipole_order;
+    double mu, total_mass, dipole_scalar;
+    double a, b, c, bmax, enlarge;
+    double gmin, gmax, gmean, gmode, gsigma, vd_a, vd_norm, max_val, cdf_frac;
+    double ccs_const;          /* const1*const2 (src/monteTraj.cpp:91-94) */
+    double min_kin;            /* boundary energy before the 1% factor */
+    double dispersion_radius, multipole_radius;
+    double rot[9], max_xyz[3];
+    double fit_abc[3], fit_rmsd; /* least-squares axes before enlargement */
+    int32_t n_boundary;
+    double setup_ms;
+} cosims_mol_info;
+int cosims_mol_info_get(const cosims_mol *m, cosims_mol_info *out);
+/* Prepared atoms (n each, xyz 3n) and clusters; any pointer may be NULL. */
+int cosims_mol_atoms(const cosims_mol *m, double *xyz, double *charge, double *sig6, double *eps24m,
+                     double *eps4);
+int cosims_mol_clusters(const cosims_mol *m, int32_t *start, int32_t *atoms, double *center,
+                        double *maxdist, double *pos, double *q, double *dip, double *quad,
+                        int32_t *replace);
+
+/* forceAccel::getReducedVec (src/forceAccel.cpp:887-908) for n positions. */
+int cosims_accel(const cosims_mol *m, int32_t prec, int64_t n, const double *pos, double *acc);
+/* forceAccel::getPotential (src/forceAccel.cpp:943-982), FP64, n positions. */
+int cosims_potential(const cosims_mol *m, int64_t n, const double *pos, double *u);
+
+/* Initial conditions of trajectories first..first+n-1 of (seed, job,
+ * iteration): rejection sampling of monteTraj::setUpTraj
+ * (src/monteTraj.cpp:174-278) driven by Philox4x32-10 instead of
+ * mt19937_64 + rand().  hit=0 marks a line that misses the ellipsoid. */
+int cosims_sample(const cosims_mol *m, uint64_t seed, uint32_t job, uint32_t iteration, int64_t first,
+                  int64_t n, double *p0, double *v0, double *g, int32_t *hit);
+
+/* diffEqSolver::verletVel (src/diffEqSolver.cpp:81-211) for n trajectories,
+ * one GPU lane each.  Outputs: chi (rad), ok (return value), steps
+ * (p.stepCount), ergy (% energy deviation of the accepted attempt),
+ * total_steps (all steps over all attempts; may be NULL). */
+int cosims_verlet(const cosims_mol *m, int32_t prec, int64_t n, const double *p0, const double *v0,
+                  double *chi, int32_t *ok, int32_t *steps, double *ergy, int64_t *total_steps);
+
+/* monteTraj::runTraj (src/monteTraj.cpp:281-454): iter CCS integrals of
+ * traj trajectories, all in one persistent launch per attempt round. */
+typedef struct cosims_run_cfg {
+    int32_t prec;          /* COSIMS_PREC_* */
+    uint32_t job;          /* Philox key extension (request id) */
+    int32_t iter;          /* 0 => params.iter */
+    int64_t traj;          /* 0 => params.traj */
+    int32_t wg_size;       /* 0 => tuned default */
+    int32_t blocks_per_cu; /* 0 => tuned default */
+    /* optional per-trajectory outputs (iter*traj entries each, may be NULL) */
+    double *chi;
+    int32_t *status;       /* 0 miss, 1 success, 2 failure */
+    double *g;
+} cosims_run_cfg;
+
+#define COSIMS_MAX_ROUNDS 4
+typedef struct cosims_ccs {
+    double mean, sd, wmean, werr, max_ergy;   /* src/monteTraj.cpp:412-442 */
+    int64_t n_traj, n_hit, n_fail, n_retry;
+    int64_t total_steps;                      /* integration steps, all attempts */
+    int64_t pair_evals;                       /* force-loop atom pairs */
+    int64_t pot_evals;                        /* getPotential atom pairs */
+    double run_ms;                            /* trajectory phase wall time */
+    int32_t n_rounds;
+    double traj_kernel_ms[COSIMS_MAX_ROUNDS]; /* HIP-event time of each trajectory launch */
+    int64_t round_steps[COSIMS_MAX_ROUNDS];   /* integration steps of each launch */
+    int64_t round_evals[COSIMS_MAX_ROUNDS];   /* force evaluations of each launch */
+    int64_t round_items[COSIMS_MAX_ROUNDS];   /* trajectories queued for each launch */
+    double check_kernel_ms, sample_kernel_ms;
+} cosims_ccs;
+/* omega/omega_err: iter entries each. */
+int cosims_run(const cosims_mol *m, const cosims_params *p, const cosims_run_cfg *cfg, double *omega,
+               double *omega_err, cosims_ccs *out);
+
+const char *cosims_strerror(int code);
+const char *cosims_last_error(void);
+/* Device name and compute-unit count of the current device. */
+int cosims_device_info(char *name, size_t cap, int32_t *cus);
+/* Waits for all work on the current device (benchmark brackets). */
+int cosims_device_sync(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

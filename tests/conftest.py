@@ -1,0 +1,64 @@
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+sys.path.insert(0, ROOT)
+
+CASES = ["acau_charged", "acau_neutral", "acau_phos_sigma_chg", "acau_phos_sigma", "ion128", "ion2000"]
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
+
+
+def golden(name):
+    with open(os.path.join(GOLDEN, name + ".json")) as f:
+        return json.load(f)
+
+
+def golden_clusters(g):
+    """Reference PCM groups + the oracle's multipoles of them."""
+    import oracle
+    A = g["atoms"]
+    sc = g["scalars"]
+    xyz = np.array(A["xyz"])
+    cl = oracle.pcm_clusters(xyz, A["q"], int(sc["max_cluster_size"]))
+    return cl
+
+
+def golden_oracle_mol(g, flags):
+    """Oracle molecule built from the reference's prepared atoms."""
+    import oracle
+    A = g["atoms"]
+    sc = g["scalars"]
+    ch, mp, co, order = flags
+    ft = oracle.lib().or_force_type(ch, mp, co)
+    cl = golden_clusters(g)
+    return oracle.Mol(np.array(A["xyz"]).reshape(-1, 3), A["q"], A["sig6"], A["eps24m"], A["eps4"], ch,
+                      sc["mu"], ft, cl, sc["multipole_radius"], sc["dispersion_radius"], order)
+
+
+def golden_engine_mol(g, flags):
+    """Engine molecule injected with the reference's prepared atoms/clusters."""
+    from cosims_amd import engine as E
+    A = g["atoms"]
+    sc = g["scalars"]
+    ch, mp, co, order = flags
+    p = E.default_params(temp=sc["temp"], dt=sc["dt"], charge=ch, multipole=mp, dispersion_cutoff=co,
+                         multipole_order=order, max_cluster_size=int(sc["max_cluster_size"]))
+    pcm = g["pcm"]
+    clusters = dict(start=pcm["start"], atoms=pcm["atoms"], center=pcm["center"], maxdist=pcm["maxdist"])
+    return E.Molecule.from_prepared(p, np.array(A["xyz"]), A["q"], A["sig6"], A["eps24m"], A["eps4"],
+                                    sc["mu"], (sc["a"], sc["b"], sc["c"]), clusters,
+                                    sc["dispersion_radius"], sc["multipole_radius"])
+
+
+@pytest.fixture(scope="session")
+def oracle_lib():
+    import oracle
+    return oracle.lib()

@@ -1,0 +1,93 @@
+"""Whole CCS runs on the GPU against the oracle (MI355X).
+
+The workload is the 128-atom synthetic ion (C/H/N/O/P 30:50:10:8:2, R = 9 A,
+q ~ N(0, 0.3 e) shifted to +1 e, He at 298 K, exact electrostatics + full LJ)
+at 3 CCS integrals x 20 000 trajectories.  The engine draws the initial
+conditions; the same ones are fed to the oracle.
+
+Tolerances: FP64 GPU per-trajectory median |dchi| <= 1e-8 rad on non-failed
+trajectories and CCS within 0.05 %; mixed precision CCS within 0.1 % of the
+FP64 oracle on the same (paired) initial conditions.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from cosims_amd import engine as E
+from cosims_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+ITERS, T = 3, 20000
+
+
+@pytest.fixture(scope="module")
+def ion128():
+    p = E.default_params(temp=298.0, traj=T, iter=ITERS, multipole=0, dispersion_cutoff=0, seed=20261203)
+    p.print_rate = T // 5
+    m = E.Molecule.from_file(os.path.join(GOLD, "ion128.mfj"), params=p)
+    return m
+
+
+def _oracle_for(m):
+    A = m.atoms()
+    inf = m.info()
+    return oracle.Mol(A["xyz"], A["q"], A["sig6"], A["eps24m"], A["eps4"], inf["charged"], inf["mu"],
+                      inf["force_type"])
+
+
+@pytest.fixture(scope="module")
+def fp64_run(ion128):
+    return ion128.run(E.PREC_FP64, job=0, keep=True)
+
+
+@pytest.fixture(scope="module")
+def oracle_run(ion128):
+    m = ion128
+    inf = m.info()
+    mo = _oracle_for(m)
+    chi = np.zeros(ITERS * T); ok = np.zeros(ITERS * T, np.int32); hit = np.zeros(ITERS * T, np.int32)
+    omega = []; err = []
+    for k in range(ITERS):
+        s = m.sample(m.params.seed, 0, k, 0, T)
+        h = s["hit"] == 1
+        r = mo.verlet(m.params.dt, s["p0"][h], s["v0"][h], (inf["a"], inf["b"], inf["c"]))
+        sl = slice(k * T, (k + 1) * T)
+        c = np.zeros(T); o = np.zeros(T, np.int32)
+        c[h] = r["chi"]; o[h] = r["ok"]
+        chi[sl] = c; ok[sl] = o; hit[sl] = s["hit"]
+        red = oracle.reduce_iteration(c, o, s["hit"], inf["ccs_const"], m.params.print_rate)
+        omega.append(red["omega"]); err.append(red["err"])
+    return dict(chi=chi, ok=ok, hit=hit, omega=np.array(omega), err=np.array(err))
+
+
+def test_fp64_run_matches_oracle(fp64_run, oracle_run):
+    r, o = fp64_run, oracle_run
+    hit = o["hit"] == 1
+    assert np.array_equal(r["status"] != 0, hit)
+    good = hit & (o["ok"] == 1) & (r["status"] == 1)
+    d = np.abs(r["chi"][good] - o["chi"][good])
+    assert np.median(d) <= 1e-8
+    assert np.mean(r["status"][hit] == np.where(o["ok"][hit] == 1, 1, 2)) >= 0.995
+    np.testing.assert_allclose(r["omega"], o["omega"], rtol=5e-4)
+    fin = oracle.reduce_final(o["omega"], o["err"])
+    assert abs(r["wmean"] - fin["wmean"]) / fin["wmean"] <= 5e-4
+
+
+def test_mixed_run_matches_fp64_oracle(ion128, oracle_run):
+    r = ion128.run(E.PREC_MIXED, job=0, keep=True)
+    o = oracle_run
+    assert np.array_equal(r["status"] != 0, o["hit"] == 1)
+    fin = oracle.reduce_final(o["omega"], o["err"])
+    dev = abs(r["wmean"] - fin["wmean"]) / fin["wmean"]
+    assert dev <= 1e-3, dev
+
+
+def test_run_is_deterministic(ion128):
+    a = ion128.run(E.PREC_MIXED, job=5, iters=1, traj=4000, keep=True)
+    b = ion128.run(E.PREC_MIXED, job=5, iters=1, traj=4000, keep=True)
+    assert np.array_equal(a["chi"], b["chi"])
+    assert a["omega"][0] == b["omega"][0]
