@@ -509,6 +509,8 @@ struct Stream {
 
 }  // namespace
 
+int cosims::set_last_error(int code, const std::string &msg) { return fail(code, msg); }
+
 // -------------------------------------------------------------- handle ---
 struct cosims_mol {
     Params params;

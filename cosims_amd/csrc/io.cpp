@@ -38,9 +38,9 @@ extern "C" {
 
 int cosims_params_read(const char *path, cosims_params *p)
 {
-    if (!path || !p) return COSIMS_E_INVALID;
+    if (!path || !p) return cosims::set_last_error(COSIMS_E_INVALID, "cosims_params_read: null argument");
     std::ifstream f(path);
-    if (!f) return COSIMS_E_IO;
+    if (!f) return cosims::set_last_error(COSIMS_E_IO, std::string("cannot open ") + path);
     bool print_rate_set = false;
     std::string line;
     while (std::getline(f, line)) {
@@ -78,19 +78,21 @@ int cosims_params_read(const char *path, cosims_params *p)
 
 int cosims_molfile_read(const char *mol_path, const char *atoms_path, cosims_molfile **out)
 {
-    if (!mol_path || !out) return COSIMS_E_INVALID;
+    if (!mol_path || !out) return cosims::set_last_error(COSIMS_E_INVALID, "cosims_molfile_read: null argument");
     std::vector<cosims::TypeRow> table = cosims::default_types();
-    if (atoms_path && *atoms_path && !cosims::apply_atoms_file(table, atoms_path)) return COSIMS_E_IO;
+    if (atoms_path && *atoms_path && !cosims::apply_atoms_file(table, atoms_path))
+        return cosims::set_last_error(COSIMS_E_IO, std::string("cannot open ") + atoms_path);
     std::string path(mol_path), err;
-    if (path.size() < 4 || path[path.size() - 4] != '.') return COSIMS_E_INVALID;
+    if (path.size() < 4 || path[path.size() - 4] != '.')
+        return cosims::set_last_error(COSIMS_E_INVALID, "molecule file needs a .pdb or .mfj extension");
     std::string ext = path.substr(path.size() - 3);
     std::transform(ext.begin(), ext.end(), ext.begin(), ::tolower);
     cosims::MolFile mf;
     bool ok;
     if (ext == "pdb") ok = cosims::read_pdb(path, mf, err);
     else if (ext == "mfj") ok = cosims::read_mfj(path, table, mf, err);
-    else return COSIMS_E_INVALID;
-    if (!ok) return COSIMS_E_IO;
+    else return cosims::set_last_error(COSIMS_E_INVALID, "molecule file needs a .pdb or .mfj extension");
+    if (!ok) return cosims::set_last_error(COSIMS_E_IO, err);
     const int n = (int)mf.symbols.size();
     auto *f = new cosims_molfile();
     std::memset(f, 0, sizeof(*f));
@@ -108,9 +110,10 @@ int cosims_molfile_read(const char *mol_path, const char *atoms_path, cosims_mol
                 row = &r;
                 break;
             }
-        if (!row) {  // the reference continues with uninitialised parameters
+        if (!row) {  // the reference continues with uninitialised parameters (src/molecule2.cpp:826-835)
             cosims_molfile_free(f);
-            return COSIMS_E_UNSUPPORTED;
+            return cosims::set_last_error(COSIMS_E_UNSUPPORTED, "atom " + std::to_string(i + 1) +
+                                                                    ": type not in the force field");
         }
         f->mass[i] = row->mass;
         f->sigma[i] = row->sigma;

@@ -91,6 +91,10 @@ int prepare(const Params &p, int n, const double *xyz, const double *mass, const
 // CCS constant).
 int complete_prepared(const Params &p, Prepared &m, std::string &err);
 
+// Records the message returned by cosims_last_error() on this thread and
+// returns code (defined in engine.hip).
+int set_last_error(int code, const std::string &msg);
+
 // getPotential on the host (boundary search).
 double potential_host(const Prepared &m, const double pos[3]);
 

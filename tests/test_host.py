@@ -1,0 +1,136 @@
+"""Engine host logic on the CPU: the library loads and exports the C ABI,
+its readers and input-file parser, and its molecule setup (no GPU calls)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import molio
+from cosims_amd import engine as E
+
+from conftest import CASES, GOLDEN, ROOT, golden
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "cosims_engine.h")).read()
+    declared = set(re.findall(r"\b(cosims_\w+)\s*\(", hdr))
+    assert declared, "no declarations found"
+    L = E.lib()
+    missing = [s for s in sorted(declared) if not hasattr(L, s)]
+    assert not missing, missing
+    assert declared == set(E.EXPORTS)
+
+
+def test_params_defaults_and_input_file(tmp_path):
+    p = E.default_params()
+    assert (p.temp, p.dt, p.traj, p.iter, p.multipole, p.dispersion_cutoff) == (298, 0.01, 50000, 10, 1, 1)
+    assert (p.multipole_order, p.max_cluster_size, p.projection, p.print_rate) == (1, 4, 1, 20000)
+    f = tmp_path / "in.txt"
+    f.write_text("# comment line\nname myion\ntemp 350\ntraj 12.5\niter 4\nmultipole false\n"
+                 "dispersion_cutoff 0\nmax_cluster_size 4.5\nseed 42\ncharge 3\nbogus 1\ntemp 300\n"
+                 "incomplete\n")
+    E.read_params(str(f), p)
+    assert p.name == b"myion"
+    assert p.temp == 300.0            # last value wins
+    assert p.traj == 12500            # thousands
+    assert p.iter == 4 and p.seed == 42
+    assert p.multipole == 0 and p.dispersion_cutoff == 0
+    assert p.max_cluster_size == 4    # atoi
+    assert p.charge == 1 and p.total_charge == 3.0
+    assert p.print_rate == 2500       # traj/5 when not given
+
+
+@pytest.mark.parametrize("mol,atoms", [("acau.mfj", None), ("acau.pdb", None),
+                                       ("acau_phos_sigma_chg.mfj", "atoms.dat"),
+                                       ("acau_phos_sigma.pdb", "atoms.dat"), ("ion128.mfj", None)])
+def test_molfile_reader_matches_oracle_reader(mol, atoms):
+    f = E.read_molfile(os.path.join(GOLDEN, mol), os.path.join(GOLDEN, atoms) if atoms else None)
+    tab = molio.default_table()
+    if atoms:
+        molio.apply_atoms_file(tab, os.path.join(GOLDEN, atoms))
+    path = os.path.join(GOLDEN, mol)
+    m = molio.read_pdb(path) if mol.endswith(".pdb") else molio.read_mfj(path, tab)
+    par = molio.resolve(tab, m["symbols"])
+    assert f["n"] == len(par)
+    assert np.array_equal(f["xyz"], np.array(m["xyz"]))
+    assert np.array_equal(f["mass"], [p[0] for p in par])
+    assert np.array_equal(f["sigma"], [p[1] for p in par])
+    assert np.array_equal(f["epsilon"], [p[2] for p in par])
+    assert f["charged"] == m["charged"]
+    if m["charged"]:
+        assert np.array_equal(f["charge"], m["charge"])
+        assert f["total_charge"] == m["total_charge"]
+    assert f["sigma_max"] == max(r[3] for r in tab)
+
+
+def _engine_setup(g, threads=0):
+    f = E.read_molfile(os.path.join(GOLDEN, g["mol"]),
+                       os.path.join(GOLDEN, g["atoms_dat"]) if g["atoms_dat"] else None)
+    p = E.default_params()
+    if g["input"]:
+        import tempfile
+        with tempfile.NamedTemporaryFile("w", suffix=".in", delete=False) as t:
+            t.write(g["input"])
+        E.read_params(t.name, p)
+        os.unlink(t.name)
+    p.charge = int(f["charged"])
+    p.threads = threads
+    return E.Molecule.create(p, f["xyz"], f["mass"], f["sigma"], f["epsilon"],
+                             f["charge"] if f["charged"] else None, f["sigma_max"])
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_engine_setup_matches_reference(case):
+    g = golden(case)
+    sc = g["scalars"]
+    m = _engine_setup(g, threads=4)
+    inf = m.info()
+    A = m.atoms()
+    assert np.array_equal(A["xyz"].reshape(-1), g["atoms"]["xyz"])
+    for k in ("q", "sig6", "eps24m", "eps4"):
+        assert np.array_equal(A[k], g["atoms"][k]), k
+    for k in ("mu", "a", "b", "c", "bmax", "gmin", "gmax", "max_val", "dispersion_radius", "multipole_radius"):
+        assert inf[k] == sc[k], k
+    assert inf["vd_norm"] == sc["norm"]
+    assert inf["ccs_const"] == sc["ccs_scale"] * (sc["bmax"] * sc["bmax"])
+    assert inf["dipole_scalar"] == sc["dipole_scalar"]
+    cl = m.clusters()
+    if sc["multipole"] or sc["dispersion_cutoff"]:
+        assert list(cl["start"]) == g["pcm"]["start"]
+        assert list(cl["atoms"]) == g["pcm"]["atoms"]
+        assert np.array_equal(cl["center"], g["pcm"]["center"])
+        assert np.array_equal(cl["maxdist"], g["pcm"]["maxdist"])
+        # multipoles and centre-of-charge replacement against the oracle
+        o = oracle.pcm_clusters(A["xyz"], A["q"], int(sc["max_cluster_size"]))
+        for k, ko in (("q", "q"), ("dip", "dip"), ("quad", "quad"), ("pos", "pos"), ("replace", "replace")):
+            assert np.array_equal(cl[k], o[ko]), k
+    else:
+        assert cl is None
+
+
+def test_setup_thread_count_does_not_change_result():
+    g = golden("acau_charged")
+    a = _engine_setup(g, threads=1).info()
+    b = _engine_setup(g, threads=8).info()
+    assert (a["a"], a["b"], a["c"], a["n_boundary"]) == (b["a"], b["b"], b["c"], b["n_boundary"])
+
+
+def test_errors_are_reported():
+    p = E.default_params()
+    with pytest.raises(E.CosimsError, match="invalid argument"):
+        E.Molecule.create(p, np.zeros((0, 3)), [], [], [])
+    with pytest.raises(E.CosimsError):
+        E.read_molfile(os.path.join(GOLDEN, "does_not_exist.mfj"))
+    # cut-off force types need clusters when injected
+    p = E.default_params(charge=1, multipole=1, dispersion_cutoff=1)
+    with pytest.raises(E.CosimsError, match="clusters"):
+        E.Molecule.from_prepared(p, np.zeros((1, 3)), [1.0], [1.0], [1.0], [1.0], 3.9, (10, 10, 10))
+
+
+def test_unknown_atom_type_is_rejected(tmp_path):
+    f = tmp_path / "x.mfj"
+    f.write_text("x\n1\n2\nang\ncalc\n1.0\n0 0 0 12 0.5\n1.5 0 0 99 0.5\n")
+    with pytest.raises(E.CosimsError, match="type not in the force field"):
+        E.read_molfile(str(f))
