@@ -108,7 +108,7 @@ def main():
         times.append(time.perf_counter() - t1)
         runs.append(r)
         log(f"step {k}: {times[-1] * 1e3:.1f} ms, traj kernel {r['traj_kernel_ms']} ms, "
-            f"CCS {r['wmean']:.3f} +- {r['werr']:.3f}")
+            f"CCS {r['wmean']:.3f} +- {r['werr']:.3f}, lanes {r['lanes']} wg {r['wg_size']}")
     E.device_sync()
     elapsed = time.perf_counter() - t_all
     value = n_traj * a.steps / elapsed
@@ -116,10 +116,13 @@ def main():
 
     fp64 = None
     if not a.no_fp64:
+        m.run(E.PREC_FP64, job=1, iters=1, traj=2000)   # tunes the FP64 launch shape outside the timing
+        E.device_sync()
         t1 = time.perf_counter()
         fp64 = m.run(E.PREC_FP64, job=0)
         fp64_s = time.perf_counter() - t1
-        log(f"fp64: {fp64_s * 1e3:.1f} ms, CCS {fp64['wmean']:.3f} +- {fp64['werr']:.3f}")
+        log(f"fp64: {fp64_s * 1e3:.1f} ms, CCS {fp64['wmean']:.3f} +- {fp64['werr']:.3f}, "
+            f"lanes {fp64['lanes']} wg {fp64['wg_size']}")
 
     # roofline of the dominant kernel (first trajectory launch of each step)
     kms = np.array([r["traj_kernel_ms"][0] for r in runs])
@@ -139,6 +142,9 @@ def main():
                      "traffic": None, "kernel": "k_traj<mixed,type4>",
                      "kernel_ms": float(kms.mean())},
         "ccs": {"mixed_wmean": mixed["wmean"], "mixed_werr": mixed["werr"]},
+        "launch": {"lanes_per_trajectory": mixed["lanes"], "wg_size": mixed["wg_size"],
+                   "n_retry_or_restart": mixed["n_retry"], "n_fail": mixed["n_fail"],
+                   "rounds": mixed["n_rounds"], "sample_ms": mixed["sample_kernel_ms"]},
         "steps_per_s": mixed["total_steps"] / (times[-1]),
         "pair_evals_per_s": mixed["pair_evals"] / times[-1],
     }
@@ -151,6 +157,7 @@ def main():
         out["fp64_value"] = n_traj / (fp64_s)
         out["mixed_over_fp64"] = value / out["fp64_value"]
         out["fp64_kernel_ms"] = fp64["traj_kernel_ms"][0]
+        out["fp64_launch"] = {"lanes_per_trajectory": fp64["lanes"], "wg_size": fp64["wg_size"]}
     threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
     if a.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(m, a.cpu_seconds, threads)

@@ -30,6 +30,7 @@
 #include "cosims_engine.h"
 #include "kernels.hpp"
 #include "prep.hpp"
+#include "traj_group.hpp"
 
 using namespace cosims;
 
@@ -477,6 +478,7 @@ AccFn accel_kernel(int prec, int ft) { return prec == COSIMS_PREC_FP64 ? acc_fn<
 // ------------------------------------------------------ device state ---
 struct Device {
     int id = -1, cus = 0;
+    size_t lds_max = 65536;  // largest dynamic LDS one workgroup may opt into
     char name[256] = {0};
 };
 
@@ -490,6 +492,11 @@ int device_init(Device &d)
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(COSIMS_E_NODEVICE, std::string("engine is built for gfx950, device is ") + prop.gcnArchName);
     d.cus = prop.multiProcessorCount;
+    int optin = 0;
+    if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, d.id) == hipSuccess && optin > 0)
+        d.lds_max = (size_t)optin;
+    else
+        d.lds_max = prop.sharedMemPerBlock;
     std::snprintf(d.name, sizeof(d.name), "%s", prop.name);
     return 0;
 }
@@ -523,6 +530,11 @@ struct cosims_mol {
     mutable Device dev;
     mutable MolDev md{};
     mutable void *dmem = nullptr;
+    mutable const double *soa = nullptr;  // 7 FP64 arrays of npad: x y z q s6 e24 e4
+    mutable int npad = 0;
+    // autotuned launch shape per precision: lanes per trajectory, workgroup size
+    mutable std::mutex tune_mu;
+    mutable int tuned_lanes[2] = {0, 0}, tuned_wg[2] = {0, 0};
     ~cosims_mol() { if (dmem) (void)hipFree(dmem); }
 };
 
@@ -569,9 +581,11 @@ int upload(const cosims_mol *M)
 {
     const Prepared &p = M->prep;
     const int n = p.n, ncl = p.ncl;
+    const int npad = (n + 63) / 64 * 64;
     size_t o64 = 0, o32 = align_up(o64 + sizeof(AtomF64) * n), oc64 = align_up(o32 + sizeof(AtomF32) * n),
            oc32 = align_up(oc64 + sizeof(AtomF64) * n), oh64 = align_up(oc32 + sizeof(AtomF32) * n),
-           oh32 = align_up(oh64 + sizeof(ClusterF64) * ncl), total = align_up(oh32 + sizeof(ClusterF32) * ncl);
+           oh32 = align_up(oh64 + sizeof(ClusterF64) * ncl), osoa = align_up(oh32 + sizeof(ClusterF32) * ncl),
+           total = align_up(osoa + sizeof(double) * 7 * npad);
     std::vector<char> host(total, 0);
     auto *a64 = (AtomF64 *)(host.data() + o64);
     auto *a32 = (AtomF32 *)(host.data() + o32);
@@ -594,6 +608,18 @@ int upload(const cosims_mol *M)
         fill_head(h64[c], p, c);
         fill_head(h32[c], p, c);
     }
+    // FP64 SoA in atom order, padded with inert atoms (no charge, no LJ, far away)
+    double *soa = (double *)(host.data() + osoa);
+    for (int i = 0; i < npad; i++) {
+        const bool real = i < n;
+        soa[i] = real ? p.xyz[3 * i] : 1.0e6;
+        soa[npad + i] = real ? p.xyz[3 * i + 1] : 1.0e6;
+        soa[2 * npad + i] = real ? p.xyz[3 * i + 2] : 1.0e6;
+        soa[3 * npad + i] = real ? p.q[i] : 0.0;
+        soa[4 * npad + i] = real ? p.sig6[i] : 0.0;
+        soa[5 * npad + i] = real ? p.eps24m[i] : 0.0;
+        soa[6 * npad + i] = real ? p.eps4[i] : 0.0;
+    }
     HIP_CHECK(hipMalloc(&M->dmem, total));
     HIP_CHECK(hipMemcpy(M->dmem, host.data(), total, hipMemcpyHostToDevice));
     char *d = (char *)M->dmem;
@@ -605,6 +631,8 @@ int upload(const cosims_mol *M)
     m.a64 = (const AtomF64 *)(d + o64); m.a32 = (const AtomF32 *)(d + o32);
     m.c64 = (const AtomF64 *)(d + oc64); m.c32 = (const AtomF32 *)(d + oc32);
     m.h64 = (const ClusterF64 *)(d + oh64); m.h32 = (const ClusterF32 *)(d + oh32);
+    M->soa = (const double *)(d + osoa);
+    M->npad = npad;
     return 0;
 }
 
@@ -977,6 +1005,176 @@ int run_rounds(const cosims_mol *M, int prec, int64_t n, TrajBufs &B, hipStream_
 
 }  // namespace
 
+
+namespace {
+
+// ------------------------------------------------- group-mode launches ---
+using GrpFn = void (*)(GroupArgs);
+
+template <int P, int FT>
+GrpFn grp_fn_l(int L)
+{
+    switch (L) {
+    case 4: return k_traj_grp<P, FT, 4>;
+    case 8: return k_traj_grp<P, FT, 8>;
+    case 16: return k_traj_grp<P, FT, 16>;
+    case 32: return k_traj_grp<P, FT, 32>;
+    case 64: return k_traj_grp<P, FT, 64>;
+    default: return nullptr;
+    }
+}
+
+GrpFn group_kernel(int prec, int ft, int L)
+{
+    if (ft == 4) return prec == COSIMS_PREC_FP64 ? grp_fn_l<0, 4>(L) : grp_fn_l<1, 4>(L);
+    if (ft == 6) return prec == COSIMS_PREC_FP64 ? grp_fn_l<0, 6>(L) : grp_fn_l<1, 6>(L);
+    return nullptr;
+}
+
+size_t group_lds_bytes(const cosims_mol *M, int prec)
+{
+    return (size_t)M->npad * (prec == COSIMS_PREC_FP64 ? 6 * sizeof(double) : sizeof(float4) + sizeof(float2));
+}
+
+bool group_mode_possible(const cosims_mol *M, int prec)
+{
+    const int ft = M->prep.force_type;
+    return (ft == 4 || ft == 6) && group_lds_bytes(M, prec) <= M->dev.lds_max;
+}
+
+struct GroupRun {
+    double ms = 0;
+    int64_t steps = 0, evals = 0, pots = 0;
+    int grid = 0;
+};
+
+// One persistent launch over the item list B.items[0] (count n_items).
+int run_group(const cosims_mol *M, int prec, int n_items, TrajBufs &B, hipStream_t s, int L, int wg, int bpc,
+              GroupRun &out)
+{
+    GrpFn fn = group_kernel(prec, M->prep.force_type, L);
+    if (!fn) return fail(COSIMS_E_INVALID, "no group kernel for this force type / lane count");
+    const size_t lds = group_lds_bytes(M, prec);
+    HIP_CHECK(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int occ = bpc;
+    if (occ <= 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)fn, wg, lds) != hipSuccess ||
+                     occ <= 0))
+        occ = 1;
+    const int groups_per_wg = wg / L;
+    int64_t grid = (int64_t)occ * M->dev.cus;
+    grid = std::max<int64_t>(1, std::min<int64_t>(grid, (n_items + groups_per_wg - 1) / groups_per_wg));
+    int32_t *ctr = B.counters.as<int32_t>();
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ctr + 1, 2, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, B.evals.as<int32_t>(), 4 * COSIMS_MAX_ROUNDS, 0);
+    GroupArgs G{};
+    G.m = M->md;
+    G.npad = M->npad;
+    const double *S = M->soa;
+    G.sx = S; G.sy = S + M->npad; G.sz = S + 2 * M->npad; G.sq = S + 3 * M->npad;
+    G.s6 = S + 4 * M->npad; G.se24 = S + 5 * M->npad; G.se4 = S + 6 * M->npad;
+    G.items = B.items[0].as<int32_t>();
+    G.n_items = n_items;
+    G.counter = ctr + 1;
+    G.p0 = B.p0.as<double>(); G.v0 = B.v0.as<double>();
+    G.status = B.status.as<int8_t>(); G.chi = B.chi.as<double>(); G.ergy = B.ergy.as<double>();
+    G.cchi = B.exitv2.as<double>();  // the lane kernel's exit buffer holds cos(chi) here
+    G.steps = B.steps.as<int32_t>(); G.totsteps = B.tot.as<int32_t>();
+    G.dt0 = M->params.dt;
+    G.max_steps = kMaxStepsPerAttempt;
+    G.evals = B.evals.as<unsigned long long>();
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, s));
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(wg), lds, s, G);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(e1, s));
+    hipLaunchKernelGGL(k_chi, dim3((unsigned)((n_items + 255) / 256)), dim3(256), 0, s, G.items, n_items, G.status,
+                       G.cchi, G.chi);
+    HIP_CHECK(hipGetLastError());
+    unsigned long long ev[3];
+    HIP_CHECK(hipMemcpyAsync(ev, B.evals.p, sizeof(ev), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    out.ms = ms;
+    out.steps = (int64_t)ev[0];
+    out.evals = (int64_t)ev[1];
+    out.pots = (int64_t)ev[2];
+    out.grid = (int)grid;
+    return 0;
+}
+
+// k_sample into B with hit compaction into B.items[0] (count in counters[0]).
+int sample_into(const cosims_mol *M, uint64_t seed, uint32_t job, int64_t n, int64_t per_iter, TrajBufs &B,
+                hipStream_t s, float *ms)
+{
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, B.counters.as<int32_t>(), 64, 0);
+    SampleArgs A = sample_args(M, seed, job);
+    A.first = 0; A.n = n; A.per_iter = per_iter;
+    A.p0 = B.p0.as<double>(); A.v0 = B.v0.as<double>(); A.g = B.g.as<double>();
+    A.status = B.status.as<int8_t>(); A.items = B.items[0].as<int32_t>(); A.n_items = B.counters.as<int32_t>();
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, s));
+    hipLaunchKernelGGL(k_sample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, A);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(e1, s));
+    HIP_CHECK(hipEventSynchronize(e1));
+    HIP_CHECK(hipEventElapsedTime(ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 0;
+}
+
+// Autotuner: lanes per trajectory x workgroup size on a calibration batch of
+// the molecule itself (fixed Philox stream), fastest time per trajectory.
+int autotune(const cosims_mol *M, int prec, int &L_out, int &wg_out)
+{
+    std::lock_guard<std::mutex> lk(M->tune_mu);
+    const int pi = prec == COSIMS_PREC_FP64 ? 0 : 1;
+    if (M->tuned_lanes[pi]) {
+        L_out = M->tuned_lanes[pi];
+        wg_out = M->tuned_wg[pi];
+        return 0;
+    }
+    Stream st;
+    HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+    const int64_t n = std::max<int64_t>(16384, (int64_t)M->dev.cus * 96);
+    TrajBufs B;
+    if (int rc = B.alloc(n)) return rc;
+    float sms = 0;
+    if (int rc = sample_into(M, 0x5eed5eedULL, 0xffffffffu, n, n, B, st.s, &sms)) return rc;
+    int32_t hits = 0;
+    HIP_CHECK(hipMemcpy(&hits, B.counters.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+    double best = 1e300;
+    int bL = 16, bW = 256;
+    const int Ls[] = {4, 8, 16, 32, 64};
+    const int Ws[] = {256, 512, 1024};
+    for (int L : Ls) {
+        if (L > 4 && M->prep.n < 8 * L) continue;  // too few atoms per lane
+        for (int W : Ws) {
+            GroupRun R;
+            if (int rc = run_group(M, prec, hits, B, st.s, L, W, 0, R)) return rc;
+            if (R.ms < best) {
+                best = R.ms;
+                bL = L;
+                bW = W;
+            }
+        }
+    }
+    M->tuned_lanes[pi] = bL;
+    M->tuned_wg[pi] = bW;
+    L_out = bL;
+    wg_out = bW;
+    return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 int cosims_sample(const cosims_mol *M, uint64_t seed, uint32_t job, uint32_t iteration, int64_t first, int64_t n,
@@ -1014,6 +1212,11 @@ int cosims_verlet(const cosims_mol *M, int32_t prec, int64_t n, const double *p0
     if (n > INT32_MAX / 4) return fail(COSIMS_E_INVALID, "cosims_verlet: batch too large");
     if (n == 0) return 0;
     if (int rc = ensure_device(M)) return rc;
+    // FP64 runs the lane-per-trajectory kernel: atom-order sums, bit-identical
+    // to the reference; mixed precision runs the tuned group kernel.
+    int L = 1, wg = 256;
+    if (prec == COSIMS_PREC_MIXED && group_mode_possible(M, prec))
+        if (int rc = autotune(M, prec, L, wg)) return rc;
     Stream st;
     HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
     TrajBufs B;
@@ -1028,8 +1231,13 @@ int cosims_verlet(const cosims_mol *M, int32_t prec, int64_t n, const double *p0
     HIP_CHECK(hipMemcpyAsync(B.counters.p, cnt, sizeof(cnt), hipMemcpyHostToDevice, st.s));
     std::vector<int8_t> s8(n, ST_PENDING);
     HIP_CHECK(hipMemcpyAsync(B.status.p, s8.data(), n, hipMemcpyHostToDevice, st.s));
-    RoundStats R;
-    rc = run_rounds(M, prec, n, B, st.s, 256, 0, R);
+    if (L == 1) {
+        RoundStats R;
+        rc = run_rounds(M, prec, n, B, st.s, 256, 0, R);
+    } else {
+        GroupRun G;
+        rc = run_group(M, prec, (int)n, B, st.s, L, wg, 0, G);
+    }
     if (rc) return rc;
     std::vector<int32_t> tot(n);
     HIP_CHECK(hipMemcpyAsync(chi, B.chi.p, sizeof(double) * n, hipMemcpyDeviceToHost, st.s));
@@ -1059,35 +1267,66 @@ int cosims_run(const cosims_mol *M, const cosims_params *P, const cosims_run_cfg
     if (iters <= 0 || T <= 0) return fail(COSIMS_E_INVALID, "cosims_run: no trajectories");
     const int64_t n = (int64_t)iters * T;
     if (n > INT32_MAX / 4) return fail(COSIMS_E_INVALID, "cosims_run: too many trajectories for one batch");
-    const int wg = c.wg_size > 0 ? c.wg_size : 256;
-    if (wg % 64 || wg > 256) return fail(COSIMS_E_INVALID, "wg_size must be a multiple of 64, at most 256");
+    if (c.lanes != 0 && c.lanes != 1 && c.lanes != 4 && c.lanes != 8 && c.lanes != 16 && c.lanes != 32 &&
+        c.lanes != 64)
+        return fail(COSIMS_E_INVALID, "lanes must be 0 (tuned), 1, 4, 8, 16, 32 or 64");
     std::memset(out, 0, sizeof(*out));
     if (int rc = ensure_device(M)) return rc;
+    // launch shape: explicit, or tuned (group kernel where the molecule fits LDS)
+    int L = c.lanes, wg = c.wg_size;
+    if (L > 1 && !group_mode_possible(M, prec))
+        return fail(COSIMS_E_INVALID, "group kernel needs force type 4 or 6 and a molecule that fits LDS");
+    if (L == 0) {
+        if (group_mode_possible(M, prec)) {
+            int tl = 0, tw = 0;
+            if (int rc = autotune(M, prec, tl, tw)) return rc;
+            L = tl;
+            if (wg <= 0) wg = tw;
+        } else {
+            L = 1;
+        }
+    }
+    if (wg <= 0) wg = 256;
+    if (wg % 64 || wg > (L == 1 ? 256 : 1024) || wg % L)
+        return fail(COSIMS_E_INVALID, "wg_size must be a multiple of 64 (at most 256 lane-mode, 1024 group-mode)");
     auto t0 = std::chrono::steady_clock::now();
     Stream st;
     HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
     TrajBufs B;
     int rc = B.alloc(n);
     if (rc) return rc;
-    HIP_CHECK(hipMemsetAsync(B.counters.p, 0, sizeof(int32_t) * 64, st.s));
-    SampleArgs A = sample_args(M, P->seed, c.job);
-    A.first = 0; A.n = n; A.per_iter = T;
-    A.p0 = B.p0.as<double>(); A.v0 = B.v0.as<double>(); A.g = B.g.as<double>();
-    A.status = B.status.as<int8_t>(); A.items = B.items[0].as<int32_t>(); A.n_items = B.counters.as<int32_t>();
-    hipEvent_t es0, es1;
-    HIP_CHECK(hipEventCreate(&es0));
-    HIP_CHECK(hipEventCreate(&es1));
-    HIP_CHECK(hipEventRecord(es0, st.s));
-    hipLaunchKernelGGL(k_sample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st.s, A);
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipEventRecord(es1, st.s));
-    RoundStats R;
-    rc = run_rounds(M, prec, n, B, st.s, wg, c.blocks_per_cu, R);
-    if (rc) return rc;
     float sms = 0;
-    HIP_CHECK(hipEventElapsedTime(&sms, es0, es1));
-    (void)hipEventDestroy(es0);
-    (void)hipEventDestroy(es1);
+    if ((rc = sample_into(M, P->seed, c.job, n, T, B, st.s, &sms))) return rc;
+    int32_t hits = 0;
+    HIP_CHECK(hipMemcpyAsync(&hits, B.counters.p, sizeof(int32_t), hipMemcpyDeviceToHost, st.s));
+    HIP_CHECK(hipStreamSynchronize(st.s));
+    if (L == 1) {
+        RoundStats R;
+        rc = run_rounds(M, prec, n, B, st.s, wg, c.blocks_per_cu, R);
+        if (rc) return rc;
+        out->n_retry = R.retries;
+        out->n_rounds = R.rounds;
+        for (int r = 0; r < R.rounds; r++) {
+            out->traj_kernel_ms[r] = R.traj_ms[r];
+            out->round_items[r] = R.items[r];
+            out->round_steps[r] = R.steps[r];
+            out->round_evals[r] = R.evals[r];
+        }
+        out->check_kernel_ms = R.check_ms;
+    } else if (hits > 0) {
+        GroupRun G;
+        rc = run_group(M, prec, hits, B, st.s, L, wg, c.blocks_per_cu, G);
+        if (rc) return rc;
+        out->n_rounds = 1;
+        out->traj_kernel_ms[0] = G.ms;
+        out->round_items[0] = hits;
+        out->round_steps[0] = G.steps;
+        out->round_evals[0] = G.evals;
+        out->pot_evals = G.pots * (int64_t)M->prep.n;
+    }
+    out->lanes = L;
+    out->wg_size = wg;
+    out->sample_kernel_ms = sms;
     DBuf red;
     HIP_CHECK(red.alloc(sizeof(double) * 6 * iters));
     ReduceArgs RA{};
@@ -1104,15 +1343,15 @@ int cosims_run(const cosims_mol *M, const cosims_params *P, const cosims_run_cfg
     HIP_CHECK(hipStreamSynchronize(st.s));
     // Denominator currSuccTraj of one OpenMP thread (src/monteTraj.cpp:456-521):
     // failures (a miss repeats the previous success flag) counted since the
-    // last progress update every ceil(print_rate) trajectories.
+    // last progress update every print_rate trajectories.
     const int64_t upd = P->print_rate > 0 ? P->print_rate : std::max<int64_t>(1, (int64_t)(T / 5.0));
     double mx = 0;
     for (int k = 0; k < iters; k++) {
-        const int8_t *s = status.data() + (int64_t)k * T;
+        const int8_t *st8 = status.data() + (int64_t)k * T;
         bool succ = false;
         int64_t orbit = 0;
         for (int64_t i = 0; i < T; i++) {
-            if (s[i] != ST_MISS) succ = (s[i] == ST_OK);
+            if (st8[i] != ST_MISS) succ = (st8[i] == ST_OK);
             if (!succ) orbit++;
             if ((i + 1) % upd == 0) orbit = 0;
         }
@@ -1144,20 +1383,11 @@ int cosims_run(const cosims_mol *M, const cosims_params *P, const cosims_run_cfg
     out->werr = 1.0 / std::sqrt(we);
     out->max_ergy = mx;
     out->n_traj = n;
-    out->n_retry = R.retries;
-    out->n_rounds = R.rounds;
-    for (int r = 0; r < R.rounds; r++) {
-        out->traj_kernel_ms[r] = R.traj_ms[r];
-        out->round_items[r] = R.items[r];
-        out->round_steps[r] = R.steps[r];
-        out->round_evals[r] = R.evals[r];
-    }
-    out->check_kernel_ms = R.check_ms;
-    out->sample_kernel_ms = sms;
     const int ft = M->prep.force_type;
     const int64_t npair = (ft == 4 || ft == 6) ? M->prep.n : 0;
-    out->pair_evals = npair * (out->total_steps + out->n_hit + out->n_retry);
-    out->pot_evals = (int64_t)M->prep.n * (2 * out->n_hit + out->n_retry);
+    int64_t evals = 0;
+    for (int r = 0; r < out->n_rounds; r++) evals += out->round_evals[r];
+    out->pair_evals = npair * evals;
     out->run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return 0;
 }

@@ -73,7 +73,7 @@ class MolInfo(C.Structure):
 
 class RunCfg(C.Structure):
     _fields_ = [("prec", C.c_int32), ("job", C.c_uint32), ("iter", C.c_int32), ("traj", C.c_int64),
-                ("wg_size", C.c_int32), ("blocks_per_cu", C.c_int32), ("chi", C.c_void_p),
+                ("lanes", C.c_int32), ("wg_size", C.c_int32), ("blocks_per_cu", C.c_int32), ("chi", C.c_void_p),
                 ("status", C.c_void_p), ("g", C.c_void_p)]
 
 
@@ -84,7 +84,7 @@ class Ccs(C.Structure):
                [("run_ms", C.c_double), ("n_rounds", C.c_int32),
                 ("traj_kernel_ms", C.c_double * MAX_ROUNDS), ("round_steps", C.c_int64 * MAX_ROUNDS),
                 ("round_evals", C.c_int64 * MAX_ROUNDS), ("round_items", C.c_int64 * MAX_ROUNDS), ("check_kernel_ms", C.c_double),
-                ("sample_kernel_ms", C.c_double)]
+                ("sample_kernel_ms", C.c_double), ("lanes", C.c_int32), ("wg_size", C.c_int32)]
 
 
 def lib():
@@ -303,11 +303,12 @@ class Molecule:
         return dict(chi=chi, ok=ok, steps=steps, ergy=ergy, total_steps=tot)
 
     def run(self, prec=PREC_MIXED, job=0, iters=0, traj=0, keep=False, params: Params | None = None,
-            wg_size=0, blocks_per_cu=0):
+            lanes=0, wg_size=0, blocks_per_cu=0):
         p = params if params is not None else self.params
         it = iters or p.iter
         T = traj or p.traj
-        cfg = RunCfg(prec=prec, job=job, iter=it, traj=T, wg_size=wg_size, blocks_per_cu=blocks_per_cu)
+        cfg = RunCfg(prec=prec, job=job, iter=it, traj=T, lanes=lanes, wg_size=wg_size,
+                     blocks_per_cu=blocks_per_cu)
         chi = status = g = None
         if keep:
             chi = np.empty(it * T); status = np.empty(it * T, np.int32); g = np.empty(it * T)

@@ -176,8 +176,10 @@ typedef struct cosims_run_cfg {
     uint32_t job;          /* Philox key extension (request id) */
     int32_t iter;          /* 0 => params.iter */
     int64_t traj;          /* 0 => params.traj */
+    int32_t lanes;         /* lanes per trajectory: 0 tuned, 1 lane-per-trajectory
+                              (atom-order sums, bit-identical FP64), 4..64 group kernel */
     int32_t wg_size;       /* 0 => tuned default */
-    int32_t blocks_per_cu; /* 0 => tuned default */
+    int32_t blocks_per_cu; /* 0 => occupancy limit */
     /* optional per-trajectory outputs (iter*traj entries each, may be NULL) */
     double *chi;
     int32_t *status;       /* 0 miss, 1 success, 2 failure */
@@ -198,6 +200,7 @@ typedef struct cosims_ccs {
     int64_t round_evals[COSIMS_MAX_ROUNDS];   /* force evaluations of each launch */
     int64_t round_items[COSIMS_MAX_ROUNDS];   /* trajectories queued for each launch */
     double check_kernel_ms, sample_kernel_ms;
+    int32_t lanes, wg_size;                   /* launch shape used */
 } cosims_ccs;
 /* omega/omega_err: iter entries each. */
 int cosims_run(const cosims_mol *m, const cosims_params *p, const cosims_run_cfg *cfg, double *omega,

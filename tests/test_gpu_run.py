@@ -41,7 +41,8 @@ def _oracle_for(m):
 
 @pytest.fixture(scope="module")
 def fp64_run(ion128):
-    return ion128.run(E.PREC_FP64, job=0, keep=True)
+    # lane-per-trajectory kernel: atom-order sums, the reference's operation order
+    return ion128.run(E.PREC_FP64, job=0, keep=True, lanes=1)
 
 
 @pytest.fixture(scope="module")
@@ -77,6 +78,20 @@ def test_fp64_run_matches_oracle(fp64_run, oracle_run):
     assert abs(r["wmean"] - fin["wmean"]) / fin["wmean"] <= 5e-4
 
 
+@pytest.mark.parametrize("lanes", [4, 16, 64])
+def test_fp64_group_run_matches_oracle(ion128, oracle_run, lanes):
+    """Group kernel: the same FP64 pair terms summed in lane-split order."""
+    r = ion128.run(E.PREC_FP64, job=0, keep=True, lanes=lanes)
+    o = oracle_run
+    hit = o["hit"] == 1
+    assert np.array_equal(r["status"] != 0, hit)
+    good = hit & (o["ok"] == 1) & (r["status"] == 1)
+    d = np.abs(r["chi"][good] - o["chi"][good])
+    assert np.median(d) <= 1e-8, np.median(d)
+    assert np.mean(r["status"][hit] == np.where(o["ok"][hit] == 1, 1, 2)) >= 0.99
+    np.testing.assert_allclose(r["omega"], o["omega"], rtol=5e-4)
+
+
 def test_mixed_run_matches_fp64_oracle(ion128, oracle_run):
     r = ion128.run(E.PREC_MIXED, job=0, keep=True)
     o = oracle_run
@@ -86,8 +101,18 @@ def test_mixed_run_matches_fp64_oracle(ion128, oracle_run):
     assert dev <= 1e-3, dev
 
 
-def test_run_is_deterministic(ion128):
-    a = ion128.run(E.PREC_MIXED, job=5, iters=1, traj=4000, keep=True)
-    b = ion128.run(E.PREC_MIXED, job=5, iters=1, traj=4000, keep=True)
+@pytest.mark.parametrize("lanes", [1, 8])
+def test_run_is_deterministic(ion128, lanes):
+    a = ion128.run(E.PREC_MIXED, job=5, iters=1, traj=4000, keep=True, lanes=lanes)
+    b = ion128.run(E.PREC_MIXED, job=5, iters=1, traj=4000, keep=True, lanes=lanes)
     assert np.array_equal(a["chi"], b["chi"])
     assert a["omega"][0] == b["omega"][0]
+
+
+def test_mixed_lane_and_group_kernels_agree(ion128):
+    """Same initial conditions through both mixed kernels: CCS agrees to
+    well within the statistical error."""
+    a = ion128.run(E.PREC_MIXED, job=9, iters=2, traj=10000, lanes=1)
+    b = ion128.run(E.PREC_MIXED, job=9, iters=2, traj=10000, lanes=16)
+    assert abs(a["wmean"] - b["wmean"]) / a["wmean"] < 1e-3
+    assert b["lanes"] == 16 and a["lanes"] == 1
