@@ -1130,8 +1130,10 @@ int sample_into(const cosims_mol *M, uint64_t seed, uint32_t job, int64_t n, int
     return 0;
 }
 
-// Autotuner: lanes per trajectory x workgroup size on a calibration batch of
-// the molecule itself (fixed Philox stream), fastest time per trajectory.
+// Autotuner: lanes per trajectory x workgroup size, timed on the molecule
+// itself (fixed Philox stream).  Stage 1 screens every shape on a small
+// batch; stage 2 re-times the three best on a batch large enough that the
+// end-of-launch tail is amortised, and keeps the fastest per trajectory.
 int autotune(const cosims_mol *M, int prec, int &L_out, int &wg_out)
 {
     std::lock_guard<std::mutex> lk(M->tune_mu);
@@ -1143,33 +1145,42 @@ int autotune(const cosims_mol *M, int prec, int &L_out, int &wg_out)
     }
     Stream st;
     HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
-    const int64_t n = std::max<int64_t>(16384, (int64_t)M->dev.cus * 96);
+    const int64_t n1 = 16384, n2 = std::max<int64_t>(n1, (int64_t)M->dev.cus * 512);
     TrajBufs B;
-    if (int rc = B.alloc(n)) return rc;
-    float sms = 0;
-    if (int rc = sample_into(M, 0x5eed5eedULL, 0xffffffffu, n, n, B, st.s, &sms)) return rc;
-    int32_t hits = 0;
-    HIP_CHECK(hipMemcpy(&hits, B.counters.p, sizeof(int32_t), hipMemcpyDeviceToHost));
-    double best = 1e300;
-    int bL = 16, bW = 256;
+    if (int rc = B.alloc(n2)) return rc;
+    struct Cand {
+        int L, W;
+        double ms;
+    };
+    std::vector<Cand> cands;
     const int Ls[] = {4, 8, 16, 32, 64};
     const int Ws[] = {256, 512, 1024};
+    float sms = 0;
+    if (int rc = sample_into(M, 0x5eed5eedULL, 0xfffffffeu, n1, n1, B, st.s, &sms)) return rc;
+    int32_t hits = 0;
+    HIP_CHECK(hipMemcpy(&hits, B.counters.p, sizeof(int32_t), hipMemcpyDeviceToHost));
     for (int L : Ls) {
         if (L > 4 && M->prep.n < 8 * L) continue;  // too few atoms per lane
         for (int W : Ws) {
             GroupRun R;
             if (int rc = run_group(M, prec, hits, B, st.s, L, W, 0, R)) return rc;
-            if (R.ms < best) {
-                best = R.ms;
-                bL = L;
-                bW = W;
-            }
+            cands.push_back({L, W, R.ms});
         }
     }
-    M->tuned_lanes[pi] = bL;
-    M->tuned_wg[pi] = bW;
-    L_out = bL;
-    wg_out = bW;
+    std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
+    if (cands.size() > 3) cands.resize(3);
+    if (int rc = sample_into(M, 0x5eed5eedULL, 0xfffffffdu, n2, n2, B, st.s, &sms)) return rc;
+    HIP_CHECK(hipMemcpy(&hits, B.counters.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (Cand &c : cands) {
+        GroupRun R;
+        if (int rc = run_group(M, prec, hits, B, st.s, c.L, c.W, 0, R)) return rc;
+        c.ms = R.ms;
+    }
+    std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
+    M->tuned_lanes[pi] = cands[0].L;
+    M->tuned_wg[pi] = cands[0].W;
+    L_out = cands[0].L;
+    wg_out = cands[0].W;
     return 0;
 }
 

@@ -20,6 +20,8 @@
 #include <fstream>
 #include <sstream>
 #include <atomic>
+#include <random>
+#include <stdlib.h>
 #include <thread>
 
 #include "cosims_engine.h"
@@ -734,6 +736,71 @@ int prepare(const Params &p, int n, const double *xyz, const double *mass, const
         if (rc) return rc;
     }
     return complete_prepared(p, m, err);
+}
+
+void sample_compat(const Prepared &P, uint64_t seed, int iters, int64_t traj, double *p0, double *v0, double *g,
+                   int32_t *hit)
+{
+    // rand() of a fresh process: glibc's additive generator in its default
+    // 128-byte state seeded with 1, kept private here instead of touching
+    // the caller's global rand() state.
+    struct random_data rd;
+    std::memset(&rd, 0, sizeof(rd));
+    static thread_local char state[128];
+    initstate_r(1, state, sizeof(state), &rd);
+    const double gMin = P.vd.gmin, gMax = P.vd.gmax, maxVal = P.vd.max_val, bMax = P.bmax;
+    const double maxDist = (float)bMax;  // monteTraj::maxDist is a float (src/monteTraj.h:67)
+    unsigned restart = 0;
+    for (int it = 0; it < iters; it++) {
+        std::mt19937_64 gen((unsigned long long)(long long)seed);
+        gen.discard(it == 0 ? 0ULL : (unsigned long long)restart);
+        std::uniform_real_distribution<double> distr(0.0, 1.0);
+        int count = 0;
+        for (int64_t n = 0; n < traj; n++) {
+            const int64_t o = (int64_t)it * traj + n;
+            double gg, phi, theta, gamma, b;
+            for (;;) {
+                count += 8;
+                gg = distr(gen) * (gMax - gMin) + gMin;
+                int32_t r = 0;
+                random_r(&rd, &r);
+                double gCheck = ((double)r / (double)RAND_MAX) * maxVal;
+                phi = distr(gen) * 2.0 * kPi;
+                theta = distr(gen) * kPi;
+                double thetaCheck = 0.5 * distr(gen);
+                gamma = distr(gen) * 2.0 * kPi;
+                b = distr(gen) * bMax;
+                double bCheck = distr(gen) * (2 / bMax);
+                if (thetaCheck <= std::sin(theta) / 2 && bCheck <= 2 * b / (bMax * bMax) &&
+                    gCheck <= veldist_pdf(P.vd, gg))
+                    break;
+            }
+            double X = b * std::cos(gamma) * std::cos(phi) * std::cos(theta) - b * std::sin(gamma) * std::sin(phi) +
+                       maxDist * std::cos(phi) * std::sin(theta);
+            double Y = b * std::cos(gamma) * std::sin(phi) * std::cos(theta) + b * std::sin(gamma) * std::cos(phi) +
+                       maxDist * std::sin(phi) * std::sin(theta);
+            double Z = maxDist * std::cos(theta) - b * std::cos(gamma) * std::sin(theta);
+            double VX = -gg * std::cos(phi) * std::sin(theta), VY = -gg * std::sin(phi) * std::sin(theta),
+                   VZ = -gg * std::cos(theta);
+            bool h = true;
+            if (P.projection) {
+                V3 v(VX / P.a, VY / P.b, VZ / P.c), p(X / P.a, Y / P.b, Z / P.c);
+                V3 cr((p.y * v.z) - (p.z * v.y), (p.z * v.x) - (p.x * v.z), (p.x * v.y) - (p.y * v.x));
+                double disc = -cr.dot(cr) + v.dot(v);
+                if (disc < 0) {
+                    h = false;
+                } else {
+                    double pp = -(p.dot(v) / (v.dot(v))) - std::sqrt(disc) / v.dot(v);
+                    X = X + VX * pp; Y = Y + VY * pp; Z = Z + VZ * pp;
+                }
+            }
+            p0[3 * o] = X; p0[3 * o + 1] = Y; p0[3 * o + 2] = Z;
+            v0[3 * o] = VX; v0[3 * o + 1] = VY; v0[3 * o + 2] = VZ;
+            g[o] = gg;
+            hit[o] = h ? 1 : 0;
+        }
+        restart = (unsigned)count;
+    }
 }
 
 }  // namespace cosims

@@ -91,6 +91,14 @@ int prepare(const Params &p, int n, const double *xyz, const double *mass, const
 // CCS constant).
 int complete_prepared(const Params &p, Prepared &m, std::string &err);
 
+// Initial conditions in the reference's own random stream for a single
+// OpenMP thread (monteTraj::runTraj / setUpTraj, src/monteTraj.cpp:174-278,
+// 301-369): per CCS integral a fresh std::mt19937_64(seed) discarded by the
+// previous integral's draw count (restartGen, :344-349, :374), and one
+// process-wide rand() stream seeded 1.  Writes iters*traj entries.
+void sample_compat(const Prepared &P, uint64_t seed, int iters, int64_t traj, double *p0, double *v0, double *g,
+                   int32_t *hit);
+
 // Records the message returned by cosims_last_error() on this thread and
 // returns code (defined in engine.hip).
 int set_last_error(int code, const std::string &msg);
