@@ -1214,6 +1214,15 @@ int cosims_sample(const cosims_mol *M, uint64_t seed, uint32_t job, uint32_t ite
     return 0;
 }
 
+int cosims_sample_reference(const cosims_mol *M, uint64_t seed, int32_t iters, int64_t traj, double *p0, double *v0,
+                            double *g, int32_t *hit)
+{
+    if (!M || iters <= 0 || traj <= 0 || !p0 || !v0 || !g || !hit)
+        return fail(COSIMS_E_INVALID, "cosims_sample_reference: bad arguments");
+    sample_compat(M->prep, seed, iters, traj, p0, v0, g, hit);
+    return 0;
+}
+
 int cosims_verlet(const cosims_mol *M, int32_t prec, int64_t n, const double *p0, const double *v0, double *chi,
                   int32_t *ok, int32_t *steps, double *ergy, int64_t *total_steps)
 {
@@ -1307,7 +1316,33 @@ int cosims_run(const cosims_mol *M, const cosims_params *P, const cosims_run_cfg
     int rc = B.alloc(n);
     if (rc) return rc;
     float sms = 0;
-    if ((rc = sample_into(M, P->seed, c.job, n, T, B, st.s, &sms))) return rc;
+    if (c.rng == COSIMS_RNG_REFERENCE) {
+        // the reference's stream, drawn on the host and uploaded
+        std::vector<double> hp0(3 * n), hv0(3 * n), hg(n);
+        std::vector<int32_t> hh(n), items;
+        std::vector<int8_t> s8(n);
+        sample_compat(M->prep, P->seed, iters, T, hp0.data(), hv0.data(), hg.data(), hh.data());
+        items.reserve(n);
+        for (int64_t i = 0; i < n; i++) {
+            s8[i] = hh[i] ? ST_PENDING : ST_MISS;
+            if (hh[i]) items.push_back((int32_t)i);
+        }
+        int32_t cnt[64] = {0};
+        cnt[0] = (int32_t)items.size();
+        HIP_CHECK(hipMemcpyAsync(B.p0.p, hp0.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, st.s));
+        HIP_CHECK(hipMemcpyAsync(B.v0.p, hv0.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, st.s));
+        HIP_CHECK(hipMemcpyAsync(B.g.p, hg.data(), sizeof(double) * n, hipMemcpyHostToDevice, st.s));
+        HIP_CHECK(hipMemcpyAsync(B.status.p, s8.data(), n, hipMemcpyHostToDevice, st.s));
+        if (!items.empty())
+            HIP_CHECK(hipMemcpyAsync(B.items[0].p, items.data(), sizeof(int32_t) * items.size(),
+                                     hipMemcpyHostToDevice, st.s));
+        HIP_CHECK(hipMemcpyAsync(B.counters.p, cnt, sizeof(cnt), hipMemcpyHostToDevice, st.s));
+        HIP_CHECK(hipStreamSynchronize(st.s));
+    } else if (c.rng == COSIMS_RNG_PHILOX) {
+        if ((rc = sample_into(M, P->seed, c.job, n, T, B, st.s, &sms))) return rc;
+    } else {
+        return fail(COSIMS_E_INVALID, "unknown rng");
+    }
     int32_t hits = 0;
     HIP_CHECK(hipMemcpyAsync(&hits, B.counters.p, sizeof(int32_t), hipMemcpyDeviceToHost, st.s));
     HIP_CHECK(hipStreamSynchronize(st.s));

@@ -16,12 +16,15 @@ _LIB = None
 
 PREC_FP64 = 0
 PREC_MIXED = 1
+RNG_PHILOX = 0
+RNG_REFERENCE = 1
 MAX_ROUNDS = 4
 
 EXPORTS = [
     "cosims_params_default", "cosims_params_read", "cosims_molfile_read", "cosims_molfile_free",
     "cosims_mol_create", "cosims_mol_create_prepared", "cosims_mol_destroy", "cosims_mol_upload", "cosims_mol_info_get",
     "cosims_mol_atoms", "cosims_mol_clusters", "cosims_accel", "cosims_potential", "cosims_sample",
+    "cosims_sample_reference",
     "cosims_verlet", "cosims_run", "cosims_strerror", "cosims_last_error", "cosims_device_info",
     "cosims_device_sync",
 ]
@@ -73,8 +76,8 @@ class MolInfo(C.Structure):
 
 class RunCfg(C.Structure):
     _fields_ = [("prec", C.c_int32), ("job", C.c_uint32), ("iter", C.c_int32), ("traj", C.c_int64),
-                ("lanes", C.c_int32), ("wg_size", C.c_int32), ("blocks_per_cu", C.c_int32), ("chi", C.c_void_p),
-                ("status", C.c_void_p), ("g", C.c_void_p)]
+                ("lanes", C.c_int32), ("wg_size", C.c_int32), ("blocks_per_cu", C.c_int32), ("rng", C.c_int32),
+                ("chi", C.c_void_p), ("status", C.c_void_p), ("g", C.c_void_p)]
 
 
 class Ccs(C.Structure):
@@ -110,6 +113,7 @@ def lib():
         L.cosims_potential.argtypes = [vp, C.c_int64, vp, vp]
         L.cosims_sample.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, C.c_int64,
                                     vp, vp, vp, vp]
+        L.cosims_sample_reference.argtypes = [vp, C.c_uint64, C.c_int32, C.c_int64, vp, vp, vp, vp]
         L.cosims_verlet.argtypes = [vp, C.c_int32, C.c_int64, vp, vp, vp, vp, vp, vp, vp]
         L.cosims_run.argtypes = [vp, C.POINTER(Params), C.POINTER(RunCfg), vp, vp, C.POINTER(Ccs)]
         L.cosims_strerror.restype = C.c_char_p
@@ -293,6 +297,12 @@ class Molecule:
         _check(lib().cosims_sample(self._h, seed, job, iteration, first, n, _p(p0), _p(v0), _p(g), _p(hit)))
         return dict(p0=p0, v0=v0, g=g, hit=hit)
 
+    def sample_reference(self, seed, iters, traj):
+        n = iters * traj
+        p0 = np.empty((n, 3)); v0 = np.empty((n, 3)); g = np.empty(n); hit = np.empty(n, np.int32)
+        _check(lib().cosims_sample_reference(self._h, seed, iters, traj, _p(p0), _p(v0), _p(g), _p(hit)))
+        return dict(p0=p0, v0=v0, g=g, hit=hit)
+
     def verlet(self, p0, v0, prec=PREC_FP64):
         p0 = _f64(p0).reshape(-1, 3); v0 = _f64(v0).reshape(-1, 3)
         n = p0.shape[0]
@@ -303,12 +313,12 @@ class Molecule:
         return dict(chi=chi, ok=ok, steps=steps, ergy=ergy, total_steps=tot)
 
     def run(self, prec=PREC_MIXED, job=0, iters=0, traj=0, keep=False, params: Params | None = None,
-            lanes=0, wg_size=0, blocks_per_cu=0):
+            lanes=0, wg_size=0, blocks_per_cu=0, rng=RNG_PHILOX):
         p = params if params is not None else self.params
         it = iters or p.iter
         T = traj or p.traj
         cfg = RunCfg(prec=prec, job=job, iter=it, traj=T, lanes=lanes, wg_size=wg_size,
-                     blocks_per_cu=blocks_per_cu)
+                     blocks_per_cu=blocks_per_cu, rng=rng)
         chi = status = g = None
         if keep:
             chi = np.empty(it * T); status = np.empty(it * T, np.int32); g = np.empty(it * T)

@@ -31,6 +31,9 @@ extern "C" {
 #define COSIMS_E_UNSUPPORTED (-6) /* input the reference handles only by undefined behaviour */
 #define COSIMS_E_NODEVICE (-7)    /* no usable gfx950 device */
 
+#define COSIMS_RNG_PHILOX 0    /* counter-based Philox4x32-10 stream (default) */
+#define COSIMS_RNG_REFERENCE 1 /* the reference's own single-thread stream */
+
 #define COSIMS_PREC_FP64 0  /* all-FP64 force sum, reference expression order */
 #define COSIMS_PREC_MIXED 1 /* FP32 force sum, FP64 Verlet / energy check / CCS */
 
@@ -162,6 +165,14 @@ int cosims_potential(const cosims_mol *m, int64_t n, const double *pos, double *
 int cosims_sample(const cosims_mol *m, uint64_t seed, uint32_t job, uint32_t iteration, int64_t first,
                   int64_t n, double *p0, double *v0, double *g, int32_t *hit);
 
+/* Initial conditions exactly as a single-thread reference run draws them
+ * (monteTraj::runTraj / setUpTraj, src/monteTraj.cpp:174-278, 301-374):
+ * per CCS integral a fresh std::mt19937_64(seed) discarded by the previous
+ * integral's draw count, one rand() stream seeded 1 for the whole run.
+ * Host-side; iters*traj entries each. */
+int cosims_sample_reference(const cosims_mol *m, uint64_t seed, int32_t iters, int64_t traj, double *p0, double *v0,
+                            double *g, int32_t *hit);
+
 /* diffEqSolver::verletVel (src/diffEqSolver.cpp:81-211) for n trajectories,
  * one GPU lane each.  Outputs: chi (rad), ok (return value), steps
  * (p.stepCount), ergy (% energy deviation of the accepted attempt),
@@ -180,6 +191,7 @@ typedef struct cosims_run_cfg {
                               (atom-order sums, bit-identical FP64), 4..64 group kernel */
     int32_t wg_size;       /* 0 => tuned default */
     int32_t blocks_per_cu; /* 0 => occupancy limit */
+    int32_t rng;           /* COSIMS_RNG_*: initial-condition stream */
     /* optional per-trajectory outputs (iter*traj entries each, may be NULL) */
     double *chi;
     int32_t *status;       /* 0 miss, 1 success, 2 failure */

@@ -20,6 +20,7 @@
 #include "impactParam.h"
 #include "maxBoltz.h"
 #include "molecule2.h"
+#include "monteTraj.h"
 #include "myStructs.h"
 #include "PCM_Cluster.h"
 #include "VelocityDistribution.h"
@@ -202,6 +203,38 @@ void ref_verlet(int n, const double *p0, const double *v0, double a, double b, d
         steps[i] = lp.stepCount;
         ergy[i] = lp.maxErgyPctDev;
     }
+}
+
+// Initial conditions from the reference's own setUpTraj, drawn the way a
+// single-thread runTraj draws them (src/monteTraj.cpp:301-374).
+int ref_sample(long long seed, int iters, int traj, double *p0, double *v0, double *g, int *hit, int *tries)
+{
+    static threadSharedParams shared;
+    static monteTraj *mt = nullptr;
+    if (!mt) mt = new monteTraj(*g_mol, shared);
+    srand(1);
+    unsigned restart = 0;
+    for (int it = 0; it < iters; it++) {
+        mt19937_64 generator(seed);
+        generator.discard(it == 0 ? 0 : restart);
+        uniform_real_distribution<double> distr(0.0, 1.0);
+        localThreadParams p;
+        p.ranNumCount = 0;
+        p.missCount = 0;
+        for (int n = 0; n < traj; n++) {
+            vector3D<double> ip, iv;
+            const int o = it * traj + n;
+            int before = p.ranNumCount;
+            bool h = mt->setUpTraj(0, n, ip, iv, p, generator, distr);
+            hit[o] = h ? 1 : 0;
+            g[o] = p.g;
+            tries[o] = (p.ranNumCount - before) / 8;
+            p0[3 * o] = ip.X; p0[3 * o + 1] = ip.Y; p0[3 * o + 2] = ip.Z;
+            v0[3 * o] = iv.X; v0[3 * o + 1] = iv.Y; v0[3 * o + 2] = iv.Z;
+        }
+        restart = p.ranNumCount;
+    }
+    return 0;
 }
 
 } // extern "C"

@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import ctypes as C
 import json
+import math
 import os
 import sys
 import tempfile
@@ -71,6 +72,7 @@ def load_ref():
     L.ref_accel.argtypes = [C.c_int, d, d]
     L.ref_potential.argtypes = [C.c_int, d, d]
     L.ref_verlet.argtypes = [C.c_int, d, d, C.c_double, C.c_double, C.c_double, d, i, i, d]
+    L.ref_sample.argtypes = [C.c_longlong, C.c_int, C.c_int, d, d, d, i, i]
     return L
 
 
@@ -157,6 +159,59 @@ def run_case(L, case, work):
     return out
 
 
+def stream_case(L, work):
+    """The reference's own initial-condition stream (setUpTraj driven as a
+    single-thread runTraj) for the neutral example, 3 integrals x 1500."""
+    argv = ["cosims", "-m", os.path.join(work, "acau.pdb"), "-s", "123456789", "-n", "1",
+            "-i", os.path.join(work, "neutral.in")]
+    open(os.path.join(work, "neutral.in"), "w").write("name\tacau_neutral\n")
+    arr = (C.c_char_p * len(argv))(*[a.encode() for a in argv])
+    n = L.ref_setup(len(argv), arr, work.encode())
+    assert n > 0
+    iters, traj = 3, 1500
+    m = iters * traj
+    p0 = np.empty(3 * m); v0 = np.empty(3 * m); g = np.empty(m)
+    hit = np.empty(m, np.int32); tries = np.empty(m, np.int32)
+    L.ref_sample(123456789, iters, traj, p0, v0, g, hit, tries)
+    P = p0.reshape(-1, 3); V = v0.reshape(-1, 3)
+    h = hit == 1
+    first = [k * traj + i for k in range(iters) for i in range(200)]
+    return dict(case="acau_neutral_stream", mol="acau.pdb", input="name\tacau_neutral\n", seed=123456789,
+                iters=iters, traj=traj, hit="".join(str(int(x)) for x in hit), tries=[int(x) for x in tries],
+                g=ff(g), first=first, p0_first=ff(P[first]), v0_first=ff(V[first]),
+                hit_p0_fsum=[math.fsum(P[h][:, k]) for k in range(3)],
+                hit_v0_fsum=[math.fsum(V[h][:, k]) for k in range(3)])
+
+
+LOGS = [
+    # example directory, log, molecule file, input-file text, atoms.dat
+    ("acau_neutral", "omega_acau_neutral_0.txt", "acau.pdb", "name\tacau_neutral\n", None),
+    ("acau_charged", "omega_acau.txt", "acau.mfj", None, None),
+    ("change_atom_type", "omega_acau_phos_sigma_0.txt", "acau_phos_sigma.pdb", "name\tacau_phos_sigma\n",
+     "atoms.dat"),
+    ("change_atom_type", "omega_acau_phos_sigma_chg_0.txt", "acau_phos_sigma_chg.mfj",
+     "name\tacau_phos_sigma_chg\n", "atoms.dat"),
+]
+
+
+def example_logs():
+    """Recorded results of the reference's example runs (text logs)."""
+    import re
+    out = []
+    for d, log, mol, inp, atoms in LOGS:
+        text = open(f"/root/reference/examples/{d}/{log}").read()
+        seed = int(re.search(r"Mersenne-Twister:\s+(\d+)", text).group(1))
+        iters = int(re.search(r"Number of CCS integrals:\s+(\d+)", text).group(1))
+        traj = int(re.search(r"trajectories per CCS integral:\s+(\d+)", text).group(1))
+        om = re.findall(r"Omega:\s+([\d.]+) \+/- ([\d.]+)", text)
+        fin = {k: re.search(pat + r"\s+([\d.]+)", text).group(1) for k, pat in
+               (("mean", "Mean Cross-section:"), ("sd", "Standard Dev:"), ("wmean", "Weighted CCS Mean:"),
+                ("werr", "Weighted Error:"), ("max_ergy", "Max Energy Deviation:"))}
+        out.append(dict(log=f"examples/{d}/{log}", mol=mol, input=inp, atoms_dat=atoms, seed=seed, iters=iters,
+                        traj=traj, omega=[o[0] for o in om], omega_err=[o[1] for o in om], final=fin))
+    return out
+
+
 def main(names=None):
     if not os.path.exists(REFLIB):
         raise SystemExit("oracle/_ref/libcosims_ref.so missing: run `make -C oracle ref` first")
@@ -164,9 +219,14 @@ def main(names=None):
         path = os.path.join(HERE, key + ".mfj")
         if not os.path.exists(path):
             synth.write_mfj(path, key, synth.config_ion(key))
-    for case in CASES:
-        if names and case["name"] not in names:
-            continue
+    if not names or "logs" in names:
+        with open(os.path.join(HERE, "example_logs.json"), "w") as f:
+            json.dump(example_logs(), f, indent=1)
+        print("example_logs written", flush=True)
+    jobs = [c for c in CASES if not names or c["name"] in names]
+    if not names or "stream" in names:
+        jobs.append(dict(name="acau_neutral_stream", stream=True))
+    for case in jobs:
         # one process per case: the reference keeps its state in globals
         pid = os.fork()
         if pid == 0:
@@ -179,7 +239,7 @@ def main(names=None):
             saved = os.dup(1)
             os.dup2(devnull, 1)
             try:
-                res = run_case(L, case, work)
+                res = stream_case(L, work) if case.get("stream") else run_case(L, case, work)
             finally:
                 os.dup2(saved, 1)
             with open(os.path.join(HERE, case["name"] + ".json"), "w") as f:

@@ -134,3 +134,24 @@ def test_unknown_atom_type_is_rejected(tmp_path):
     f.write_text("x\n1\n2\nang\ncalc\n1.0\n0 0 0 12 0.5\n1.5 0 0 99 0.5\n")
     with pytest.raises(E.CosimsError, match="type not in the force field"):
         E.read_molfile(str(f))
+
+
+def test_reference_stream_matches_setUpTraj():
+    """The engine's reference-stream sampler against the reference's own
+    monteTraj::setUpTraj driven as a single-thread runTraj (golden vectors):
+    identical acceptance (draw counts), speeds and start points."""
+    import math
+    s = golden("acau_neutral_stream")
+    g = golden("acau_neutral")
+    m = _engine_setup(g)
+    r = m.sample_reference(s["seed"], s["iters"], s["traj"])
+    assert "".join(str(int(x)) for x in r["hit"]) == s["hit"]
+    assert np.array_equal(r["g"], s["g"])
+    first = np.array(s["first"])
+    h = r["hit"] == 1
+    # misses: the reference leaves the previous start point in place; only hits are compared
+    hf = h[first]
+    assert np.array_equal(r["p0"][first][hf], np.array(s["p0_first"]).reshape(-1, 3)[hf])
+    assert np.array_equal(r["v0"][first][hf], np.array(s["v0_first"]).reshape(-1, 3)[hf])
+    assert [math.fsum(r["p0"][h][:, k]) for k in range(3)] == s["hit_p0_fsum"]
+    assert [math.fsum(r["v0"][h][:, k]) for k in range(3)] == s["hit_v0_fsum"]
