@@ -86,6 +86,7 @@ struct SampleArgs {
     int32_t *hit;                   // optional
     int32_t *items;                 // optional compacted hit list
     int32_t *n_items;
+    int32_t item_base;              // added to the indices written to items
 };
 
 // setUpTraj: rejection sampling of (g, phi, theta, gamma, b) and projection
@@ -160,7 +161,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A)
             int base = 0;
             if (lane == leader) base = atomicAdd(A.n_items, __popcll(mask));
             base = __shfl(base, leader);
-            if (h) A.items[base + __popcll(mask & ((1ull << lane) - 1))] = (int32_t)j;
+            if (h) A.items[base + __popcll(mask & ((1ull << lane) - 1))] = A.item_base + (int32_t)j;
         }
     }
 }
@@ -516,6 +517,116 @@ struct Stream {
 
 }  // namespace
 
+// ------------------------------------------------- boundary marches ---
+struct MarchArgs {
+    MolDev m;          // atoms (a64, reference order), n, charged, dipole_scalar
+    const March *w;
+    double mk;
+    int64_t max_points;
+    MarchHit *out;
+};
+
+// One wavefront per march (impactParam::setBoundry, src/impactParam.cpp:
+// 602-736): 64 consecutive points per pass, lane j rebuilding p_{k+j} by the
+// same sequential FP64 additions as the host loop and evaluating getPotential
+// there in reference atom order (scalar atom loads shared by the wave); the
+// first lane that crosses (or passes the axis) ends the march.
+__global__ void __launch_bounds__(64) k_march(MarchArgs A)
+{
+    const March &w = A.w[blockIdx.x];
+    const int lane = threadIdx.x;
+    const double e0 = potential(A.m, w.start[0], w.start[1], w.start[2]);
+    const double sl = (e0 + A.mk < 0) ? -1.0 : 1.0;
+    const double vx = w.dir[0] * sl, vy = w.dir[1] * sl, vz = w.dir[2] * sl;
+    const double sx = w.start[0], sy = w.start[1];
+    double bx = w.start[0], by = w.start[1], bz = w.start[2];  // point kb
+    double qx = w.prev0[0], qy = w.prev0[1], qz = w.prev0[2];  // point kb - 1
+    int64_t kb = 0;
+    if (w.first) {
+        bx = bx + vx; by = by + vy; bz = bz + vz;
+        kb = 1;
+    }
+    for (; kb < A.max_points; kb += 64) {
+        double px = bx, py = by, pz = bz;
+        for (int q = 0; q < lane; q++) {
+            px = px + vx; py = py + vy; pz = pz + vz;
+        }
+        const double en = potential(A.m, px, py, pz);
+        const bool inr = kb + lane < A.max_points;
+        const bool cross = inr && (en + A.mk) * sl < 0;
+        const bool axis = inr && w.axis && ((sx == 0 && sy / py <= 0) || (sy == 0 && sx / px <= 0) ||
+                                            (sx / px <= 0 && sy / py <= 0));
+        const int src = (lane + 63) & 63;
+        const double nx = __shfl(px, src), ny = __shfl(py, src), nz = __shfl(pz, src);
+        const unsigned long long mask = __ballot(cross || axis);
+        if (mask) {
+            const int k = __ffsll(mask) - 1;
+            if (lane == k) {
+                MarchHit &h = A.out[blockIdx.x];
+                h.found = cross ? 1 : 0;
+                h.p[0] = px; h.p[1] = py; h.p[2] = pz;
+                h.prev[0] = k ? nx : qx; h.prev[1] = k ? ny : qy; h.prev[2] = k ? nz : qz;
+            }
+            return;
+        }
+        qx = __shfl(px, 63); qy = __shfl(py, 63); qz = __shfl(pz, 63);
+        bx = qx + vx; by = qy + vy; bz = qz + vz;
+    }
+    if (lane == 0) A.out[blockIdx.x].found = -1;
+}
+
+// Boundary marches on the GPU for prepare(): the molecule's atoms are
+// uploaded once per setup, one k_march wavefront per march.
+struct GpuMarcher : BoundaryMarcher {
+    void *atoms = nullptr;
+    ~GpuMarcher() override
+    {
+        if (atoms) (void)hipFree(atoms);
+    }
+    int run(const Prepared &p, double mk, const std::vector<March> &w, std::vector<MarchHit> &out,
+            std::string &err) override
+    {
+        int rc = run_(p, mk, w, out);
+        if (rc) err = g_err;
+        return rc;
+    }
+    int run_(const Prepared &p, double mk, const std::vector<March> &w, std::vector<MarchHit> &out)
+    {
+        if (!atoms) {
+            Device d;
+            if (int rc = device_init(d)) return rc;
+            std::vector<AtomF64> a(p.n);
+            for (int i = 0; i < p.n; i++) {
+                a[i].x = p.xyz[3 * i]; a[i].y = p.xyz[3 * i + 1]; a[i].z = p.xyz[3 * i + 2];
+                a[i].q = p.q[i]; a[i].s6 = p.sig6[i]; a[i].e24 = p.eps24m[i]; a[i].e4 = p.eps4[i]; a[i].pad = 0;
+            }
+            HIP_CHECK(hipMalloc(&atoms, sizeof(AtomF64) * p.n));
+            HIP_CHECK(hipMemcpy(atoms, a.data(), sizeof(AtomF64) * p.n, hipMemcpyHostToDevice));
+        }
+        Stream st;
+        HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+        DBuf dw, dh;
+        HIP_CHECK(dw.alloc(sizeof(March) * w.size()));
+        HIP_CHECK(dh.alloc(sizeof(MarchHit) * w.size()));
+        HIP_CHECK(hipMemcpyAsync(dw.p, w.data(), sizeof(March) * w.size(), hipMemcpyHostToDevice, st.s));
+        MarchArgs A{};
+        A.m.n = p.n;
+        A.m.charged = p.charged ? 1 : 0;
+        A.m.dipole_scalar = p.dipole_scalar;
+        A.m.a64 = (const AtomF64 *)atoms;
+        A.w = dw.as<March>();
+        A.mk = mk;
+        A.max_points = kMaxMarchPoints;
+        A.out = dh.as<MarchHit>();
+        hipLaunchKernelGGL(k_march, dim3((unsigned)w.size()), dim3(64), 0, st.s, A);
+        HIP_CHECK(hipGetLastError());
+        out.resize(w.size());
+        HIP_CHECK(hipMemcpyAsync(out.data(), dh.p, sizeof(MarchHit) * w.size(), hipMemcpyDeviceToHost, st.s));
+        HIP_CHECK(hipStreamSynchronize(st.s));
+        return 0;
+    }
+};
+
 int cosims::set_last_error(int code, const std::string &msg) { return fail(code, msg); }
 
 // -------------------------------------------------------------- handle ---
@@ -532,6 +643,8 @@ struct cosims_mol {
     mutable void *dmem = nullptr;
     mutable const double *soa = nullptr;  // 7 FP64 arrays of npad: x y z q s6 e24 e4
     mutable int npad = 0;
+    mutable MolRec rec{};                 // table record of this molecule (device pointers)
+    mutable const MolRec *rec_dev = nullptr;  // the same record in device memory
     // autotuned launch shape per precision: lanes per trajectory, workgroup size
     mutable std::mutex tune_mu;
     mutable int tuned_lanes[2] = {0, 0}, tuned_wg[2] = {0, 0};
@@ -585,7 +698,8 @@ int upload(const cosims_mol *M)
     size_t o64 = 0, o32 = align_up(o64 + sizeof(AtomF64) * n), oc64 = align_up(o32 + sizeof(AtomF32) * n),
            oc32 = align_up(oc64 + sizeof(AtomF64) * n), oh64 = align_up(oc32 + sizeof(AtomF32) * n),
            oh32 = align_up(oh64 + sizeof(ClusterF64) * ncl), osoa = align_up(oh32 + sizeof(ClusterF32) * ncl),
-           total = align_up(osoa + sizeof(double) * 7 * npad);
+           oxq = align_up(osoa + sizeof(double) * 7 * npad), ose = align_up(oxq + sizeof(float4) * npad),
+           orec = align_up(ose + sizeof(float2) * npad), total = align_up(orec + sizeof(MolRec));
     std::vector<char> host(total, 0);
     auto *a64 = (AtomF64 *)(host.data() + o64);
     auto *a32 = (AtomF32 *)(host.data() + o32);
@@ -620,8 +734,14 @@ int upload(const cosims_mol *M)
         soa[5 * npad + i] = real ? p.eps24m[i] : 0.0;
         soa[6 * npad + i] = real ? p.eps4[i] : 0.0;
     }
+    // FP32 SoA of the same (x, y, z, q | sigma^6, 24 eps/mu)
+    auto *xq = (float4 *)(host.data() + oxq);
+    auto *se = (float2 *)(host.data() + ose);
+    for (int i = 0; i < npad; i++) {
+        xq[i] = make_float4((float)soa[i], (float)soa[npad + i], (float)soa[2 * npad + i], (float)soa[3 * npad + i]);
+        se[i] = make_float2((float)soa[4 * npad + i], (float)soa[5 * npad + i]);
+    }
     HIP_CHECK(hipMalloc(&M->dmem, total));
-    HIP_CHECK(hipMemcpy(M->dmem, host.data(), total, hipMemcpyHostToDevice));
     char *d = (char *)M->dmem;
     MolDev &m = M->md;
     m.n = n; m.ncl = ncl; m.force_type = p.force_type; m.charged = p.charged ? 1 : 0;
@@ -633,6 +753,16 @@ int upload(const cosims_mol *M)
     m.h64 = (const ClusterF64 *)(d + oh64); m.h32 = (const ClusterF32 *)(d + oh32);
     M->soa = (const double *)(d + osoa);
     M->npad = npad;
+    MolRec &r = M->rec;
+    r.m = m;
+    r.xq = (const float4 *)(d + oxq);
+    r.se = (const float2 *)(d + ose);
+    r.soa = M->soa;
+    r.npad = npad;
+    r.dt0 = M->params.dt;
+    std::memcpy(host.data() + orec, &r, sizeof(MolRec));
+    M->rec_dev = (const MolRec *)(d + orec);
+    HIP_CHECK(hipMemcpy(M->dmem, host.data(), total, hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -723,7 +853,14 @@ int cosims_mol_create(const cosims_params *p, const cosims_atoms *a, cosims_mol 
     std::unique_ptr<cosims_mol> M(new cosims_mol());
     M->params = to_params(p);
     std::string err;
-    int rc = prepare(M->params, a->n, a->xyz, a->mass, a->sigma, a->epsilon, a->charge, a->sigma_max, M->prep, err);
+    int rc;
+    if (p->setup_gpu) {
+        GpuMarcher gm;
+        rc = prepare(M->params, a->n, a->xyz, a->mass, a->sigma, a->epsilon, a->charge, a->sigma_max, M->prep, err,
+                     &gm);
+    } else {
+        rc = prepare(M->params, a->n, a->xyz, a->mass, a->sigma, a->epsilon, a->charge, a->sigma_max, M->prep, err);
+    }
     if (rc) return fail(rc, err);
     return finish_create(M.release(), out, t0);
 }
@@ -1010,6 +1147,7 @@ namespace {
 
 // ------------------------------------------------- group-mode launches ---
 using GrpFn = void (*)(GroupArgs);
+using MultiFn = void (*)(MultiArgs);
 
 template <int P, int FT>
 GrpFn grp_fn_l(int L)
@@ -1031,68 +1169,109 @@ GrpFn group_kernel(int prec, int ft, int L)
     return nullptr;
 }
 
+template <int P, int FT>
+MultiFn multi_fn_l(int L)
+{
+    switch (L) {
+    case 4: return k_traj_multi<P, FT, 4>;
+    case 8: return k_traj_multi<P, FT, 8>;
+    case 16: return k_traj_multi<P, FT, 16>;
+    case 32: return k_traj_multi<P, FT, 32>;
+    case 64: return k_traj_multi<P, FT, 64>;
+    default: return nullptr;
+    }
+}
+
+template <int P>
+MultiFn multi_fn(int ft, int L)
+{
+    switch (ft) {
+    case 1: return multi_fn_l<P, 1>(L);
+    case 2: return multi_fn_l<P, 2>(L);
+    case 3: return multi_fn_l<P, 3>(L);
+    case 4: return multi_fn_l<P, 4>(L);
+    case 5: return multi_fn_l<P, 5>(L);
+    case 6: return multi_fn_l<P, 6>(L);
+    default: return nullptr;
+    }
+}
+
+MultiFn multi_kernel(int prec, int ft, int L) { return prec == COSIMS_PREC_FP64 ? multi_fn<0>(ft, L) : multi_fn<1>(ft, L); }
+
 size_t group_lds_bytes(const cosims_mol *M, int prec)
 {
     return (size_t)M->npad * (prec == COSIMS_PREC_FP64 ? 6 * sizeof(double) : sizeof(float4) + sizeof(float2));
 }
 
-bool group_mode_possible(const cosims_mol *M, int prec)
+// The LDS-staged kernel serves the exact force types when the molecule fits
+// one workgroup's LDS; everything else runs the table kernel.
+bool lds_mode(const cosims_mol *M, int prec)
 {
     const int ft = M->prep.force_type;
     return (ft == 4 || ft == 6) && group_lds_bytes(M, prec) <= M->dev.lds_max;
 }
 
+// Work units one lane of a group walks per force evaluation: atoms, or
+// clusters for the clustered types.
+int group_units(const Prepared &p)
+{
+    const int ft = p.force_type;
+    return (ft == 4 || ft == 6) ? p.n : p.ncl;
+}
+
 struct GroupRun {
     double ms = 0;
-    int64_t steps = 0, evals = 0, pots = 0;
+    int64_t steps = 0, evals = 0, pots = 0, pairs = 0;
     int grid = 0;
 };
 
-// One persistent launch over the item list B.items[0] (count n_items).
-int run_group(const cosims_mol *M, int prec, int n_items, TrajBufs &B, hipStream_t s, int L, int wg, int bpc,
-              GroupRun &out)
+TrajIO traj_io(TrajBufs &B, const int32_t *items, int n_items)
 {
-    GrpFn fn = group_kernel(prec, M->prep.force_type, L);
-    if (!fn) return fail(COSIMS_E_INVALID, "no group kernel for this force type / lane count");
-    const size_t lds = group_lds_bytes(M, prec);
-    HIP_CHECK(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    TrajIO io{};
+    io.items = items;
+    io.n_items = n_items;
+    io.counter = B.counters.as<int32_t>() + 1;
+    io.p0 = B.p0.as<double>(); io.v0 = B.v0.as<double>();
+    io.status = B.status.as<int8_t>(); io.chi = B.chi.as<double>(); io.ergy = B.ergy.as<double>();
+    io.cchi = B.exitv2.as<double>();  // the lane kernel's exit buffer holds cos(chi) here
+    io.steps = B.steps.as<int32_t>(); io.totsteps = B.tot.as<int32_t>();
+    io.max_steps = kMaxStepsPerAttempt;
+    io.evals = B.evals.as<unsigned long long>();
+    return io;
+}
+
+// Grid of a persistent group launch: occupancy-limited, never more
+// workgroups than the items can fill.
+int64_t group_grid(const void *fn, int wg, size_t lds, int bpc, int cus, int64_t n_items, int L)
+{
     int occ = bpc;
-    if (occ <= 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)fn, wg, lds) != hipSuccess ||
-                     occ <= 0))
+    if (occ <= 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, wg, lds) != hipSuccess || occ <= 0))
         occ = 1;
     const int groups_per_wg = wg / L;
-    int64_t grid = (int64_t)occ * M->dev.cus;
-    grid = std::max<int64_t>(1, std::min<int64_t>(grid, (n_items + groups_per_wg - 1) / groups_per_wg));
+    int64_t grid = (int64_t)occ * cus;
+    return std::max<int64_t>(1, std::min<int64_t>(grid, (n_items + groups_per_wg - 1) / groups_per_wg));
+}
+
+// One persistent launch of fn(args) with HIP-event timing on stream s, then
+// acos of the accepted exits and the launch's counters.
+template <class Args>
+int timed_group_launch(void (*fn)(Args), const Args &args, int64_t grid, int wg, size_t lds, hipStream_t s,
+                       TrajBufs &B, int n_items, GroupRun &out)
+{
     int32_t *ctr = B.counters.as<int32_t>();
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ctr + 1, 2, 0);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, B.evals.as<int32_t>(), 4 * COSIMS_MAX_ROUNDS, 0);
-    GroupArgs G{};
-    G.m = M->md;
-    G.npad = M->npad;
-    const double *S = M->soa;
-    G.sx = S; G.sy = S + M->npad; G.sz = S + 2 * M->npad; G.sq = S + 3 * M->npad;
-    G.s6 = S + 4 * M->npad; G.se24 = S + 5 * M->npad; G.se4 = S + 6 * M->npad;
-    G.items = B.items[0].as<int32_t>();
-    G.n_items = n_items;
-    G.counter = ctr + 1;
-    G.p0 = B.p0.as<double>(); G.v0 = B.v0.as<double>();
-    G.status = B.status.as<int8_t>(); G.chi = B.chi.as<double>(); G.ergy = B.ergy.as<double>();
-    G.cchi = B.exitv2.as<double>();  // the lane kernel's exit buffer holds cos(chi) here
-    G.steps = B.steps.as<int32_t>(); G.totsteps = B.tot.as<int32_t>();
-    G.dt0 = M->params.dt;
-    G.max_steps = kMaxStepsPerAttempt;
-    G.evals = B.evals.as<unsigned long long>();
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, s));
-    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(wg), lds, s, G);
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(wg), lds, s, args);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(e1, s));
-    hipLaunchKernelGGL(k_chi, dim3((unsigned)((n_items + 255) / 256)), dim3(256), 0, s, G.items, n_items, G.status,
-                       G.cchi, G.chi);
+    hipLaunchKernelGGL(k_chi, dim3((unsigned)((n_items + 255) / 256)), dim3(256), 0, s, args.io.items, n_items,
+                       args.io.status, args.io.cchi, args.io.chi);
     HIP_CHECK(hipGetLastError());
-    unsigned long long ev[3];
+    unsigned long long ev[4];
     HIP_CHECK(hipMemcpyAsync(ev, B.evals.p, sizeof(ev), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     float ms = 0;
@@ -1103,8 +1282,45 @@ int run_group(const cosims_mol *M, int prec, int n_items, TrajBufs &B, hipStream
     out.steps = (int64_t)ev[0];
     out.evals = (int64_t)ev[1];
     out.pots = (int64_t)ev[2];
+    out.pairs = (int64_t)ev[3];
     out.grid = (int)grid;
     return 0;
+}
+
+// Table-kernel launch over the item list items[0..n_items): mols is a device
+// array of MolRec, tmol the molecule of each trajectory (NULL: molecule 0).
+int run_multi(const MolRec *mols, const int32_t *tmol, int ft, int prec, int cus, const int32_t *items, int n_items,
+              TrajBufs &B, hipStream_t s, int L, int wg, int bpc, GroupRun &out)
+{
+    MultiFn fn = multi_kernel(prec, ft, L);
+    if (!fn) return fail(COSIMS_E_INVALID, "no table kernel for this force type / lane count");
+    MultiArgs G{};
+    G.io = traj_io(B, items, n_items);
+    G.mols = mols;
+    G.tmol = tmol;
+    const int64_t grid = group_grid((const void *)fn, wg, 0, bpc, cus, n_items, L);
+    return timed_group_launch(fn, G, grid, wg, 0, s, B, n_items, out);
+}
+
+// One persistent group launch of one molecule over the item list B.items[0].
+int run_group(const cosims_mol *M, int prec, int n_items, TrajBufs &B, hipStream_t s, int L, int wg, int bpc,
+              GroupRun &out)
+{
+    if (!lds_mode(M, prec))
+        return run_multi(M->rec_dev, nullptr, M->prep.force_type, prec, M->dev.cus, B.items[0].as<int32_t>(), n_items,
+                         B, s, L, wg, bpc, out);
+    GrpFn fn = group_kernel(prec, M->prep.force_type, L);
+    if (!fn) return fail(COSIMS_E_INVALID, "no group kernel for this force type / lane count");
+    const size_t lds = group_lds_bytes(M, prec);
+    HIP_CHECK(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    GroupArgs G{};
+    G.io = traj_io(B, B.items[0].as<int32_t>(), n_items);
+    G.m = M->md;
+    G.npad = M->npad;
+    G.soa = M->soa;
+    G.dt0 = M->params.dt;
+    const int64_t grid = group_grid((const void *)fn, wg, lds, bpc, M->dev.cus, n_items, L);
+    return timed_group_launch(fn, G, grid, wg, lds, s, B, n_items, out);
 }
 
 // k_sample into B with hit compaction into B.items[0] (count in counters[0]).
@@ -1130,10 +1346,88 @@ int sample_into(const cosims_mol *M, uint64_t seed, uint32_t job, int64_t n, int
     return 0;
 }
 
-// Autotuner: lanes per trajectory x workgroup size, timed on the molecule
-// itself (fixed Philox stream).  Stage 1 screens every shape on a small
-// batch; stage 2 re-times the three best on a batch large enough that the
-// end-of-launch tail is amortised, and keeps the fastest per trajectory.
+// Per-integral CCS and the final statistics of monteTraj::runTraj from the
+// per-integral sums of k_reduce (total, total^2, max ergy, steps, hits,
+// fails) and the trajectory status codes.
+void finish_ccs(const int8_t *status, const double *sums, int iters, int64_t T, int64_t print_rate, double *omega,
+                double *omega_err, cosims_ccs *out)
+{
+    // Denominator currSuccTraj of one OpenMP thread (src/monteTraj.cpp:456-521):
+    // failures (a miss repeats the previous success flag) counted since the
+    // last progress update every print_rate trajectories.
+    const int64_t upd = print_rate > 0 ? print_rate : std::max<int64_t>(1, (int64_t)(T / 5.0));
+    double mx = 0;
+    for (int k = 0; k < iters; k++) {
+        const int8_t *st8 = status + (int64_t)k * T;
+        bool succ = false;
+        int64_t orbit = 0;
+        for (int64_t i = 0; i < T; i++) {
+            if (st8[i] != ST_MISS) succ = (st8[i] == ST_OK);
+            if (!succ) orbit++;
+            if ((i + 1) % upd == 0) orbit = 0;
+        }
+        const double d = (double)(T - orbit);
+        const double tot = sums[6 * k], tot2 = sums[6 * k + 1];
+        omega[k] = tot / d;
+        omega_err[k] = std::sqrt((tot2 / d - (tot / d) * (tot / d)) / d);
+        mx = std::max(mx, sums[6 * k + 2]);
+        out->total_steps += (int64_t)sums[6 * k + 3];
+        out->n_hit += (int64_t)sums[6 * k + 4];
+        out->n_fail += (int64_t)sums[6 * k + 5];
+    }
+    // final statistics (src/monteTraj.cpp:412-425)
+    double avg = 0, sd = 0, we = 0, wm = 0;
+    for (int k = 0; k < iters; k++) avg += omega[k];
+    avg = avg / (double)iters;
+    for (int k = 0; k < iters; k++) {
+        sd += (omega[k] - avg) * (omega[k] - avg);
+        we += 1.0 / (omega_err[k] * omega_err[k]);
+        wm += omega[k] / (omega_err[k] * omega_err[k]);
+    }
+    wm /= we;
+    out->mean = avg;
+    out->sd = std::sqrt(sd / (double)iters);
+    out->wmean = wm;
+    out->werr = 1.0 / std::sqrt(we);
+    out->max_ergy = mx;
+    out->n_traj = (int64_t)iters * T;
+}
+
+// Launch-shape search shared by the single-molecule and batch tuners:
+// every lanes-per-trajectory L (the atom tile each lane strides over) x
+// workgroup size W (W/L trajectories per workgroup) is timed on a small
+// item set; the three fastest are re-timed on a set large enough that the
+// end-of-launch tail is amortised.  run(set, L, W, ms) times one launch.
+template <class RunFn>
+int tune_shapes(int units, RunFn run, int &L_out, int &W_out)
+{
+    struct Cand {
+        int L, W;
+        double ms;
+    };
+    std::vector<Cand> cands;
+    const int Ls[] = {4, 8, 16, 32, 64};
+    const int Ws[] = {256, 512, 1024};
+    for (int L : Ls) {
+        if (L > 4 && units < 8 * L) continue;  // too few atoms (clusters) per lane
+        for (int W : Ws) {
+            double ms = 0;
+            if (int rc = run(0, L, W, ms)) return rc;
+            cands.push_back({L, W, ms});
+        }
+    }
+    std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
+    if (cands.size() > 3) cands.resize(3);
+    for (Cand &c : cands)
+        if (int rc = run(1, c.L, c.W, c.ms)) return rc;
+    std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
+    L_out = cands[0].L;
+    W_out = cands[0].W;
+    return 0;
+}
+
+// Single-molecule autotuner, timed on the molecule itself (fixed Philox
+// stream); the result is cached on the handle per precision.
 int autotune(const cosims_mol *M, int prec, int &L_out, int &wg_out)
 {
     std::lock_guard<std::mutex> lk(M->tune_mu);
@@ -1145,43 +1439,194 @@ int autotune(const cosims_mol *M, int prec, int &L_out, int &wg_out)
     }
     Stream st;
     HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
-    const int64_t n1 = 16384, n2 = std::max<int64_t>(n1, (int64_t)M->dev.cus * 512);
+    const int64_t nset[2] = {16384, std::max<int64_t>(16384, (int64_t)M->dev.cus * 512)};
     TrajBufs B;
-    if (int rc = B.alloc(n2)) return rc;
-    struct Cand {
-        int L, W;
-        double ms;
-    };
-    std::vector<Cand> cands;
-    const int Ls[] = {4, 8, 16, 32, 64};
-    const int Ws[] = {256, 512, 1024};
-    float sms = 0;
-    if (int rc = sample_into(M, 0x5eed5eedULL, 0xfffffffeu, n1, n1, B, st.s, &sms)) return rc;
+    if (int rc = B.alloc(nset[1])) return rc;
+    int cur = -1;
     int32_t hits = 0;
-    HIP_CHECK(hipMemcpy(&hits, B.counters.p, sizeof(int32_t), hipMemcpyDeviceToHost));
-    for (int L : Ls) {
-        if (L > 4 && M->prep.n < 8 * L) continue;  // too few atoms per lane
-        for (int W : Ws) {
-            GroupRun R;
-            if (int rc = run_group(M, prec, hits, B, st.s, L, W, 0, R)) return rc;
-            cands.push_back({L, W, R.ms});
+    auto run = [&](int set, int L, int W, double &ms) -> int {
+        if (set != cur) {
+            float sms = 0;
+            if (int rc = sample_into(M, 0x5eed5eedULL, 0xfffffffeu - set, nset[set], nset[set], B, st.s, &sms))
+                return rc;
+            HIP_CHECK(hipMemcpy(&hits, B.counters.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+            cur = set;
+        }
+        GroupRun R;
+        if (int rc = run_group(M, prec, hits, B, st.s, L, W, 0, R)) return rc;
+        ms = R.ms;
+        return 0;
+    };
+    if (int rc = tune_shapes(group_units(M->prep), run, L_out, wg_out)) return rc;
+    M->tuned_lanes[pi] = L_out;
+    M->tuned_wg[pi] = wg_out;
+    return 0;
+}
+
+}  // namespace
+
+namespace {
+
+// ------------------------------------------------------------- batches ---
+// A batch laid out on the device: per-trajectory buffers of all jobs back
+// to back, the molecule of each trajectory, the MolRec table, and the work
+// list ordered force type by force type, larger molecules first.
+struct BatchPlan {
+    struct Job {
+        const cosims_mol *mol;
+        uint64_t seed;
+        uint32_t job;
+        int iters;
+        int64_t T, n, off;  // trajectories per integral, total, first slot
+        int ft, units;
+    };
+    std::vector<Job> jobs;
+    std::vector<int> order;              // job indices in work-list order
+    struct Seg {
+        int ft;
+        int64_t begin, end;              // work-list range of this force type
+    };
+    std::vector<Seg> segs;
+    int64_t n = 0;
+    int cus = 0;
+    TrajBufs B;
+    DBuf tmol, table;
+    float sample_ms = 0;
+};
+
+// Validates the jobs and computes the layout (host only).
+int plan_batch(const cosims_batch_job *jobs, int32_t njobs, int64_t cap_per_job, BatchPlan &P)
+{
+    if (!jobs || njobs <= 0) return fail(COSIMS_E_INVALID, "cosims_run_batch: no jobs");
+    P.jobs.clear();
+    P.n = 0;
+    for (int k = 0; k < njobs; k++) {
+        const cosims_batch_job &J = jobs[k];
+        if (!J.mol) return fail(COSIMS_E_INVALID, "cosims_run_batch: job without molecule");
+        if (int rc = ensure_device(J.mol)) return rc;
+        BatchPlan::Job b{};
+        b.mol = J.mol;
+        b.seed = J.seed;
+        b.job = J.job;
+        b.iters = J.iter > 0 ? J.iter : J.mol->params.iter;
+        b.T = J.traj > 0 ? J.traj : J.mol->params.traj;
+        if (b.iters <= 0 || b.T <= 0) return fail(COSIMS_E_INVALID, "cosims_run_batch: job without trajectories");
+        if (cap_per_job > 0) {  // tuning subsets: one integral of at most cap trajectories
+            b.iters = 1;
+            b.T = std::min<int64_t>(b.T, cap_per_job);
+        }
+        b.n = (int64_t)b.iters * b.T;
+        b.off = P.n;
+        b.ft = J.mol->prep.force_type;
+        b.units = group_units(J.mol->prep);
+        P.n += b.n;
+        if (P.n > INT32_MAX / 4) return fail(COSIMS_E_INVALID, "cosims_run_batch: too many trajectories for one batch");
+        P.jobs.push_back(b);
+    }
+    P.cus = jobs[0].mol->dev.cus;
+    for (const auto &J : P.jobs)
+        if (J.mol->dev.id != jobs[0].mol->dev.id)
+            return fail(COSIMS_E_INVALID, "cosims_run_batch: molecules on different devices");
+    P.order.resize(njobs);
+    for (int k = 0; k < njobs; k++) P.order[k] = k;
+    std::stable_sort(P.order.begin(), P.order.end(), [&](int a, int b) {
+        const auto &x = P.jobs[a], &y = P.jobs[b];
+        return x.ft != y.ft ? x.ft < y.ft : x.units > y.units;
+    });
+    return 0;
+}
+
+// Device buffers, molecule table, and the sampled work list of a plan.
+int stage_batch(BatchPlan &P, hipStream_t s)
+{
+    const int nj = (int)P.jobs.size();
+    if (int rc = P.B.alloc(P.n)) return rc;
+    HIP_CHECK(P.tmol.alloc(sizeof(int32_t) * P.n));
+    HIP_CHECK(P.table.alloc(sizeof(MolRec) * nj));
+    std::vector<MolRec> recs(nj);
+    for (int k = 0; k < nj; k++) recs[k] = P.jobs[k].mol->rec;
+    HIP_CHECK(hipMemcpyAsync(P.table.p, recs.data(), sizeof(MolRec) * nj, hipMemcpyHostToDevice, s));
+    int32_t *ctr = P.B.counters.as<int32_t>();
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ctr, 64, 0);
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, s));
+    P.segs.clear();
+    int64_t seg_begin = 0;
+    for (size_t i = 0; i < P.order.size(); i++) {
+        const int k = P.order[i];
+        const BatchPlan::Job &J = P.jobs[k];
+        const unsigned nb = (unsigned)((J.n + 255) / 256);
+        hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, s, P.tmol.as<int32_t>() + J.off, J.n, (int32_t)k);
+        SampleArgs A = sample_args(J.mol, J.seed, J.job);
+        A.first = 0; A.n = J.n; A.per_iter = J.T;
+        A.p0 = P.B.p0.as<double>() + 3 * J.off; A.v0 = P.B.v0.as<double>() + 3 * J.off;
+        A.g = P.B.g.as<double>() + J.off; A.status = P.B.status.as<int8_t>() + J.off;
+        A.items = P.B.items[0].as<int32_t>(); A.n_items = ctr; A.item_base = (int32_t)J.off;
+        hipLaunchKernelGGL(k_sample, dim3(nb), dim3(256), 0, s, A);
+        HIP_CHECK(hipGetLastError());
+        const bool last_of_type = i + 1 == P.order.size() || P.jobs[P.order[i + 1]].ft != J.ft;
+        if (last_of_type) {
+            int32_t end = 0;
+            HIP_CHECK(hipMemcpyAsync(&end, ctr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            P.segs.push_back({J.ft, seg_begin, (int64_t)end});
+            seg_begin = end;
         }
     }
-    std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
-    if (cands.size() > 3) cands.resize(3);
-    if (int rc = sample_into(M, 0x5eed5eedULL, 0xfffffffdu, n2, n2, B, st.s, &sms)) return rc;
-    HIP_CHECK(hipMemcpy(&hits, B.counters.p, sizeof(int32_t), hipMemcpyDeviceToHost));
-    for (Cand &c : cands) {
-        GroupRun R;
-        if (int rc = run_group(M, prec, hits, B, st.s, c.L, c.W, 0, R)) return rc;
-        c.ms = R.ms;
-    }
-    std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
-    M->tuned_lanes[pi] = cands[0].L;
-    M->tuned_wg[pi] = cands[0].W;
-    L_out = cands[0].L;
-    wg_out = cands[0].W;
+    HIP_CHECK(hipEventRecord(e1, s));
+    HIP_CHECK(hipEventSynchronize(e1));
+    HIP_CHECK(hipEventElapsedTime(&P.sample_ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return 0;
+}
+
+// One table-kernel launch per force type over the staged work list.
+int launch_batch(BatchPlan &P, int prec, int L, int wg, int bpc, hipStream_t s, GroupRun &tot, int &launches)
+{
+    launches = 0;
+    for (const auto &g : P.segs) {
+        const int cnt = (int)(g.end - g.begin);
+        if (cnt <= 0) continue;
+        GroupRun R;
+        if (int rc = run_multi(P.table.as<MolRec>(), P.tmol.as<int32_t>(), g.ft, prec, P.cus,
+                               P.B.items[0].as<int32_t>() + g.begin, cnt, P.B, s, L, wg, bpc, R))
+            return rc;
+        tot.ms += R.ms;
+        tot.steps += R.steps;
+        tot.evals += R.evals;
+        tot.pots += R.pots;
+        tot.pairs += R.pairs;
+        launches++;
+    }
+    return 0;
+}
+
+int tune_batch(const cosims_batch_job *jobs, int32_t njobs, int prec, int &L_out, int &W_out)
+{
+    Stream st;
+    HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+    BatchPlan sets[2];
+    const int64_t caps[2] = {128, 512};
+    int units = 0;  // largest molecule: the lane counts worth trying
+    for (int k = 0; k < 2; k++) {
+        if (int rc = plan_batch(jobs, njobs, caps[k], sets[k])) return rc;
+        for (auto &J : sets[k].jobs) {
+            J.job = 0xfffffff0u - (uint32_t)k;  // a stream of its own
+            units = std::max(units, J.units);
+        }
+        if (int rc = stage_batch(sets[k], st.s)) return rc;
+    }
+    auto run = [&](int set, int L, int W, double &ms) -> int {
+        GroupRun R;
+        int launches = 0;
+        if (int rc = launch_batch(sets[set], prec, L, W, 0, st.s, R, launches)) return rc;
+        ms = R.ms;
+        return 0;
+    };
+    return tune_shapes(units, run, L_out, W_out);
 }
 
 }  // namespace
@@ -1235,7 +1680,7 @@ int cosims_verlet(const cosims_mol *M, int32_t prec, int64_t n, const double *p0
     // FP64 runs the lane-per-trajectory kernel: atom-order sums, bit-identical
     // to the reference; mixed precision runs the tuned group kernel.
     int L = 1, wg = 256;
-    if (prec == COSIMS_PREC_MIXED && group_mode_possible(M, prec))
+    if (prec == COSIMS_PREC_MIXED)
         if (int rc = autotune(M, prec, L, wg)) return rc;
     Stream st;
     HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
@@ -1292,19 +1737,13 @@ int cosims_run(const cosims_mol *M, const cosims_params *P, const cosims_run_cfg
         return fail(COSIMS_E_INVALID, "lanes must be 0 (tuned), 1, 4, 8, 16, 32 or 64");
     std::memset(out, 0, sizeof(*out));
     if (int rc = ensure_device(M)) return rc;
-    // launch shape: explicit, or tuned (group kernel where the molecule fits LDS)
+    // launch shape: explicit, or tuned
     int L = c.lanes, wg = c.wg_size;
-    if (L > 1 && !group_mode_possible(M, prec))
-        return fail(COSIMS_E_INVALID, "group kernel needs force type 4 or 6 and a molecule that fits LDS");
     if (L == 0) {
-        if (group_mode_possible(M, prec)) {
-            int tl = 0, tw = 0;
-            if (int rc = autotune(M, prec, tl, tw)) return rc;
-            L = tl;
-            if (wg <= 0) wg = tw;
-        } else {
-            L = 1;
-        }
+        int tl = 0, tw = 0;
+        if (int rc = autotune(M, prec, tl, tw)) return rc;
+        L = tl;
+        if (wg <= 0) wg = tw;
     }
     if (wg <= 0) wg = 256;
     if (wg % 64 || wg > (L == 1 ? 256 : 1024) || wg % L)
@@ -1387,54 +1826,104 @@ int cosims_run(const cosims_mol *M, const cosims_params *P, const cosims_run_cfg
     if (c.chi) HIP_CHECK(hipMemcpyAsync(c.chi, B.chi.p, sizeof(double) * n, hipMemcpyDeviceToHost, st.s));
     if (c.g) HIP_CHECK(hipMemcpyAsync(c.g, B.g.p, sizeof(double) * n, hipMemcpyDeviceToHost, st.s));
     HIP_CHECK(hipStreamSynchronize(st.s));
-    // Denominator currSuccTraj of one OpenMP thread (src/monteTraj.cpp:456-521):
-    // failures (a miss repeats the previous success flag) counted since the
-    // last progress update every print_rate trajectories.
-    const int64_t upd = P->print_rate > 0 ? P->print_rate : std::max<int64_t>(1, (int64_t)(T / 5.0));
-    double mx = 0;
-    for (int k = 0; k < iters; k++) {
-        const int8_t *st8 = status.data() + (int64_t)k * T;
-        bool succ = false;
-        int64_t orbit = 0;
-        for (int64_t i = 0; i < T; i++) {
-            if (st8[i] != ST_MISS) succ = (st8[i] == ST_OK);
-            if (!succ) orbit++;
-            if ((i + 1) % upd == 0) orbit = 0;
-        }
-        const double d = (double)(T - orbit);
-        const double tot = sums[6 * k], tot2 = sums[6 * k + 1];
-        omega[k] = tot / d;
-        omega_err[k] = std::sqrt((tot2 / d - (tot / d) * (tot / d)) / d);
-        mx = std::max(mx, sums[6 * k + 2]);
-        out->total_steps += (int64_t)sums[6 * k + 3];
-        out->n_hit += (int64_t)sums[6 * k + 4];
-        out->n_fail += (int64_t)sums[6 * k + 5];
-    }
     if (c.status)
         for (int64_t i = 0; i < n; i++)
             c.status[i] = status[i] == ST_OK ? 1 : (status[i] == ST_MISS ? 0 : 2);
-    // final statistics (src/monteTraj.cpp:412-425)
-    double avg = 0, sd = 0, we = 0, wm = 0;
-    for (int k = 0; k < iters; k++) avg += omega[k];
-    avg = avg / (double)iters;
-    for (int k = 0; k < iters; k++) {
-        sd += (omega[k] - avg) * (omega[k] - avg);
-        we += 1.0 / (omega_err[k] * omega_err[k]);
-        wm += omega[k] / (omega_err[k] * omega_err[k]);
-    }
-    wm /= we;
-    out->mean = avg;
-    out->sd = std::sqrt(sd / (double)iters);
-    out->wmean = wm;
-    out->werr = 1.0 / std::sqrt(we);
-    out->max_ergy = mx;
+    finish_ccs(status.data(), sums.data(), iters, T, P->print_rate, omega, omega_err, out);
     out->n_traj = n;
     const int ft = M->prep.force_type;
-    const int64_t npair = (ft == 4 || ft == 6) ? M->prep.n : 0;
+    const int64_t npair = (ft == 4 || ft == 6) ? M->prep.n : 0;  // clustered types: branch-dependent
     int64_t evals = 0;
     for (int r = 0; r < out->n_rounds; r++) evals += out->round_evals[r];
     out->pair_evals = npair * evals;
     out->run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+int cosims_tune_batch(const cosims_batch_job *jobs, int32_t njobs, int32_t prec, int32_t *lanes, int32_t *wg_size)
+{
+    if (!lanes || !wg_size) return fail(COSIMS_E_INVALID, "cosims_tune_batch: null output");
+    if (prec != COSIMS_PREC_FP64 && prec != COSIMS_PREC_MIXED) return fail(COSIMS_E_INVALID, "unknown precision");
+    int L = 0, W = 0;
+    if (int rc = tune_batch(jobs, njobs, prec, L, W)) return rc;
+    *lanes = L;
+    *wg_size = W;
+    return 0;
+}
+
+int cosims_run_batch(cosims_batch_job *jobs, int32_t njobs, const cosims_batch_cfg *cfg, cosims_batch_stats *stats)
+{
+    if (!cfg || !stats) return fail(COSIMS_E_INVALID, "cosims_run_batch: null argument");
+    const int prec = cfg->prec;
+    if (prec != COSIMS_PREC_FP64 && prec != COSIMS_PREC_MIXED) return fail(COSIMS_E_INVALID, "unknown precision");
+    for (int k = 0; jobs && k < njobs; k++)
+        if (!jobs[k].omega || !jobs[k].omega_err || !jobs[k].out)
+            return fail(COSIMS_E_INVALID, "cosims_run_batch: job without outputs");
+    int L = cfg->lanes, wg = cfg->wg_size;
+    if (L != 0 && L != 4 && L != 8 && L != 16 && L != 32 && L != 64)
+        return fail(COSIMS_E_INVALID, "lanes must be 0 (tuned), 4, 8, 16, 32 or 64");
+    std::memset(stats, 0, sizeof(*stats));
+    auto t0 = std::chrono::steady_clock::now();
+    BatchPlan P;
+    if (int rc = plan_batch(jobs, njobs, 0, P)) return rc;
+    if (L == 0) {
+        int tl = 0, tw = 0;
+        if (int rc = tune_batch(jobs, njobs, prec, tl, tw)) return rc;
+        L = tl;
+        if (wg <= 0) wg = tw;
+    }
+    if (wg <= 0) wg = 512;
+    if (wg % 64 || wg > 1024) return fail(COSIMS_E_INVALID, "wg_size must be a multiple of 64, at most 1024");
+    Stream st;
+    HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+    if (int rc = stage_batch(P, st.s)) return rc;
+    GroupRun R;
+    int launches = 0;
+    if (int rc = launch_batch(P, prec, L, wg, cfg->blocks_per_cu, st.s, R, launches)) return rc;
+    // per-integral sums of every job (k_reduce per job), then the host statistics
+    int64_t total_iters = 0;
+    for (const auto &J : P.jobs) total_iters += J.iters;
+    DBuf red;
+    HIP_CHECK(red.alloc(sizeof(double) * 6 * total_iters));
+    int64_t it0 = 0;
+    std::vector<int64_t> it_base(P.jobs.size());
+    for (size_t k = 0; k < P.jobs.size(); k++) {
+        const auto &J = P.jobs[k];
+        it_base[k] = it0;
+        ReduceArgs RA{};
+        RA.status = P.B.status.as<int8_t>() + J.off; RA.chi = P.B.chi.as<double>() + J.off;
+        RA.ergy = P.B.ergy.as<double>() + J.off; RA.totsteps = P.B.tot.as<int32_t>() + J.off;
+        RA.per_iter = J.T; RA.ccs_const = J.mol->prep.ccs_const; RA.out = red.as<double>() + 6 * it0;
+        hipLaunchKernelGGL(k_reduce, dim3(J.iters), dim3(256), 0, st.s, RA);
+        HIP_CHECK(hipGetLastError());
+        it0 += J.iters;
+    }
+    std::vector<double> sums(6 * total_iters);
+    std::vector<int8_t> status(P.n);
+    HIP_CHECK(hipMemcpyAsync(sums.data(), red.p, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, st.s));
+    HIP_CHECK(hipMemcpyAsync(status.data(), P.B.status.p, P.n, hipMemcpyDeviceToHost, st.s));
+    HIP_CHECK(hipStreamSynchronize(st.s));
+    for (size_t k = 0; k < P.jobs.size(); k++) {
+        const auto &J = P.jobs[k];
+        cosims_ccs *o = jobs[k].out;
+        std::memset(o, 0, sizeof(*o));
+        finish_ccs(status.data() + J.off, sums.data() + 6 * it_base[k], J.iters, J.T, J.mol->params.print_rate,
+                   jobs[k].omega, jobs[k].omega_err, o);
+        o->lanes = L;
+        o->wg_size = wg;
+        stats->n_hit += o->n_hit;
+        stats->total_steps += o->total_steps;
+    }
+    stats->lanes = L;
+    stats->wg_size = wg;
+    stats->launches = launches;
+    stats->kernel_ms = R.ms;
+    stats->sample_ms = P.sample_ms;
+    stats->n_traj = P.n;
+    stats->force_evals = R.evals;
+    stats->pot_evals = R.pots;
+    stats->pair_evals = R.pairs;
+    stats->run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return 0;
 }
 

@@ -261,69 +261,76 @@ __device__ __forceinline__ void cl_far(const MolDev &m, const CAS ClusterT<T> &H
     }
 }
 
-// multipoleCutoff (type 1, CUT) / multipoleOnly (type 2)
-// src/forceAccel.cpp:221-445, 447-633
-template <typename T, bool CUT>
-__device__ __forceinline__ void acc_multipole(const MolDev &m, T px, T py, T pz, T &ox, T &oy, T &oz)
+// One cluster of the clustered force types: the loop bodies of
+// src/forceAccel.cpp:244-438 (type 1, multipoleCutoff), :470-626 (type 2,
+// multipoleOnly), :655-742 (type 3, dispersionOnly), :819-851 (type 5,
+// dispersionNoCharge).
+template <typename T, int FT>
+__device__ __forceinline__ void cl_one(const MolDev &m, const CAS ClusterT<T> &H, T px, T py, T pz, ClAcc<T> &A)
 {
-    const auto H = Layout<T>::heads(m);
-    ClAcc<T> A;
-    for (int n = 0; n < m.ncl; n++) {
-        T dx0 = px - H[n].px, dx1 = py - H[n].py, dx2 = pz - H[n].pz;
+    T dx0 = px - H.px, dx1 = py - H.py, dx2 = pz - H.pz;
+    if constexpr (FT == 1 || FT == 2) {
         T xx = dx0 * dx0, yy = dx1 * dx1, zz = dx2 * dx2;
         T r2 = xx + yy + zz;
-        const int s = H[n].start, e = H[n].end;
-        if (r2 > H[n].mpr) {
-            cl_far<T>(m, H[n], dx0, dx1, dx2, r2, A);
-            if (CUT) {
-                if (!(r2 > H[n].dpr)) cl_lj<T, true>(m, s, e, H[n].qs6, H[n].qe24, px, py, pz, A);
+        const int s = H.start, e = H.end;
+        if (r2 > H.mpr) {
+            cl_far<T>(m, H, dx0, dx1, dx2, r2, A);
+            if (FT == 1) {
+                if (!(r2 > H.dpr)) cl_lj<T, true>(m, s, e, H.qs6, H.qe24, px, py, pz, A);
             } else {
                 cl_lj<T, false>(m, s, e, T(0), T(0), px, py, pz, A);
             }
-        } else if (!CUT || !(r2 > H[n].dpr)) {
+        } else if (FT == 2 || !(r2 > H.dpr)) {
             cl_exact<T, true>(m, s, e, px, py, pz, A);
         } else {
             cl_exact<T, false>(m, s, e, px, py, pz, A);
         }
+    } else {
+        T r2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+        if constexpr (FT == 3) {
+            if (!(r2 > H.dpr)) cl_exact<T, true>(m, H.start, H.end, px, py, pz, A);
+            else cl_exact<T, false>(m, H.start, H.end, px, py, pz, A);
+        } else {
+            if (!(r2 > H.dpr)) cl_lj<T, false>(m, H.start, H.end, T(0), T(0), px, py, pz, A);
+        }
     }
+}
+
+// Polarisation epilogue of the clustered types (src/forceAccel.cpp:440-442
+// and :628-630 with the antisymmetric aMat part, :744-746 without it, none
+// for type 5).
+template <typename T, int FT>
+__device__ __forceinline__ void cl_epilogue(const MolDev &m, ClAcc<T> &A, T &ox, T &oy, T &oz)
+{
     const T ds = (T)m.dipole_scalar;
-    A.sumX += ds * (A.dXX * A.fX + (A.sXY + A.aXY) * A.fY + (A.sXZ + A.aXZ) * A.fZ);
-    A.sumY += ds * ((A.sXY - A.aXY) * A.fX + A.dYY * A.fY + (A.sYZ + A.aYZ) * A.fZ);
-    A.sumZ += ds * ((A.sXZ - A.aXZ) * A.fX + (A.sYZ - A.aYZ) * A.fY + A.dZZ * A.fZ);
+    if constexpr (FT == 1 || FT == 2) {
+        A.sumX += ds * (A.dXX * A.fX + (A.sXY + A.aXY) * A.fY + (A.sXZ + A.aXZ) * A.fZ);
+        A.sumY += ds * ((A.sXY - A.aXY) * A.fX + A.dYY * A.fY + (A.sYZ + A.aYZ) * A.fZ);
+        A.sumZ += ds * ((A.sXZ - A.aXZ) * A.fX + (A.sYZ - A.aYZ) * A.fY + A.dZZ * A.fZ);
+    } else if constexpr (FT == 3) {
+        A.sumX += ds * (A.dXX * A.fX + A.sXY * A.fY + A.sXZ * A.fZ);
+        A.sumY += ds * (A.sXY * A.fX + A.dYY * A.fY + A.sYZ * A.fZ);
+        A.sumZ += ds * (A.sXZ * A.fX + A.sYZ * A.fY + A.dZZ * A.fZ);
+    }
     ox = A.sumX; oy = A.sumY; oz = A.sumZ;
 }
 
-// dispersionOnly, type 3 (src/forceAccel.cpp:635-749)
-template <typename T>
-__device__ __forceinline__ void acc_type3(const MolDev &m, T px, T py, T pz, T &ox, T &oy, T &oz)
+// Clustered force types over clusters first, first+stride, ...: stride 1 is
+// the reference's cluster order; the group kernel splits clusters over the
+// lanes of a group and sums the partial accumulators afterwards.
+template <typename T, int FT>
+__device__ __forceinline__ void cl_sum(const MolDev &m, int first, int stride, T px, T py, T pz, ClAcc<T> &A)
 {
     const auto H = Layout<T>::heads(m);
-    ClAcc<T> A;
-    for (int n = 0; n < m.ncl; n++) {
-        T dx0 = px - H[n].px, dx1 = py - H[n].py, dx2 = pz - H[n].pz;
-        T r2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
-        if (!(r2 > H[n].dpr)) cl_exact<T, true>(m, H[n].start, H[n].end, px, py, pz, A);
-        else cl_exact<T, false>(m, H[n].start, H[n].end, px, py, pz, A);
-    }
-    const T ds = (T)m.dipole_scalar;
-    A.sumX += ds * (A.dXX * A.fX + A.sXY * A.fY + A.sXZ * A.fZ);
-    A.sumY += ds * (A.sXY * A.fX + A.dYY * A.fY + A.sYZ * A.fZ);
-    A.sumZ += ds * (A.sXZ * A.fX + A.sYZ * A.fY + A.dZZ * A.fZ);
-    ox = A.sumX; oy = A.sumY; oz = A.sumZ;
+    for (int n = first; n < m.ncl; n += stride) cl_one<T, FT>(m, H[n], px, py, pz, A);
 }
 
-// dispersionNoCharge, type 5 (src/forceAccel.cpp:807-854)
-template <typename T>
-__device__ __forceinline__ void acc_type5(const MolDev &m, T px, T py, T pz, T &ox, T &oy, T &oz)
+template <typename T, int FT>
+__device__ __forceinline__ void acc_clustered(const MolDev &m, T px, T py, T pz, T &ox, T &oy, T &oz)
 {
-    const auto H = Layout<T>::heads(m);
     ClAcc<T> A;
-    for (int n = 0; n < m.ncl; n++) {
-        T dx0 = px - H[n].px, dx1 = py - H[n].py, dx2 = pz - H[n].pz;
-        T r2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
-        if (!(r2 > H[n].dpr)) cl_lj<T, false>(m, H[n].start, H[n].end, T(0), T(0), px, py, pz, A);
-    }
-    ox = A.sumX; oy = A.sumY; oz = A.sumZ;
+    cl_sum<T, FT>(m, 0, 1, px, py, pz, A);
+    cl_epilogue<T, FT>(m, A, ox, oy, oz);
 }
 
 // ---------------------------------------------------------------------------
@@ -389,22 +396,16 @@ __device__ __forceinline__ void accel(const MolDev &m, double px, double py, dou
                                       double &oz)
 {
     if constexpr (PREC == 0) {
-        if constexpr (FT == 1) acc_multipole<double, true>(m, px, py, pz, ox, oy, oz);
-        else if constexpr (FT == 2) acc_multipole<double, false>(m, px, py, pz, ox, oy, oz);
-        else if constexpr (FT == 3) acc_type3<double>(m, px, py, pz, ox, oy, oz);
-        else if constexpr (FT == 4) acc_type4<double>(m, px, py, pz, ox, oy, oz);
-        else if constexpr (FT == 5) acc_type5<double>(m, px, py, pz, ox, oy, oz);
-        else acc_type6<double>(m, px, py, pz, ox, oy, oz);
+        if constexpr (FT == 4) acc_type4<double>(m, px, py, pz, ox, oy, oz);
+        else if constexpr (FT == 6) acc_type6<double>(m, px, py, pz, ox, oy, oz);
+        else acc_clustered<double, FT>(m, px, py, pz, ox, oy, oz);
     } else {
         const float fx = (float)px, fy = (float)py, fz = (float)pz;
         if constexpr (FT == 4) acc4_fast(m, fx, fy, fz, ox, oy, oz);
         else if constexpr (FT == 6) acc6_fast(m, fx, fy, fz, ox, oy, oz);
         else {
             float a, b, c;
-            if constexpr (FT == 1) acc_multipole<float, true>(m, fx, fy, fz, a, b, c);
-            else if constexpr (FT == 2) acc_multipole<float, false>(m, fx, fy, fz, a, b, c);
-            else if constexpr (FT == 3) acc_type3<float>(m, fx, fy, fz, a, b, c);
-            else acc_type5<float>(m, fx, fy, fz, a, b, c);
+            acc_clustered<float, FT>(m, fx, fy, fz, a, b, c);
             ox = a; oy = b; oz = c;
         }
     }

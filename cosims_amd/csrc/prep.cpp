@@ -367,66 +367,109 @@ int set_inertia(Prepared &m)
 }
 
 // ------------------------------------------------------------ boundary --
-void boundary_points(const Prepared &m, double min_kin, int threads, std::vector<V3> &out)
+// One march on the host, the loop of setBoundry (src/impactParam.cpp:666-690)
+// and of its z-axis search (:616-631).
+MarchHit march_host(const Prepared &m, double mk, const March &w)
 {
-    const int numRot = 20;
-    const double velMag = 0.1;
-    double posMag = std::fabs(std::max(std::fabs(m.max_xyz[0]), std::fabs(m.max_xyz[1])));
-    double pos[3] = {0, 0, std::fabs(m.max_xyz[2]) + 20}, prev[3];
-    double e = potential_host(m, pos), slope = 1.0, zMax = 0;
-    min_kin = 0.01 * min_kin;
-    if (e + min_kin < 0) slope = -1;
-    for (;;) {
-        std::memcpy(prev, pos, sizeof(pos));
-        pos[2] -= 0.5 * slope;
-        e = potential_host(m, pos);
-        if ((e + min_kin) * slope < 0) {
-            zMax = std::max(pos[2], prev[2]);
-            break;
+    MarchHit h{};
+    V3 p(w.start[0], w.start[1], w.start[2]);
+    const double e0 = potential_host(m, w.start);
+    const double sl = (e0 + mk < 0) ? -1.0 : 1.0;
+    const V3 v = V3(w.dir[0], w.dir[1], w.dir[2]) * sl;
+    V3 pv(w.prev0[0], w.prev0[1], w.prev0[2]);
+    if (w.first) p = p + v;
+    const double sx = w.start[0], sy = w.start[1];
+    for (int64_t k = w.first; k < kMaxMarchPoints; k++) {
+        const double q[3] = {p.x, p.y, p.z};
+        const double en = potential_host(m, q);
+        if ((en + mk) * sl < 0) {
+            h.found = 1;
+            h.p[0] = p.x; h.p[1] = p.y; h.p[2] = p.z;
+            h.prev[0] = pv.x; h.prev[1] = pv.y; h.prev[2] = pv.z;
+            return h;
         }
+        if (w.axis && ((sx == 0 && sy / p.y <= 0) || (sy == 0 && sx / p.x <= 0) || (sx / p.x <= 0 && sy / p.y <= 0))) {
+            h.found = 0;
+            return h;
+        }
+        pv = p;
+        p = p + v;
     }
-    zMax *= 1.3;
-    posMag += std::fabs(zMax - std::fabs(m.max_xyz[2]));
-    int levels = (int)std::ceil((double)std::max(25, (int)std::ceil(zMax)));
-    double zInc = 2 * zMax / (levels);
-    std::vector<V3> slot((size_t)(levels + 1) * numRot);
-    auto ray = [&](int i, int n) {
-        double zPos = -zMax + i * zInc;
-        double sl = 1;
-        double theta = n * 2 * kPi / ((double)numRot);
-        double sx = posMag * std::cos(theta), sy = posMag * std::sin(theta);
-        V3 p(sx, sy, zPos), pv;
-        V3 v(-velMag * std::cos(theta), -velMag * std::sin(theta), 0.0);
-        double pp[3] = {p.x, p.y, p.z};
-        double en = potential_host(m, pp);
-        if (en + min_kin < 0) sl = -1;
-        v = v * sl;
-        for (;;) {
-            double q[3] = {p.x, p.y, p.z};
-            en = potential_host(m, q);
-            if ((en + min_kin) * sl < 0) {
-                slot[(size_t)i * numRot + n] = (pv.mag() > p.mag()) ? pv : p;
-                return;
-            }
-            if ((sx == 0 && sy / p.y <= 0) || (sy == 0 && sx / p.x <= 0) || (sx / p.x <= 0 && sy / p.y <= 0))
-                return;
-            pv = p;
-            p = p + v;
-        }
-    };
+    h.found = -1;
+    return h;
+}
+
+void march_all(const Prepared &m, double mk, const std::vector<March> &w, std::vector<MarchHit> &out, int threads)
+{
+    out.assign(w.size(), MarchHit{});
     int nthr = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
-    int total = (levels + 1) * numRot;
-    nthr = std::min(nthr, total);
+    nthr = std::max(1, std::min<int>(nthr, (int)w.size()));
     std::vector<std::thread> pool;
     std::atomic_int next{0};
     for (int t = 0; t < nthr; t++)
         pool.emplace_back([&]() {
-            for (int k; (k = next.fetch_add(1)) < total;) ray(k / numRot, k % numRot);
+            for (int k; (k = next.fetch_add(1)) < (int)w.size();) out[k] = march_host(m, mk, w[k]);
         });
     for (auto &th : pool) th.join();
+}
+
+// Boundary points of impactParam::setBoundry (src/impactParam.cpp:602-736):
+// a march down the z axis in 0.5 A steps finds zMax; then (levels+1) x 20
+// rays at heights -zMax..zMax march towards the z axis in 0.1 A steps, each
+// keeping the point of the crossing farther from the centre.
+int boundary_points(const Prepared &m, double min_kin, int threads, BoundaryMarcher *bm, std::vector<V3> &out,
+                    std::string &err)
+{
+    const int numRot = 20;
+    const double velMag = 0.1;
+    double posMag = std::fabs(std::max(std::fabs(m.max_xyz[0]), std::fabs(m.max_xyz[1])));
+    const double mk = 0.01 * min_kin;
+    auto run = [&](const std::vector<March> &w, std::vector<MarchHit> &h) -> int {
+        if (bm) return bm->run(m, mk, w, h, err);
+        march_all(m, mk, w, h, threads);
+        return 0;
+    };
+    // z axis: pos.z -= 0.5 * slope, i.e. the step (0, 0, -0.5) * slope
+    std::vector<March> zw(1);
+    zw[0] = March{{0, 0, std::fabs(m.max_xyz[2]) + 20}, {0, 0, -0.5}, {0, 0, 0}, 1, 0};
+    std::memcpy(zw[0].prev0, zw[0].start, sizeof(zw[0].start));
+    std::vector<MarchHit> zh;
+    if (int rc = run(zw, zh)) return rc;
+    if (zh[0].found != 1) {
+        err = "boundary search: no potential crossing on the z axis";
+        return COSIMS_E_SETUP;
+    }
+    double zMax = std::max(zh[0].p[2], zh[0].prev[2]);
+    zMax *= 1.3;
+    posMag += std::fabs(zMax - std::fabs(m.max_xyz[2]));
+    int levels = (int)std::ceil((double)std::max(25, (int)std::ceil(zMax)));
+    double zInc = 2 * zMax / (levels);
+    std::vector<March> rays((size_t)(levels + 1) * numRot);
+    for (int i = 0; i <= levels; i++)
+        for (int n = 0; n < numRot; n++) {
+            double zPos = -zMax + i * zInc;
+            double theta = n * 2 * kPi / ((double)numRot);
+            March &w = rays[(size_t)i * numRot + n];
+            w.start[0] = posMag * std::cos(theta); w.start[1] = posMag * std::sin(theta); w.start[2] = zPos;
+            w.dir[0] = -velMag * std::cos(theta); w.dir[1] = -velMag * std::sin(theta); w.dir[2] = 0.0;
+            w.prev0[0] = w.prev0[1] = w.prev0[2] = 0;
+            w.first = 0;
+            w.axis = 1;
+        }
+    std::vector<MarchHit> hits;
+    if (int rc = run(rays, hits)) return rc;
     out.clear();
-    for (const V3 &q : slot)
+    for (const MarchHit &h : hits) {
+        if (h.found < 0) {
+            err = "boundary search: a ray found no potential crossing";
+            return COSIMS_E_SETUP;
+        }
+        if (h.found == 0) continue;
+        const V3 p(h.p[0], h.p[1], h.p[2]), pv(h.prev[0], h.prev[1], h.prev[2]);
+        const V3 q = (pv.mag() > p.mag()) ? pv : p;
         if (!(q.x == 0 && q.y == 0 && q.z == 0)) out.push_back(q);
+    }
+    return 0;
 }
 
 void fit_ellipsoid(const std::vector<V3> &P, Prepared &m)
@@ -674,7 +717,7 @@ int complete_prepared(const Params &p, Prepared &m, std::string &err)
 }
 
 int prepare(const Params &p, int n, const double *xyz, const double *mass, const double *sigma, const double *eps,
-            const double *charge, double sigma_max, Prepared &m, std::string &err)
+            const double *charge, double sigma_max, Prepared &m, std::string &err, BoundaryMarcher *bm)
 {
     if (n <= 0 || !xyz || !mass || !sigma || !eps) {
         err = "empty molecule";
@@ -723,7 +766,8 @@ int prepare(const Params &p, int n, const double *xyz, const double *mass, const
     m.dipole_scalar = kAlphaKe / m.mu;
     m.vd = velocity_distribution(m.mu, p.temp);
     std::vector<V3> pts;
-    boundary_points(m, m.vd.min_kin, p.threads, pts);
+    rc = boundary_points(m, m.vd.min_kin, p.threads, bm, pts, err);
+    if (rc) return rc;
     if (pts.size() < 4) {
         err = "potential boundary search found too few points";
         return COSIMS_E_SETUP;

@@ -82,10 +82,39 @@ struct Prepared {
 
 int force_type_of(bool charged, bool multipole, bool cutoff);
 
+// One potential march of the boundary search (impactParam::setBoundry,
+// src/impactParam.cpp:602-736): points p_k = p_{k-1} + v (FP64, component
+// by component) from p_0 = start, with v = dir * sl and sl = -1 when
+// U(start) + mk < 0, else 1.  Points are tested from p_first (0 or 1): the
+// walk stops at the first p_k with (U(p_k) + mk) * sl < 0 (a crossing), or,
+// when axis is set, with the ray past the z axis (the test of :684-686).
+struct March {
+    double start[3], dir[3];
+    double prev0[3];   // the point before p_first (p_{first-1}; zero for first = 0)
+    int first, axis;
+};
+// Result: found 1 (crossing at p, previous point prev), 0 (left past the
+// axis), -1 (no crossing within the step limit).
+struct MarchHit {
+    int found;
+    double p[3], prev[3];
+};
+constexpr int64_t kMaxMarchPoints = int64_t(1) << 22;
+
+// Evaluates marches somewhere other than the host loop (the GPU).  Must give
+// the same points as the host loop: the same FP64 additions and the
+// reference-order getPotential.
+struct BoundaryMarcher {
+    virtual ~BoundaryMarcher() = default;
+    virtual int run(const Prepared &m, double mk, const std::vector<March> &w, std::vector<MarchHit> &out,
+                    std::string &err) = 0;
+};
+
 // Full setup from raw atoms.  Returns 0, or a negative engine error code with
-// a message in err.
+// a message in err.  With a marcher the boundary marches run there.
 int prepare(const Params &p, int n, const double *xyz, const double *mass, const double *sigma,
-            const double *eps, const double *charge, double sigma_max, Prepared &out, std::string &err);
+            const double *eps, const double *charge, double sigma_max, Prepared &out, std::string &err,
+            BoundaryMarcher *bm = nullptr);
 
 // Completes an injected prepared molecule (cluster moments, velocity window,
 // CCS constant).

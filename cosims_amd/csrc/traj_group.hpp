@@ -1,30 +1,48 @@
-// traj_group.hpp -- group-per-trajectory persistent kernel (gfx950).
+// traj_group.hpp -- group-per-trajectory persistent kernels (gfx950).
 //
 // L lanes (a power of two, 4..64) integrate one trajectory together: every
-// lane of the group holds the same FP64 Verlet state and takes atoms j,
-// j+L, j+2L, ... of the force sum; a butterfly of lane shuffles gives every
-// lane the full sums.  A wavefront therefore carries 64/L trajectories and
-// a step costs N/L pairs per lane instead of N, so a long trajectory no
-// longer holds a whole launch hostage.  The energy-conservation test
-// (src/diffEqSolver.cpp:108, 178-203) runs inside the kernel as an FP64
-// getPotential across the group, and a failed attempt restarts at once
-// with dt/(k+1): one launch covers every attempt of every trajectory.
+// lane of the group holds the same FP64 Verlet state and takes atoms (or
+// clusters) j, j+L, j+2L, ... of the force sum; a butterfly of lane shuffles
+// gives every lane the full sums.  A wavefront therefore carries 64/L
+// trajectories and a step costs N/L pairs per lane instead of N, so a long
+// trajectory no longer holds a whole launch hostage.  The energy-conservation
+// test (src/diffEqSolver.cpp:108, 178-203) runs inside the kernel as an FP64
+// getPotential across the group, and a failed attempt restarts at once with
+// dt/(k+1): one launch covers every attempt of every trajectory.
 //
-// The molecule is staged once per workgroup in LDS (SoA: FP32 x,y,z,q |
-// sigma^6, 24 eps/mu as float4 + float2, or the FP64 equivalents), padded to
-// a multiple of 64 with inert atoms (charge and LJ parameters zero, far
-// away) so every lane runs the same trip count.  The FP64 potential reads
-// global SoA arrays (the rare path).
+// Two kernels share the trajectory state machine (group_loop):
+//   k_traj_grp    one molecule, staged once per workgroup in LDS (SoA: FP32
+//                 x,y,z,q | sigma^6, 24 eps/mu as float4 + float2, or the FP64
+//                 equivalents), exact force types 4 and 6;
+//   k_traj_multi  any number of molecules of one force type, read from HBM
+//                 through L2 (a table of MolRec, one molecule per trajectory):
+//                 molecules of uneven size in one launch, molecules too large
+//                 for LDS, and the clustered force types 1, 2, 3, 5 (lanes
+//                 split the clusters).
+// Atom arrays are padded to a multiple of 64 with inert atoms (charge and LJ
+// parameters zero, far away) so every lane of a group runs the same trip
+// count.
 #pragma once
 
 #include "kernels.hpp"
 
+#define GLB __attribute__((address_space(1)))
+
 namespace cosims {
 
-struct GroupArgs {
-    MolDev m;
-    int npad;                              // atoms incl. padding, multiple of 64
-    const double *sx, *sy, *sz, *sq, *s6, *se24, *se4;  // global FP64 SoA (npad)
+template <typename T>
+__device__ __forceinline__ const GLB T *glb(const T *p)
+{
+    return (const GLB T *)p;
+}
+
+// Native vector types for the FP32 SoA (HIP's float4 / float2 class types
+// cannot be copied out of an explicit address space).
+typedef float f4n __attribute__((ext_vector_type(4)));
+typedef float f2n __attribute__((ext_vector_type(2)));
+
+// Per-trajectory arrays of one persistent launch.
+struct TrajIO {
     const int32_t *items;
     int32_t n_items;
     int32_t *counter;
@@ -32,9 +50,33 @@ struct GroupArgs {
     int8_t *status;
     double *chi, *cchi, *ergy;             // cchi: cos(chi) of accepted trajectories
     int32_t *steps, *totsteps;
-    double dt0;
     int32_t max_steps;
-    unsigned long long *evals;             // [0] steps, [1] force evaluations, [2] potential evaluations
+    unsigned long long *evals;             // [0] steps, [1] force evaluations, [2] potential evaluations,
+                                           // [3] atom pairs of the exact force types' evaluations
+};
+
+struct GroupArgs {
+    TrajIO io;
+    MolDev m;
+    int npad;                              // atoms incl. padding, multiple of 64
+    const double *soa;                     // global FP64 SoA: x y z q s6 e24 e4, npad each
+    double dt0;
+};
+
+// One molecule of a multi-molecule launch.
+struct MolRec {
+    MolDev m;
+    const float4 *xq;                      // FP32 x, y, z, q (npad)
+    const float2 *se;                      // FP32 sigma^6, 24 eps/mu (npad)
+    const double *soa;                     // FP64 SoA as in GroupArgs
+    int32_t npad, pad_;
+    double dt0;
+};
+
+struct MultiArgs {
+    TrajIO io;
+    const MolRec *mols;
+    const int32_t *tmol;                   // molecule of each trajectory (NULL: all molecule 0)
 };
 
 // chi = vector3D::Angle of the accepted exit (src/vector3D.h:134-143) from
@@ -58,23 +100,17 @@ __device__ __forceinline__ T gsum(T v)
     return v;
 }
 
-// FP32 pair kernel of the exact-sum types, partial sums of this lane.
-template <int FT>
-struct Part32 {
-    float fx = 0, fy = 0, fz = 0, ex = 0, ey = 0, ez = 0, sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0;
-};
-
+// FP32 pair kernel of the exact-sum types over atoms j, j+L, ... < npad.
 #pragma clang fp contract(fast)
-template <int FT, int L>
-__device__ __forceinline__ void grp_force32(const float4 *__restrict__ xq, const float2 *__restrict__ se, int npad,
-                                            int j, float px, float py, float pz, double ds, double &ox, double &oy,
-                                            double &oz)
+template <int FT, int L, typename P4, typename P2>
+__device__ __forceinline__ void grp_force32(P4 xq, P2 se, int npad, int j, float px, float py, float pz, double ds,
+                                            double &ox, double &oy, double &oz)
 {
-    Part32<FT> P;
+    float fx = 0, fy = 0, fz = 0, ex = 0, ey = 0, ez = 0, sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0;
 #pragma unroll 2
     for (int k = j; k < npad; k += L) {
-        const float4 a = xq[k];
-        const float2 b = se[k];
+        const f4n a = xq[k];
+        const f2n b = se[k];
         float dx = px - a.x, dy = py - a.y, dz = pz - a.z;
         float r2 = dx * dx + dy * dy + dz * dz;
         if constexpr (FT == 4) {
@@ -82,45 +118,45 @@ __device__ __forceinline__ void grp_force32(const float4 *__restrict__ xq, const
             float ir2 = ir * ir;
             float t2 = b.x * ir2 * ir2 * ir2;
             float lj = b.y * ir2 * t2 * (2.0f * t2 - 1.0f);
-            P.fx += lj * dx; P.fy += lj * dy; P.fz += lj * dz;
+            fx += lj * dx; fy += lj * dy; fz += lj * dz;
             float qr3 = a.w * ir2 * ir;
-            P.ex += qr3 * dx; P.ey += qr3 * dy; P.ez += qr3 * dz;
+            ex += qr3 * dx; ey += qr3 * dy; ez += qr3 * dz;
             float qr5 = qr3 * ir2;
             float u = qr5 * dx, v = qr5 * dy, w = qr5 * dz;
-            P.sxx += u * dx; P.syy += v * dy; P.szz += w * dz;
-            P.sxy += u * dy; P.sxz += u * dz; P.syz += v * dz;
+            sxx += u * dx; syy += v * dy; szz += w * dz;
+            sxy += u * dy; sxz += u * dz; syz += v * dz;
         } else {
             float ir2 = __builtin_amdgcn_rcpf(r2);
             float t2 = b.x * ir2 * ir2 * ir2;
             float lj = b.y * ir2 * t2 * (2.0f * t2 - 1.0f);
-            P.fx += lj * dx; P.fy += lj * dy; P.fz += lj * dz;
+            fx += lj * dx; fy += lj * dy; fz += lj * dz;
         }
     }
-    P.fx = gsum<L>(P.fx); P.fy = gsum<L>(P.fy); P.fz = gsum<L>(P.fz);
+    fx = gsum<L>(fx); fy = gsum<L>(fy); fz = gsum<L>(fz);
     if constexpr (FT == 4) {
-        P.ex = gsum<L>(P.ex); P.ey = gsum<L>(P.ey); P.ez = gsum<L>(P.ez);
-        P.sxx = gsum<L>(P.sxx); P.syy = gsum<L>(P.syy); P.szz = gsum<L>(P.szz);
-        P.sxy = gsum<L>(P.sxy); P.sxz = gsum<L>(P.sxz); P.syz = gsum<L>(P.syz);
-        double tr = (double)P.sxx + (double)P.syy + (double)P.szz;
-        double Txx = 3.0 * P.sxx - tr, Tyy = 3.0 * P.syy - tr, Tzz = 3.0 * P.szz - tr;
-        double Txy = 3.0 * P.sxy, Txz = 3.0 * P.sxz, Tyz = 3.0 * P.syz;
-        ox = (double)P.fx - ds * (Txx * P.ex + Txy * P.ey + Txz * P.ez);
-        oy = (double)P.fy - ds * (Txy * P.ex + Tyy * P.ey + Tyz * P.ez);
-        oz = (double)P.fz - ds * (Txz * P.ex + Tyz * P.ey + Tzz * P.ez);
+        ex = gsum<L>(ex); ey = gsum<L>(ey); ez = gsum<L>(ez);
+        sxx = gsum<L>(sxx); syy = gsum<L>(syy); szz = gsum<L>(szz);
+        sxy = gsum<L>(sxy); sxz = gsum<L>(sxz); syz = gsum<L>(syz);
+        double tr = (double)sxx + (double)syy + (double)szz;
+        double Txx = 3.0 * sxx - tr, Tyy = 3.0 * syy - tr, Tzz = 3.0 * szz - tr;
+        double Txy = 3.0 * sxy, Txz = 3.0 * sxz, Tyz = 3.0 * syz;
+        ox = (double)fx - ds * (Txx * ex + Txy * ey + Txz * ez);
+        oy = (double)fy - ds * (Txy * ex + Tyy * ey + Tyz * ez);
+        oz = (double)fz - ds * (Txz * ex + Tyz * ey + Tzz * ez);
     } else {
-        ox = P.fx; oy = P.fy; oz = P.fz;
+        ox = fx; oy = fy; oz = fz;
     }
 }
 #pragma clang fp contract(off)
 
 // FP64 reference-form pair terms (src/forceAccel.cpp:764-797, 865-880),
 // partial sums of this lane, then the group butterfly and the epilogue
-// (:800-802).
-template <int FT, int L>
-__device__ __forceinline__ void grp_force64(const double *__restrict__ X, int npad, int j, double px, double py,
-                                            double pz, double ds, double &ox, double &oy, double &oz)
+// (:800-802).  X: SoA x y z q s6 e24 with stride npad.
+template <int FT, int L, typename PD>
+__device__ __forceinline__ void grp_force64(PD X, int npad, int j, double px, double py, double pz, double ds,
+                                            double &ox, double &oy, double &oz)
 {
-    const double *Y = X + npad, *Z = X + 2 * npad, *Q = X + 3 * npad, *S6 = X + 4 * npad, *E24 = X + 5 * npad;
+    const PD Y = X + npad, Z = X + 2 * npad, Q = X + 3 * npad, S6 = X + 4 * npad, E24 = X + 5 * npad;
     double sXX = 0, sYY = 0, sZZ = 0, sXY = 0, sXZ = 0, sYZ = 0, dX = 0, dY = 0, dZ = 0, tx = 0, ty = 0, tz = 0;
 #pragma unroll 2
     for (int k = j; k < npad; k += L) {
@@ -160,67 +196,154 @@ __device__ __forceinline__ void grp_force64(const double *__restrict__ X, int np
     ox = tx; oy = ty; oz = tz;
 }
 
-// getPotential across the group (src/forceAccel.cpp:943-982), FP64.
-template <int L>
-__device__ __forceinline__ double grp_potential(const GroupArgs &A, int j, double px, double py, double pz)
+// Clustered force types with the clusters split over the group's lanes:
+// the reference's per-cluster terms (cl_one), partial accumulators summed
+// across the group, then the polarisation epilogue.
+template <typename T, int FT, int L>
+__device__ __forceinline__ void grp_clustered(const MolDev &m, int j, T px, T py, T pz, double &ox, double &oy,
+                                              double &oz)
 {
+    ClAcc<T> A;
+    cl_sum<T, FT>(m, j, L, px, py, pz, A);
+    A.sumX = gsum<L>(A.sumX); A.sumY = gsum<L>(A.sumY); A.sumZ = gsum<L>(A.sumZ);
+    if constexpr (FT != 5) {
+        A.fX = gsum<L>(A.fX); A.fY = gsum<L>(A.fY); A.fZ = gsum<L>(A.fZ);
+        A.dXX = gsum<L>(A.dXX); A.dYY = gsum<L>(A.dYY); A.dZZ = gsum<L>(A.dZZ);
+        A.sXY = gsum<L>(A.sXY); A.sXZ = gsum<L>(A.sXZ); A.sYZ = gsum<L>(A.sYZ);
+    }
+    if constexpr (FT == 1 || FT == 2) {
+        A.aXY = gsum<L>(A.aXY); A.aXZ = gsum<L>(A.aXZ); A.aYZ = gsum<L>(A.aYZ);
+    }
+    T a, b, c;
+    cl_epilogue<T, FT>(m, A, a, b, c);
+    ox = a; oy = b; oz = c;
+}
+
+// getPotential across the group (src/forceAccel.cpp:943-982), FP64, over
+// the SoA X (x y z q s6 e24 e4, stride npad).
+template <int L, typename PD>
+__device__ __forceinline__ double grp_potential(PD X, int npad, bool charged, double dipole_scalar, int j,
+                                                double px, double py, double pz)
+{
+    const PD Y = X + npad, Z = X + 2 * npad, Q = X + 3 * npad, S6 = X + 4 * npad, E4 = X + 6 * npad;
     double total = 0, sX = 0, sY = 0, sZ = 0;
-    const bool charged = A.m.charged;
-    for (int k = j; k < A.npad; k += L) {
-        double xD = px - A.sx[k], yD = py - A.sy[k], zD = pz - A.sz[k];
+    for (int k = j; k < npad; k += L) {
+        double xD = px - X[k], yD = py - Y[k], zD = pz - Z[k];
         double r2 = xD * xD + yD * yD + zD * zD;
         double r6 = r2 * r2 * r2;
-        double t2 = A.s6[k] / r6;
+        double t2 = S6[k] / r6;
         double t1 = t2 * t2;
-        total += A.se4[k] * (t1 - t2);
+        total += E4[k] * (t1 - t2);
         if (charged) {
-            double r_3 = A.sq[k] / (r2 * __builtin_sqrt(r2));
+            double r_3 = Q[k] / (r2 * __builtin_sqrt(r2));
             sX += xD * r_3; sY += yD * r_3; sZ += zD * r_3;
         }
     }
     total = gsum<L>(total);
     if (charged) {
         sX = gsum<L>(sX); sY = gsum<L>(sY); sZ = gsum<L>(sZ);
-        total -= A.m.dipole_scalar * 0.5 * (sX * sX + sY * sY + sZ * sZ);
+        total -= dipole_scalar * 0.5 * (sX * sX + sY * sY + sZ * sZ);
     }
     return total;
 }
 
-enum : int { PH_IDLE = 0, PH_INIT = 1, PH_STEP = 2, PH_CHECK = 3 };
-
+// Molecule source of k_traj_grp: one molecule, atoms in LDS, constants in
+// kernel arguments (wave-uniform).
 template <int PREC, int FT, int L>
-__global__ void __launch_bounds__(1024) k_traj_grp(GroupArgs A)
-{
-    extern __shared__ __attribute__((aligned(16))) char smem[];
-    const MolDev &m = A.m;
-    const int npad = A.npad;
-    // stage the molecule (once per workgroup; the kernel is persistent)
-    if constexpr (PREC == 1) {
-        float4 *xq = (float4 *)smem;
-        float2 *se = (float2 *)(xq + npad);
-        for (int i = threadIdx.x; i < npad; i += blockDim.x) {
-            xq[i] = make_float4((float)A.sx[i], (float)A.sy[i], (float)A.sz[i], (float)A.sq[i]);
-            se[i] = make_float2((float)A.s6[i], (float)A.se24[i]);
-        }
-    } else {
-        double *X = (double *)smem;
-        for (int i = threadIdx.x; i < npad; i += blockDim.x) {
-            X[i] = A.sx[i]; X[npad + i] = A.sy[i]; X[2 * npad + i] = A.sz[i];
-            X[3 * npad + i] = A.sq[i]; X[4 * npad + i] = A.s6[i]; X[5 * npad + i] = A.se24[i];
+struct LdsSource {
+    const GroupArgs &A;
+    const char *smem;
+    __device__ void bind(int) {}
+    __device__ int pairs() const { return (FT == 4 || FT == 6) ? A.m.n : 0; }
+    __device__ double mu() const { return A.m.mu; }
+    __device__ double dt0() const { return A.dt0; }
+    __device__ bool exits(double px, double py, double pz) const
+    {
+        const double tx = px / A.m.a, ty = py / A.m.b, tz = pz / A.m.c;
+        const double tm2 = tx * tx + ty * ty + tz * tz;
+        return (tm2 > 1 && A.m.projection) || (tm2 > A.m.maxrad && !A.m.projection);
+    }
+    // idle groups evaluate at a far point: the trip count stays wave-uniform
+    __device__ void force(bool, int j, double ex, double ey, double ez, double &fx, double &fy, double &fz) const
+    {
+        const int npad = A.npad;
+        if constexpr (PREC == 1) {
+            const f4n *xq = (const f4n *)smem;
+            grp_force32<FT, L>(xq, (const f2n *)(xq + npad), npad, j, (float)ex, (float)ey, (float)ez,
+                               A.m.dipole_scalar, fx, fy, fz);
+        } else {
+            grp_force64<FT, L>((const double *)smem, npad, j, ex, ey, ez, A.m.dipole_scalar, fx, fy, fz);
         }
     }
-    __syncthreads();
+    __device__ double potential(bool, int j, double ex, double ey, double ez) const
+    {
+        return grp_potential<L>(glb(A.soa), A.npad, A.m.charged, A.m.dipole_scalar, j, ex, ey, ez);
+    }
+};
 
+// Molecule source of k_traj_multi: the molecule of the group's trajectory,
+// read from the MolRec table.  Idle groups skip the sums.
+template <int PREC, int FT, int L>
+struct TableSource {
+    const MultiArgs &A;
+    int mi = 0;
+    __device__ void bind(int t) { mi = A.tmol ? A.tmol[t] : 0; }
+    __device__ const MolRec &rec() const { return A.mols[mi]; }
+    __device__ int pairs() const { return (FT == 4 || FT == 6) ? rec().m.n : 0; }
+    __device__ double mu() const { return rec().m.mu; }
+    __device__ double dt0() const { return rec().dt0; }
+    __device__ bool exits(double px, double py, double pz) const
+    {
+        const MolDev &m = rec().m;
+        const double tx = px / m.a, ty = py / m.b, tz = pz / m.c;
+        const double tm2 = tx * tx + ty * ty + tz * tz;
+        return (tm2 > 1 && m.projection) || (tm2 > m.maxrad && !m.projection);
+    }
+    __device__ void force(bool active, int j, double ex, double ey, double ez, double &fx, double &fy,
+                          double &fz) const
+    {
+        const MolRec &r = rec();
+        if constexpr (FT == 4 || FT == 6) {
+            const int npad = active ? r.npad : 0;
+            if constexpr (PREC == 1)
+                grp_force32<FT, L>(glb((const f4n *)r.xq), glb((const f2n *)r.se), npad, j, (float)ex, (float)ey,
+                                   (float)ez, r.m.dipole_scalar, fx, fy, fz);
+            else
+                grp_force64<FT, L>(glb(r.soa), npad, j, ex, ey, ez, r.m.dipole_scalar, fx, fy, fz);
+        } else {
+            MolDev m = r.m;
+            if (!active) m.ncl = 0;
+            if constexpr (PREC == 1)
+                grp_clustered<float, FT, L>(m, j, (float)ex, (float)ey, (float)ez, fx, fy, fz);
+            else
+                grp_clustered<double, FT, L>(m, j, ex, ey, ez, fx, fy, fz);
+        }
+    }
+    __device__ double potential(bool active, int j, double ex, double ey, double ez) const
+    {
+        const MolRec &r = rec();
+        return grp_potential<L>(glb(r.soa), active ? r.npad : 0, r.m.charged, r.m.dipole_scalar, j, ex, ey, ez);
+    }
+};
+
+enum : int { PH_IDLE = 0, PH_INIT = 1, PH_STEP = 2, PH_CHECK = 3 };
+
+// The trajectory state machine of diffEqSolver::verletVel
+// (src/diffEqSolver.cpp:81-211) for the group of lanes this lane belongs
+// to, refilled from the work list until it is exhausted.
+template <int L, class Src>
+__device__ __forceinline__ void group_loop(const TrajIO &A, Src &S)
+{
     const int lane = threadIdx.x & 63;
     const int g0 = lane & ~(L - 1);   // first lane of my group
     const int j = lane & (L - 1);     // my index in the group
-    const double hm = 0.5 * m.mu;
     int t = 0, phase = PH_IDLE, tries = 0, maxtries = 8, orbits = 0, steps = 0, tot = 0;
     bool needE0 = false;
     double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0, ax = 0, ay = 0, az = 0, dt = 0, rdot = 0;
     double e0 = 0, chi = 0;
     bool exhausted = false;
     unsigned n_steps = 0, n_evals = 0, n_pots = 0;
+    unsigned long long n_pairs = 0;
 
     for (;;) {
         if (!exhausted) {
@@ -237,8 +360,9 @@ __global__ void __launch_bounds__(1024) k_traj_grp(GroupArgs A)
                     const int idx = base + __popcll(mask & ((1ull << g0) - 1));
                     if (idx < A.n_items) {
                         t = A.items[idx];
+                        S.bind(t);
                         tries = 0; maxtries = 8;
-                        dt = A.dt0;
+                        dt = S.dt0();
                         px = A.p0[3 * t]; py = A.p0[3 * t + 1]; pz = A.p0[3 * t + 2];
                         vx = A.v0[3 * t]; vy = A.v0[3 * t + 1]; vz = A.v0[3 * t + 2];
                         tot = 0;
@@ -266,25 +390,18 @@ __global__ void __launch_bounds__(1024) k_traj_grp(GroupArgs A)
         const bool needF = (phase == PH_INIT || phase == PH_STEP);
         const bool needU = (phase == PH_INIT && needE0) || phase == PH_CHECK;
         double fx = 0, fy = 0, fz = 0, u = 0;
-        if (__any(needF)) {
-            if constexpr (PREC == 1) {
-                const float4 *xq = (const float4 *)smem;
-                grp_force32<FT, L>(xq, (const float2 *)(xq + npad), npad, j, (float)ex, (float)ey, (float)ez,
-                                   m.dipole_scalar, fx, fy, fz);
-            } else {
-                grp_force64<FT, L>((const double *)smem, npad, j, ex, ey, ez, m.dipole_scalar, fx, fy, fz);
-            }
-        }
-        if (__any(needU)) u = grp_potential<L>(A, j, ex, ey, ez);
+        if (__any(needF)) S.force(needF, j, ex, ey, ez, fx, fy, fz);
+        if (__any(needU)) u = S.potential(needU, j, ex, ey, ez);
         if (j == 0) {
             n_evals += needF;
             n_steps += (phase == PH_STEP);
             n_pots += needU;
+            if (needF) n_pairs += (unsigned)S.pairs();
         }
 
         if (phase == PH_INIT) {  // src/diffEqSolver.cpp:112-127
             if (needE0) {
-                e0 = u + hm * (vx * vx + vy * vy + vz * vz);
+                e0 = u + 0.5 * S.mu() * (vx * vx + vy * vy + vz * vz);
                 needE0 = false;
             }
             ax = fx; ay = fy; az = fz;
@@ -313,6 +430,8 @@ __global__ void __launch_bounds__(1024) k_traj_grp(GroupArgs A)
                 orbits++;
                 if (orbits > 8) track = false;
             }
+            // A non-finite position or a runaway attempt can never reach the
+            // exit test; it ends as a failed trajectory (status 2).
             const bool runaway = !(fabs(px) + fabs(py) + fabs(pz) < 1.0e30) || steps > A.max_steps;
             if (runaway) {
                 if (j == 0) {
@@ -321,7 +440,7 @@ __global__ void __launch_bounds__(1024) k_traj_grp(GroupArgs A)
                 phase = PH_IDLE;
             } else if (!track) {
                 tries++;
-                dt = A.dt0 / (double)(tries + 1);
+                dt = S.dt0() / (double)(tries + 1);
                 if (!(tries < maxtries && maxtries < 10)) {
                     if (j == 0) {
                         A.status[t] = 2; A.chi[t] = 1.57079632679489661923; A.steps[t] = steps;
@@ -329,24 +448,20 @@ __global__ void __launch_bounds__(1024) k_traj_grp(GroupArgs A)
                     }
                     phase = PH_IDLE;
                 }
-            } else {
-                const double tx = px / m.a, ty = py / m.b, tz = pz / m.c;
-                const double tm2 = tx * tx + ty * ty + tz * tz;
-                if (((tm2 > 1 && m.projection) || (tm2 > m.maxrad && !m.projection)) && steps > 10) {
-                    const double w0 = A.v0[3 * t], w1 = A.v0[3 * t + 1], w2 = A.v0[3 * t + 2];
-                    const double t1 = w0 * vx + w1 * vy + w2 * vz;
-                    const double t2 = sqrt(w0 * w0 + w1 * w1 + w2 * w2) * sqrt(vx * vx + vy * vy + vz * vz);
-                    chi = t1 / t2;  // cos(chi); acos is taken after the launch (k_chi)
-                    phase = PH_CHECK;
-                }
+            } else if (S.exits(px, py, pz) && steps > 10) {
+                const double w0 = A.v0[3 * t], w1 = A.v0[3 * t + 1], w2 = A.v0[3 * t + 2];
+                const double t1 = w0 * vx + w1 * vy + w2 * vz;
+                const double t2 = sqrt(w0 * w0 + w1 * w1 + w2 * w2) * sqrt(vx * vx + vy * vy + vz * vz);
+                chi = t1 / t2;  // cos(chi); acos is taken after the launch (k_chi)
+                phase = PH_CHECK;
             }
         } else if (phase == PH_CHECK) {  // src/diffEqSolver.cpp:178-203
-            const double e1 = u + hm * (vx * vx + vy * vy + vz * vz);
+            const double e1 = u + 0.5 * S.mu() * (vx * vx + vy * vy + vz * vz);
             const double pct = 100 * fabs(e0 - e1) / e0;
             if (pct > 0.5) {
                 tries++;
                 maxtries++;
-                dt = A.dt0 / (double)(tries + 1);
+                dt = S.dt0() / (double)(tries + 1);
                 if (tries < maxtries && maxtries < 10) {
                     px = A.p0[3 * t]; py = A.p0[3 * t + 1]; pz = A.p0[3 * t + 2];
                     vx = A.v0[3 * t]; vy = A.v0[3 * t + 1]; vz = A.v0[3 * t + 2];
@@ -365,17 +480,49 @@ __global__ void __launch_bounds__(1024) k_traj_grp(GroupArgs A)
             }
         }
     }
-    unsigned long long s = n_steps, e = n_evals, p = n_pots;
+    unsigned long long s = n_steps, e = n_evals, p = n_pots, q = n_pairs;
     for (int off = 32; off > 0; off >>= 1) {
         s += __shfl_xor(s, off);
         e += __shfl_xor(e, off);
         p += __shfl_xor(p, off);
+        q += __shfl_xor(q, off);
     }
     if (lane == 0) {
         atomicAdd(A.evals, s);
         atomicAdd(A.evals + 1, e);
         atomicAdd(A.evals + 2, p);
+        atomicAdd(A.evals + 3, q);
     }
+}
+
+template <int PREC, int FT, int L>
+__global__ void __launch_bounds__(1024) k_traj_grp(GroupArgs A)
+{
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    const int npad = A.npad;
+    const double *S = A.soa;
+    // stage the molecule (once per workgroup; the kernel is persistent)
+    if constexpr (PREC == 1) {
+        float4 *xq = (float4 *)smem;
+        float2 *se = (float2 *)(xq + npad);
+        for (int i = threadIdx.x; i < npad; i += blockDim.x) {
+            xq[i] = make_float4((float)S[i], (float)S[npad + i], (float)S[2 * npad + i], (float)S[3 * npad + i]);
+            se[i] = make_float2((float)S[4 * npad + i], (float)S[5 * npad + i]);
+        }
+    } else {
+        double *X = (double *)smem;
+        for (int i = threadIdx.x; i < 6 * npad; i += blockDim.x) X[i] = S[i];
+    }
+    __syncthreads();
+    LdsSource<PREC, FT, L> src{A, smem};
+    group_loop<L>(A.io, src);
+}
+
+template <int PREC, int FT, int L>
+__global__ void __launch_bounds__(1024) k_traj_multi(MultiArgs A)
+{
+    TableSource<PREC, FT, L> src{A};
+    group_loop<L>(A.io, src);
 }
 
 }  // namespace cosims

@@ -25,8 +25,8 @@ EXPORTS = [
     "cosims_mol_create", "cosims_mol_create_prepared", "cosims_mol_destroy", "cosims_mol_upload", "cosims_mol_info_get",
     "cosims_mol_atoms", "cosims_mol_clusters", "cosims_accel", "cosims_potential", "cosims_sample",
     "cosims_sample_reference",
-    "cosims_verlet", "cosims_run", "cosims_strerror", "cosims_last_error", "cosims_device_info",
-    "cosims_device_sync",
+    "cosims_verlet", "cosims_run", "cosims_run_batch", "cosims_tune_batch", "cosims_strerror",
+    "cosims_last_error", "cosims_device_info", "cosims_device_sync",
 ]
 
 
@@ -40,7 +40,8 @@ class Params(C.Structure):
                 ("dispersion_cutoff", C.c_int32), ("dispersion_radius", C.c_double),
                 ("multipole_radius", C.c_double), ("multipole_order", C.c_int32),
                 ("max_cluster_size", C.c_int32), ("projection", C.c_int32), ("print_rate", C.c_int32),
-                ("total_charge", C.c_double), ("threads", C.c_int32), ("name", C.c_char * 128)]
+                ("total_charge", C.c_double), ("threads", C.c_int32), ("name", C.c_char * 128),
+                ("setup_gpu", C.c_int32)]
 
 
 class MolFile(C.Structure):
@@ -90,6 +91,22 @@ class Ccs(C.Structure):
                 ("sample_kernel_ms", C.c_double), ("lanes", C.c_int32), ("wg_size", C.c_int32)]
 
 
+class BatchJob(C.Structure):
+    _fields_ = [("mol", C.c_void_p), ("seed", C.c_uint64), ("job", C.c_uint32), ("iter", C.c_int32),
+                ("traj", C.c_int64), ("omega", C.c_void_p), ("omega_err", C.c_void_p), ("out", C.POINTER(Ccs))]
+
+
+class BatchCfg(C.Structure):
+    _fields_ = [("prec", C.c_int32), ("lanes", C.c_int32), ("wg_size", C.c_int32), ("blocks_per_cu", C.c_int32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("lanes", C.c_int32), ("wg_size", C.c_int32), ("launches", C.c_int32),
+                ("kernel_ms", C.c_double), ("sample_ms", C.c_double), ("run_ms", C.c_double)] + \
+               [(k, C.c_int64) for k in ("n_traj", "n_hit", "total_steps", "force_evals", "pot_evals",
+                                          "pair_evals")]
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -116,6 +133,9 @@ def lib():
         L.cosims_sample_reference.argtypes = [vp, C.c_uint64, C.c_int32, C.c_int64, vp, vp, vp, vp]
         L.cosims_verlet.argtypes = [vp, C.c_int32, C.c_int64, vp, vp, vp, vp, vp, vp, vp]
         L.cosims_run.argtypes = [vp, C.POINTER(Params), C.POINTER(RunCfg), vp, vp, C.POINTER(Ccs)]
+        L.cosims_run_batch.argtypes = [C.POINTER(BatchJob), C.c_int32, C.POINTER(BatchCfg), C.POINTER(BatchStats)]
+        L.cosims_tune_batch.argtypes = [C.POINTER(BatchJob), C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_int32)]
         L.cosims_strerror.restype = C.c_char_p
         L.cosims_strerror.argtypes = [C.c_int]
         L.cosims_last_error.restype = C.c_char_p
@@ -333,3 +353,41 @@ class Molecule:
         if keep:
             out.update(chi=chi, status=status, g=g)
         return out
+
+
+def _batch_jobs(jobs):
+    """jobs: list of (Molecule, dict(seed=, job=, iters=, traj=))."""
+    arr = (BatchJob * len(jobs))()
+    keep = []
+    for k, (m, spec) in enumerate(jobs):
+        it = spec.get("iters") or m.params.iter
+        T = spec.get("traj") or m.params.traj
+        om = np.empty(it); err = np.empty(it); res = Ccs()
+        keep.append((om, err, res))
+        arr[k] = BatchJob(mol=m._h, seed=spec.get("seed", m.params.seed), job=spec.get("job", 0), iter=it, traj=T,
+                          omega=_p(om), omega_err=_p(err), out=C.pointer(res))
+    return arr, keep
+
+
+def tune_batch(jobs, prec=PREC_MIXED):
+    """Launch shape (lanes per trajectory, workgroup size) tuned on a batch."""
+    arr, _ = _batch_jobs(jobs)
+    L = C.c_int32(); W = C.c_int32()
+    _check(lib().cosims_tune_batch(arr, len(jobs), prec, C.byref(L), C.byref(W)))
+    return L.value, W.value
+
+
+def run_batch(jobs, prec=PREC_MIXED, lanes=0, wg_size=0, blocks_per_cu=0):
+    """One persistent launch per force type over all jobs; returns (per-job
+    result dicts like Molecule.run, batch statistics)."""
+    arr, keep = _batch_jobs(jobs)
+    cfg = BatchCfg(prec=prec, lanes=lanes, wg_size=wg_size, blocks_per_cu=blocks_per_cu)
+    st = BatchStats()
+    _check(lib().cosims_run_batch(arr, len(jobs), C.byref(cfg), C.byref(st)))
+    out = []
+    for om, err, res in keep:
+        d = {k: getattr(res, k) for k, _ in Ccs._fields_ if k not in ("traj_kernel_ms", "round_steps",
+                                                                     "round_evals", "round_items")}
+        d.update(omega=om, omega_err=err)
+        out.append(d)
+    return out, {k: getattr(st, k) for k, _ in BatchStats._fields_}

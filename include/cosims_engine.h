@@ -59,6 +59,8 @@ typedef struct cosims_params {
     double total_charge;      /* totalCharge_ext (informational) */
     int32_t threads;          /* host threads for setup; 0 => all */
     char name[128];           /* name_ext */
+    int32_t setup_gpu;        /* 1: the boundary search's potential marches run on the GPU
+                                 (same points as the host loop); 0: host threads */
 } cosims_params;
 
 /* fileParams::setDefaultParams  src/fileParams.cpp:203-259 */
@@ -217,6 +219,44 @@ typedef struct cosims_ccs {
 /* omega/omega_err: iter entries each. */
 int cosims_run(const cosims_mol *m, const cosims_params *p, const cosims_run_cfg *cfg, double *omega,
                double *omega_err, cosims_ccs *out);
+
+/* Many CCS runs in one persistent launch per force type.  The reference
+ * runs one molecule per process (src/main.cpp:119-140, one
+ * monteTraj::runTraj each); here molecules of uneven size share the
+ * launch, trajectories of all of them on one work list (larger molecules
+ * first), each group of lanes reading the atoms of its own trajectory's
+ * molecule.  Each job's results are those cosims_run gives for the same
+ * molecule, seed, job id and launch shape. */
+typedef struct cosims_batch_job {
+    const cosims_mol *mol;
+    uint64_t seed;            /* Philox key (seed_ext) */
+    uint32_t job;             /* Philox key extension */
+    int32_t iter;             /* CCS integrals; 0 => the molecule's params */
+    int64_t traj;             /* trajectories per integral; 0 => the molecule's params */
+    double *omega, *omega_err;/* iter entries each (caller-owned) */
+    cosims_ccs *out;          /* per-job statistics */
+} cosims_batch_job;
+
+typedef struct cosims_batch_cfg {
+    int32_t prec;             /* COSIMS_PREC_* */
+    int32_t lanes;            /* lanes per trajectory 4..64; 0 => tuned on the batch */
+    int32_t wg_size;          /* 0 => tuned */
+    int32_t blocks_per_cu;    /* 0 => occupancy limit */
+} cosims_batch_cfg;
+
+typedef struct cosims_batch_stats {
+    int32_t lanes, wg_size, launches;
+    double kernel_ms;         /* HIP-event time of the trajectory launches */
+    double sample_ms, run_ms; /* sampling launches; whole call */
+    int64_t n_traj, n_hit, total_steps, force_evals, pot_evals;
+    int64_t pair_evals;       /* atom pairs of the exact force types' force evaluations */
+} cosims_batch_stats;
+
+int cosims_run_batch(cosims_batch_job *jobs, int32_t njobs, const cosims_batch_cfg *cfg,
+                     cosims_batch_stats *stats);
+/* The launch shape cosims_run_batch would tune for these jobs. */
+int cosims_tune_batch(const cosims_batch_job *jobs, int32_t njobs, int32_t prec, int32_t *lanes,
+                      int32_t *wg_size);
 
 const char *cosims_strerror(int code);
 const char *cosims_last_error(void);

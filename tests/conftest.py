@@ -58,6 +58,26 @@ def golden_engine_mol(g, flags):
                                     sc["dispersion_radius"], sc["multipole_radius"])
 
 
+def golden_engine_setup(g, threads=0, setup_gpu=0):
+    """Engine molecule set up from the case's raw input files, as the
+    reference's main.cpp reads them."""
+    from cosims_amd import engine as E
+    f = E.read_molfile(os.path.join(GOLDEN, g["mol"]),
+                       os.path.join(GOLDEN, g["atoms_dat"]) if g["atoms_dat"] else None)
+    p = E.default_params()
+    if g["input"]:
+        import tempfile
+        with tempfile.NamedTemporaryFile("w", suffix=".in", delete=False) as t:
+            t.write(g["input"])
+        E.read_params(t.name, p)
+        os.unlink(t.name)
+    p.charge = int(f["charged"])
+    p.threads = threads
+    p.setup_gpu = setup_gpu
+    return E.Molecule.create(p, f["xyz"], f["mass"], f["sigma"], f["epsilon"],
+                             f["charge"] if f["charged"] else None, f["sigma_max"])
+
+
 @pytest.fixture(scope="session")
 def oracle_lib():
     import oracle

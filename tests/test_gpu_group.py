@@ -1,0 +1,145 @@
+"""Group kernels beyond the LDS-staged exact types, and batched runs (MI355X).
+
+The lane-per-trajectory FP64 kernel (lanes=1) sums in the reference's atom /
+cluster order and is pinned bit-for-bit to the reference in
+test_gpu_parity.py; here the group kernels are checked against it on the
+same Philox initial conditions.
+
+Tolerances: FP64 group vs FP64 lane kernel, per trajectory, median |dchi|
+<= 1e-8 rad on trajectories both accept, success flags equal on >= 99 %,
+per-integral CCS within 5e-4; mixed precision vs FP64 on paired initial
+conditions: CCS within 1e-3.  Batched runs: every job's per-integral CCS is
+bit-identical to the same job run alone (same launch shape), and to
+cosims_run of the same molecule.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from cosims_amd import engine as E
+from cosims_amd import synth
+
+from conftest import golden, golden_engine_mol
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+# (case, flags) -> force type: [1,1,1] 1, [1,1,0] 2, [1,0,1] 3, [0,1,1] 5
+CLUSTERED = [("acau_charged", [1, 1, 1, 1]), ("acau_charged", [1, 1, 0, 1]), ("acau_charged", [1, 0, 1, 1]),
+             ("acau_charged", [1, 1, 1, 2]), ("acau_neutral", [0, 1, 1, 1]), ("ion2000", [1, 1, 1, 1])]
+
+
+def _paired(a, b):
+    both = (a["status"] == 1) & (b["status"] == 1)
+    return both, np.abs(a["chi"][both] - b["chi"][both])
+
+
+@pytest.mark.parametrize("case,flags", CLUSTERED, ids=[f"{c}-{''.join(map(str, f))}" for c, f in CLUSTERED])
+def test_clustered_group_fp64_matches_lane_kernel(case, flags):
+    m = golden_engine_mol(golden(case), flags)
+    ft = m.info()["force_type"]
+    assert ft in (1, 2, 3, 5)
+    kw = dict(job=3, iters=2, traj=2000, keep=True)
+    lane = m.run(E.PREC_FP64, lanes=1, **kw)
+    for L in (4, 16):
+        grp = m.run(E.PREC_FP64, lanes=L, wg_size=256, **kw)
+        assert grp["lanes"] == L
+        assert np.array_equal(grp["status"] != 0, lane["status"] != 0)
+        assert np.mean(grp["status"] == lane["status"]) >= 0.99
+        both, d = _paired(grp, lane)
+        assert np.median(d) <= 1e-8, (ft, L, np.median(d))
+        np.testing.assert_allclose(grp["omega"], lane["omega"], rtol=5e-4)
+
+
+@pytest.mark.parametrize("case,flags", [CLUSTERED[0], CLUSTERED[4], CLUSTERED[5]],
+                         ids=["acau_charged-1111", "acau_neutral-0111", "ion2000-1111"])
+def test_clustered_group_mixed_close_to_fp64(case, flags):
+    m = golden_engine_mol(golden(case), flags)
+    kw = dict(job=4, iters=2, traj=4000, keep=True)
+    f64 = m.run(E.PREC_FP64, lanes=1, **kw)
+    mix = m.run(E.PREC_MIXED, **kw)
+    assert mix["lanes"] >= 4
+    assert np.array_equal(mix["status"] != 0, f64["status"] != 0)
+    dev = abs(mix["wmean"] - f64["wmean"]) / f64["wmean"]
+    assert dev <= 1e-3, dev
+
+
+@pytest.fixture(scope="module")
+def ion128():
+    p = E.default_params(temp=298.0, traj=3000, iter=2, multipole=0, dispersion_cutoff=0, seed=77)
+    return E.Molecule.from_file(os.path.join(GOLD, "ion128.mfj"), params=p)
+
+
+@pytest.fixture(scope="module")
+def ion2000():
+    p = E.default_params(temp=298.0, traj=2000, iter=2, multipole=0, dispersion_cutoff=0, seed=78)
+    return E.Molecule.from_file(os.path.join(GOLD, "ion2000.mfj"), params=p)
+
+
+@pytest.mark.parametrize("prec", [E.PREC_MIXED, E.PREC_FP64])
+def test_batch_of_one_equals_run(ion128, prec):
+    """Table kernel (batch) and LDS kernel (cosims_run) give identical results."""
+    shape = dict(lanes=8, wg_size=256)
+    single = ion128.run(prec, job=11, **shape)
+    (b,), st = E.run_batch([(ion128, dict(job=11))], prec=prec, **shape)
+    assert st["lanes"] == 8 and st["launches"] == 1
+    assert np.array_equal(b["omega"], single["omega"])
+    assert b["n_hit"] == single["n_hit"] and b["total_steps"] == single["total_steps"]
+
+
+def test_batch_alone_equals_mix(ion128, ion2000):
+    """Uneven molecules and two force types in one call: each job's result
+    is the same alone and in the mix."""
+    g = golden("acau_charged")
+    ac1 = golden_engine_mol(g, [1, 1, 1, 1])   # clustered, type 1
+    ac4 = golden_engine_mol(g, [1, 0, 0, 1])   # exact, type 4
+    jobs = [(ion128, dict(job=1)), (ion2000, dict(job=2, iters=1, traj=3000)),
+            (ac1, dict(job=3, iters=2, traj=1500)), (ac4, dict(job=4, iters=1, traj=2500)),
+            (ion128, dict(job=5, iters=1, traj=1000))]
+    shape = dict(lanes=16, wg_size=512)
+    mix, st = E.run_batch(jobs, prec=E.PREC_MIXED, **shape)
+    assert st["launches"] == 2
+    assert st["n_traj"] == sum(j[1].get("iters", j[0].params.iter) * j[1].get("traj", j[0].params.traj)
+                               for j in jobs)
+    for k, job in enumerate(jobs):
+        (alone,), _ = E.run_batch([job], prec=E.PREC_MIXED, **shape)
+        assert np.array_equal(mix[k]["omega"], alone["omega"]), k
+        assert mix[k]["wmean"] == alone["wmean"]
+
+
+def test_batch_tuner_and_stats(ion128, ion2000):
+    jobs = [(ion128, dict(job=21, iters=1, traj=2000)), (ion2000, dict(job=22, iters=1, traj=2000))]
+    L, W = E.tune_batch(jobs, E.PREC_MIXED)
+    assert L in (4, 8, 16, 32, 64) and W in (256, 512, 1024)
+    res, st = E.run_batch(jobs, prec=E.PREC_MIXED)  # tunes again: timing may pick another shape
+    assert st["lanes"] in (4, 8, 16, 32, 64) and st["wg_size"] in (256, 512, 1024)
+    assert st["kernel_ms"] > 0 and st["force_evals"] > st["total_steps"] > 0
+    # every force evaluation of these exact-type molecules is a full atom sweep
+    assert st["pair_evals"] >= 128 * st["force_evals"]
+    for r in res:
+        assert r["n_traj"] == 2000 and 0 < r["n_hit"] <= 2000 and np.isfinite(r["wmean"])
+
+
+@pytest.fixture(scope="module")
+def ion8000():
+    """8,000 atoms: beyond one workgroup's LDS in FP32 and FP64 (table kernel)."""
+    import tempfile
+    ion = synth.synthetic_ion(8000, 18.0 * 4 ** (1 / 3), 0.3, 33, 3001)
+    path = os.path.join(tempfile.mkdtemp(prefix="cosims_t_"), "ion8000.mfj")
+    synth.write_mfj(path, "ion8000", ion)
+    p = E.default_params(temp=298.0, traj=1000, iter=1, multipole=0, dispersion_cutoff=0, seed=79)
+    return E.Molecule.from_file(path, params=p)
+
+
+def test_large_molecule_table_kernel(ion8000):
+    kw = dict(job=6, iters=1, traj=1000, keep=True)
+    lane = ion8000.run(E.PREC_FP64, lanes=1, **kw)
+    grp = ion8000.run(E.PREC_FP64, lanes=16, wg_size=256, **kw)
+    mix = ion8000.run(E.PREC_MIXED, lanes=16, wg_size=256, **kw)
+    both, d = _paired(grp, lane)
+    assert both.sum() > 0.9 * (lane["status"] != 0).sum()
+    assert np.median(d) <= 1e-8, np.median(d)
+    np.testing.assert_allclose(grp["omega"], lane["omega"], rtol=5e-4)
+    assert abs(mix["wmean"] - lane["wmean"]) / lane["wmean"] <= 1e-3

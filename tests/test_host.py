@@ -10,7 +10,7 @@ import oracle
 from oracle import molio
 from cosims_amd import engine as E
 
-from conftest import CASES, GOLDEN, ROOT, golden
+from conftest import CASES, GOLDEN, ROOT, golden, golden_engine_setup
 
 
 def test_library_exports_every_declared_symbol():
@@ -66,19 +66,7 @@ def test_molfile_reader_matches_oracle_reader(mol, atoms):
 
 
 def _engine_setup(g, threads=0):
-    f = E.read_molfile(os.path.join(GOLDEN, g["mol"]),
-                       os.path.join(GOLDEN, g["atoms_dat"]) if g["atoms_dat"] else None)
-    p = E.default_params()
-    if g["input"]:
-        import tempfile
-        with tempfile.NamedTemporaryFile("w", suffix=".in", delete=False) as t:
-            t.write(g["input"])
-        E.read_params(t.name, p)
-        os.unlink(t.name)
-    p.charge = int(f["charged"])
-    p.threads = threads
-    return E.Molecule.create(p, f["xyz"], f["mass"], f["sigma"], f["epsilon"],
-                             f["charge"] if f["charged"] else None, f["sigma_max"])
+    return golden_engine_setup(g, threads=threads)
 
 
 @pytest.mark.parametrize("case", CASES)
