@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(256) k_sample(SampleArgs A)
         int64_t tj = A.first + j;
         uint32_t it = A.iteration0, traj = (uint32_t)tj;
         if (A.per_iter > 0) {
-            it = (uint32_t)(tj / A.per_iter);
+            it = A.iteration0 + (uint32_t)(tj / A.per_iter);
             traj = (uint32_t)(tj % A.per_iter);
         }
         double g = 0, phi = 0, theta = 0, gamma = 0, b = 0;
@@ -502,11 +502,23 @@ int device_init(Device &d)
     return 0;
 }
 
-// RAII device buffer
+// RAII device buffer; alloc() replaces any previous allocation.
 struct DBuf {
     void *p = nullptr;
-    ~DBuf() { if (p) (void)hipFree(p); }
-    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 16); }
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { reset(); }
+    void reset()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    }
+    hipError_t alloc(size_t bytes)
+    {
+        reset();
+        return hipMalloc(&p, bytes ? bytes : 16);
+    }
     template <typename T> T *as() const { return (T *)p; }
 };
 
@@ -643,6 +655,7 @@ struct cosims_mol {
     mutable void *dmem = nullptr;
     mutable const double *soa = nullptr;  // 7 FP64 arrays of npad: x y z q s6 e24 e4
     mutable int npad = 0;
+    mutable size_t dbytes = 0;            // device bytes of the molecule
     mutable MolRec rec{};                 // table record of this molecule (device pointers)
     mutable const MolRec *rec_dev = nullptr;  // the same record in device memory
     // autotuned launch shape per precision: lanes per trajectory, workgroup size
@@ -742,6 +755,7 @@ int upload(const cosims_mol *M)
         se[i] = make_float2((float)soa[4 * npad + i], (float)soa[5 * npad + i]);
     }
     HIP_CHECK(hipMalloc(&M->dmem, total));
+    M->dbytes = total;
     char *d = (char *)M->dmem;
     MolDev &m = M->md;
     m.n = n; m.ncl = ncl; m.force_type = p.force_type; m.charged = p.charged ? 1 : 0;
@@ -814,6 +828,7 @@ const char *cosims_strerror(int code)
     case COSIMS_E_IO: return "file could not be read";
     case COSIMS_E_UNSUPPORTED: return "input outside the defined behaviour of the reference";
     case COSIMS_E_NODEVICE: return "no usable gfx950 device";
+    case COSIMS_E_PENDING: return "request not finished";
     default: return "unknown error";
     }
 }
@@ -1346,36 +1361,42 @@ int sample_into(const cosims_mol *M, uint64_t seed, uint32_t job, int64_t n, int
     return 0;
 }
 
-// Per-integral CCS and the final statistics of monteTraj::runTraj from the
-// per-integral sums of k_reduce (total, total^2, max ergy, steps, hits,
-// fails) and the trajectory status codes.
-void finish_ccs(const int8_t *status, const double *sums, int iters, int64_t T, int64_t print_rate, double *omega,
-                double *omega_err, cosims_ccs *out)
+// Per-integral sums of k_reduce: total, total^2, max ergy, steps, hits, fails.
+constexpr int kSums = 6;
+
+// CCS of one integral (src/monteTraj.cpp:378-386) from its sums and the
+// trajectory status codes.  Denominator currSuccTraj of one OpenMP thread
+// (src/monteTraj.cpp:456-521): failures (a miss repeats the previous success
+// flag) counted since the last progress update every print_rate trajectories.
+void integral_ccs(const int8_t *st8, const double *sums, int64_t T, int64_t print_rate, double &omega,
+                  double &omega_err)
 {
-    // Denominator currSuccTraj of one OpenMP thread (src/monteTraj.cpp:456-521):
-    // failures (a miss repeats the previous success flag) counted since the
-    // last progress update every print_rate trajectories.
     const int64_t upd = print_rate > 0 ? print_rate : std::max<int64_t>(1, (int64_t)(T / 5.0));
+    bool succ = false;
+    int64_t orbit = 0;
+    for (int64_t i = 0; i < T; i++) {
+        if (st8[i] != ST_MISS) succ = (st8[i] == ST_OK);
+        if (!succ) orbit++;
+        if ((i + 1) % upd == 0) orbit = 0;
+    }
+    const double d = (double)(T - orbit);
+    const double tot = sums[0], tot2 = sums[1];
+    omega = tot / d;
+    omega_err = std::sqrt((tot2 / d - (tot / d) * (tot / d)) / d);
+}
+
+// Final statistics of monteTraj::runTraj (src/monteTraj.cpp:412-425) over
+// iters integrals, plus the counters of their sums.
+void final_stats(const double *omega, const double *omega_err, const double *sums, int iters, int64_t T,
+                 cosims_ccs *out)
+{
     double mx = 0;
     for (int k = 0; k < iters; k++) {
-        const int8_t *st8 = status + (int64_t)k * T;
-        bool succ = false;
-        int64_t orbit = 0;
-        for (int64_t i = 0; i < T; i++) {
-            if (st8[i] != ST_MISS) succ = (st8[i] == ST_OK);
-            if (!succ) orbit++;
-            if ((i + 1) % upd == 0) orbit = 0;
-        }
-        const double d = (double)(T - orbit);
-        const double tot = sums[6 * k], tot2 = sums[6 * k + 1];
-        omega[k] = tot / d;
-        omega_err[k] = std::sqrt((tot2 / d - (tot / d) * (tot / d)) / d);
-        mx = std::max(mx, sums[6 * k + 2]);
-        out->total_steps += (int64_t)sums[6 * k + 3];
-        out->n_hit += (int64_t)sums[6 * k + 4];
-        out->n_fail += (int64_t)sums[6 * k + 5];
+        mx = std::max(mx, sums[kSums * k + 2]);
+        out->total_steps += (int64_t)sums[kSums * k + 3];
+        out->n_hit += (int64_t)sums[kSums * k + 4];
+        out->n_fail += (int64_t)sums[kSums * k + 5];
     }
-    // final statistics (src/monteTraj.cpp:412-425)
     double avg = 0, sd = 0, we = 0, wm = 0;
     for (int k = 0; k < iters; k++) avg += omega[k];
     avg = avg / (double)iters;
@@ -1391,6 +1412,14 @@ void finish_ccs(const int8_t *status, const double *sums, int iters, int64_t T, 
     out->werr = 1.0 / std::sqrt(we);
     out->max_ergy = mx;
     out->n_traj = (int64_t)iters * T;
+}
+
+void finish_ccs(const int8_t *status, const double *sums, int iters, int64_t T, int64_t print_rate, double *omega,
+                double *omega_err, cosims_ccs *out)
+{
+    for (int k = 0; k < iters; k++)
+        integral_ccs(status + (int64_t)k * T, sums + kSums * k, T, print_rate, omega[k], omega_err[k]);
+    final_stats(omega, omega_err, sums, iters, T, out);
 }
 
 // Launch-shape search shared by the single-molecule and batch tuners:
@@ -1479,6 +1508,7 @@ struct BatchPlan {
         int iters;
         int64_t T, n, off;  // trajectories per integral, total, first slot
         int ft, units;
+        uint32_t it0;       // Philox iteration of the first integral
     };
     std::vector<Job> jobs;
     std::vector<int> order;              // job indices in work-list order
@@ -1491,8 +1521,26 @@ struct BatchPlan {
     int cus = 0;
     TrajBufs B;
     DBuf tmol, table;
+    int64_t cap = 0, table_cap = 0;      // allocated trajectories / table entries (reused)
     float sample_ms = 0;
 };
+
+// Work-list order: force type by force type, larger molecules first.
+int order_plan(BatchPlan &P)
+{
+    const int nj = (int)P.jobs.size();
+    P.cus = P.jobs[0].mol->dev.cus;
+    for (const auto &J : P.jobs)
+        if (J.mol->dev.id != P.jobs[0].mol->dev.id)
+            return fail(COSIMS_E_INVALID, "batch: molecules on different devices");
+    P.order.resize(nj);
+    for (int k = 0; k < nj; k++) P.order[k] = k;
+    std::stable_sort(P.order.begin(), P.order.end(), [&](int a, int b) {
+        const auto &x = P.jobs[a], &y = P.jobs[b];
+        return x.ft != y.ft ? x.ft < y.ft : x.units > y.units;
+    });
+    return 0;
+}
 
 // Validates the jobs and computes the layout (host only).
 int plan_batch(const cosims_batch_job *jobs, int32_t njobs, int64_t cap_per_job, BatchPlan &P)
@@ -1523,26 +1571,22 @@ int plan_batch(const cosims_batch_job *jobs, int32_t njobs, int64_t cap_per_job,
         if (P.n > INT32_MAX / 4) return fail(COSIMS_E_INVALID, "cosims_run_batch: too many trajectories for one batch");
         P.jobs.push_back(b);
     }
-    P.cus = jobs[0].mol->dev.cus;
-    for (const auto &J : P.jobs)
-        if (J.mol->dev.id != jobs[0].mol->dev.id)
-            return fail(COSIMS_E_INVALID, "cosims_run_batch: molecules on different devices");
-    P.order.resize(njobs);
-    for (int k = 0; k < njobs; k++) P.order[k] = k;
-    std::stable_sort(P.order.begin(), P.order.end(), [&](int a, int b) {
-        const auto &x = P.jobs[a], &y = P.jobs[b];
-        return x.ft != y.ft ? x.ft < y.ft : x.units > y.units;
-    });
-    return 0;
+    return order_plan(P);
 }
 
 // Device buffers, molecule table, and the sampled work list of a plan.
 int stage_batch(BatchPlan &P, hipStream_t s)
 {
     const int nj = (int)P.jobs.size();
-    if (int rc = P.B.alloc(P.n)) return rc;
-    HIP_CHECK(P.tmol.alloc(sizeof(int32_t) * P.n));
-    HIP_CHECK(P.table.alloc(sizeof(MolRec) * nj));
+    if (P.cap < P.n) {
+        if (int rc = P.B.alloc(P.n)) return rc;
+        HIP_CHECK(P.tmol.alloc(sizeof(int32_t) * P.n));
+        P.cap = P.n;
+    }
+    if (P.table_cap < nj) {
+        HIP_CHECK(P.table.alloc(sizeof(MolRec) * nj));
+        P.table_cap = nj;
+    }
     std::vector<MolRec> recs(nj);
     for (int k = 0; k < nj; k++) recs[k] = P.jobs[k].mol->rec;
     HIP_CHECK(hipMemcpyAsync(P.table.p, recs.data(), sizeof(MolRec) * nj, hipMemcpyHostToDevice, s));
@@ -1560,7 +1604,7 @@ int stage_batch(BatchPlan &P, hipStream_t s)
         const unsigned nb = (unsigned)((J.n + 255) / 256);
         hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, s, P.tmol.as<int32_t>() + J.off, J.n, (int32_t)k);
         SampleArgs A = sample_args(J.mol, J.seed, J.job);
-        A.first = 0; A.n = J.n; A.per_iter = J.T;
+        A.first = 0; A.n = J.n; A.per_iter = J.T; A.iteration0 = J.it0;
         A.p0 = P.B.p0.as<double>() + 3 * J.off; A.v0 = P.B.v0.as<double>() + 3 * J.off;
         A.g = P.B.g.as<double>() + J.off; A.status = P.B.status.as<int8_t>() + J.off;
         A.items = P.B.items[0].as<int32_t>(); A.n_items = ctr; A.item_base = (int32_t)J.off;
@@ -1928,3 +1972,5 @@ int cosims_run_batch(cosims_batch_job *jobs, int32_t njobs, const cosims_batch_c
 }
 
 }  // extern "C"
+
+#include "server.hpp"

@@ -1,0 +1,433 @@
+// server.hpp -- CCS request server of the trajectory engine (host runtime).
+// Included by engine.hip after the batch machinery it drives.
+//
+// Requests (atoms, run parameters, a target relative standard error) pass
+//   submit -> setup worker (molecule setup, boundary marches on the GPU,
+//   device upload) -> ready -> admitted (slot and device-memory budget) ->
+//   active -> done.
+// The scheduler thread runs slices: one persistent table-kernel launch per
+// force type over blocks of trajectories of every active request (a block
+// is one CCS integral, Philox iteration = block index).  Requests that
+// arrive while a slice runs join the next slice.  After each slice every
+// request's new blocks are taken in order and the stopping rule is applied
+// after each block, so a request's result depends only on its own blocks:
+// it is the result cosims_run gives over the same blocks with the server's
+// launch shape, whatever else shared the launches.
+#pragma once
+
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <thread>
+
+namespace {
+
+constexpr int64_t kTrajBytes = 150;  // device bytes per in-flight trajectory (TrajBufs + work list)
+
+struct SJob {
+    int64_t ticket = 0;
+    cosims_params params{};
+    std::vector<double> xyz, mass, sigma, eps, charge;
+    bool has_charge = false;
+    double sigma_max = 0;
+    uint32_t job = 0;
+    double target = 0.005;
+    cosims_mol *mol = nullptr;          // owned, made by a setup worker
+    int state = 0;                      // 0 queued, 1 setup, 2 ready, 3 active, 4 done, 5 failed
+    int err = 0;
+    std::string errmsg;
+    int issued = 0;                     // blocks handed to slices
+    int used = 0;                       // blocks taken into the result
+    std::vector<double> omega, omega_err, sums;  // per block
+    bool finished = false;
+    DBuf dsums;                         // per-block sums on the device (max_blocks x kSums)
+    size_t bytes = 0;
+    double t_submit = 0, t_ready = 0, t_admit = 0, t_done = 0, setup_ms = 0;
+    cosims_ccs ccs{};
+    ~SJob()
+    {
+        if (mol) cosims_mol_destroy(mol);
+    }
+};
+
+}  // namespace
+
+struct cosims_server {
+    cosims_server_cfg cfg{};
+    std::chrono::steady_clock::time_point t0;
+    std::mutex mu;
+    std::condition_variable cv;       // scheduler and setup workers wait here
+    std::condition_variable done_cv;  // waiters
+    bool stopping = false;
+    int64_t next_ticket = 1;
+    int setups_running = 0;
+    std::deque<std::shared_ptr<SJob>> queued, ready;
+    std::vector<std::shared_ptr<SJob>> active;
+    std::map<int64_t, std::shared_ptr<SJob>> jobs;
+    size_t bytes_in_use = 0, bytes_peak = 0, budget = 0;
+    cosims_server_stats st{};
+    std::vector<std::thread> workers;
+    std::thread sched;
+    double now_ms() const
+    {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+namespace {
+
+void setup_worker(cosims_server *S)
+{
+    for (;;) {
+        std::shared_ptr<SJob> J;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv.wait(lk, [&] { return S->stopping || !S->queued.empty(); });
+            if (S->queued.empty()) return;
+            J = S->queued.front();
+            S->queued.pop_front();
+            J->state = 1;
+            S->setups_running++;
+        }
+        const double t = S->now_ms();
+        cosims_atoms a{};
+        a.n = (int32_t)J->mass.size();
+        a.xyz = J->xyz.data(); a.mass = J->mass.data(); a.sigma = J->sigma.data(); a.epsilon = J->eps.data();
+        a.charge = J->has_charge ? J->charge.data() : nullptr;
+        a.sigma_max = J->sigma_max;
+        cosims_mol *m = nullptr;
+        int rc = cosims_mol_create(&J->params, &a, &m);
+        if (rc == 0) rc = ensure_device(m);
+        std::string msg = rc ? g_err : std::string();
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->setups_running--;
+            J->setup_ms = S->now_ms() - t;
+            J->t_ready = S->now_ms();
+            if (rc) {
+                if (m) cosims_mol_destroy(m);
+                J->state = 5;
+                J->err = rc;
+                J->errmsg = msg;
+                J->t_done = J->t_ready;
+                S->st.failed++;
+                S->done_cv.notify_all();
+            } else {
+                J->mol = m;
+                J->bytes = m->dbytes + sizeof(double) * kSums * S->cfg.max_blocks +
+                           (size_t)S->cfg.block_traj * kTrajBytes * 2;
+                J->state = 2;
+                S->ready.push_back(J);
+            }
+        }
+        S->cv.notify_all();
+    }
+}
+
+// Blocks of job J in the next slice: enough to reach the target if the
+// standard error keeps falling as 1/sqrt(blocks), at most an equal share of
+// the slice, at least one.
+int blocks_for(const cosims_server *S, const SJob &J, size_t n_active)
+{
+    const auto &c = S->cfg;
+    int want = 1;
+    if (J.used < c.min_blocks) {
+        want = c.min_blocks - J.used;
+    } else if (!J.omega.empty()) {
+        cosims_ccs tmp{};
+        final_stats(J.omega.data(), J.omega_err.data(), J.sums.data(), J.used, c.block_traj, &tmp);
+        const double rel = tmp.werr / tmp.wmean;
+        const double need = J.used * (rel / J.target) * (rel / J.target);
+        want = std::max(1, (int)std::ceil(need) - J.used);
+    }
+    const int64_t share = std::max<int64_t>(1, c.slice_traj / ((int64_t)n_active * c.block_traj));
+    // an active job always has blocks left: used == issued < max_blocks until it finishes
+    return (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)want, share, (int64_t)(c.max_blocks - J.issued)}));
+}
+
+// One slice over the active jobs; returns 0 or an error code (then every
+// active job fails with it).
+int run_slice(cosims_server *S, std::vector<std::shared_ptr<SJob>> &act, BatchPlan &P, hipStream_t s)
+{
+    const auto &c = S->cfg;
+    P.jobs.clear();
+    P.n = 0;
+    std::vector<int> nb(act.size());
+    for (size_t k = 0; k < act.size(); k++) {
+        SJob &J = *act[k];
+        nb[k] = blocks_for(S, J, act.size());
+        if (J.issued + nb[k] > c.max_blocks) return fail(COSIMS_E_INVALID, "server: block count beyond max_blocks");
+        BatchPlan::Job b{};
+        b.mol = J.mol;
+        b.seed = J.params.seed;
+        b.job = J.job;
+        b.iters = nb[k];
+        b.T = c.block_traj;
+        b.n = (int64_t)b.iters * b.T;
+        b.off = P.n;
+        b.ft = J.mol->prep.force_type;
+        b.units = group_units(J.mol->prep);
+        b.it0 = (uint32_t)J.issued;
+        P.n += b.n;
+        P.jobs.push_back(b);
+    }
+    if (P.n > INT32_MAX / 4) return fail(COSIMS_E_INVALID, "server: slice too large");
+    if (int rc = order_plan(P)) return rc;
+    if (int rc = stage_batch(P, s)) return rc;
+    GroupRun R;
+    int launches = 0;
+    if (int rc = launch_batch(P, c.prec, c.lanes, c.wg_size, 0, s, R, launches)) return rc;
+    for (size_t k = 0; k < act.size(); k++) {
+        const auto &J = P.jobs[k];
+        ReduceArgs RA{};
+        RA.status = P.B.status.as<int8_t>() + J.off; RA.chi = P.B.chi.as<double>() + J.off;
+        RA.ergy = P.B.ergy.as<double>() + J.off; RA.totsteps = P.B.tot.as<int32_t>() + J.off;
+        RA.per_iter = J.T; RA.ccs_const = J.mol->prep.ccs_const;
+        RA.out = act[k]->dsums.as<double>() + kSums * act[k]->issued;
+        hipLaunchKernelGGL(k_reduce, dim3(J.iters), dim3(256), 0, s, RA);
+        HIP_CHECK(hipGetLastError());
+    }
+    std::vector<int8_t> status(P.n);
+    std::vector<std::vector<double>> sums(act.size());
+    HIP_CHECK(hipMemcpyAsync(status.data(), P.B.status.p, P.n, hipMemcpyDeviceToHost, s));
+    for (size_t k = 0; k < act.size(); k++) {
+        sums[k].resize((size_t)kSums * nb[k]);
+        HIP_CHECK(hipMemcpyAsync(sums[k].data(), act[k]->dsums.as<double>() + kSums * act[k]->issued,
+                                 sizeof(double) * kSums * nb[k], hipMemcpyDeviceToHost, s));
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    // stopping rule, block by block in block order
+    for (size_t k = 0; k < act.size(); k++) {
+        SJob &J = *act[k];
+        const auto &B = P.jobs[k];
+        for (int i = 0; i < nb[k] && !J.finished; i++) {
+            double om = 0, er = 0;
+            integral_ccs(status.data() + B.off + (int64_t)i * B.T, sums[k].data() + kSums * i, B.T,
+                         J.mol->params.print_rate, om, er);
+            J.omega.push_back(om);
+            J.omega_err.push_back(er);
+            J.sums.insert(J.sums.end(), sums[k].begin() + kSums * i, sums[k].begin() + kSums * (i + 1));
+            J.used++;
+            if (J.used >= c.min_blocks) {
+                cosims_ccs tmp{};
+                final_stats(J.omega.data(), J.omega_err.data(), J.sums.data(), J.used, c.block_traj, &tmp);
+                if (tmp.werr / tmp.wmean <= J.target || J.used >= c.max_blocks) {
+                    J.finished = true;
+                    J.ccs = tmp;
+                    J.ccs.lanes = c.lanes;
+                    J.ccs.wg_size = c.wg_size;
+                }
+            }
+        }
+        J.issued += nb[k];
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->st.slices++;
+    S->st.launches += launches;
+    S->st.traj += P.n;
+    S->st.kernel_ms += R.ms;
+    S->st.pair_evals += R.pairs;
+    S->st.force_evals += R.evals;
+    S->st.total_steps += R.steps;
+    return 0;
+}
+
+void scheduler(cosims_server *S)
+{
+    Stream st;
+    if (hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking) != hipSuccess) st.s = nullptr;
+    BatchPlan P;
+    std::vector<std::shared_ptr<SJob>> act;
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            for (;;) {
+                // admission: FIFO, while a slot and the device-memory budget allow
+                while (!S->ready.empty() && (int)S->active.size() < S->cfg.max_active &&
+                       (S->active.empty() || S->bytes_in_use + S->ready.front()->bytes <= S->budget)) {
+                    auto J = S->ready.front();
+                    S->ready.pop_front();
+                    J->state = 3;
+                    J->t_admit = S->now_ms();
+                    S->bytes_in_use += J->bytes;
+                    S->bytes_peak = std::max(S->bytes_peak, S->bytes_in_use);
+                    S->active.push_back(J);
+                }
+                if (!S->active.empty()) break;
+                if (S->stopping && S->queued.empty() && S->ready.empty() && S->setups_running == 0) return;
+                S->cv.wait(lk);
+            }
+            act = S->active;
+        }
+        int rc = 0;
+        for (auto &J : act)
+            if (!J->dsums.p && J->dsums.alloc(sizeof(double) * kSums * S->cfg.max_blocks) != hipSuccess)
+                rc = fail(COSIMS_E_NOMEM, "server: per-request sums");
+        const double t = S->now_ms();
+        if (!rc) rc = st.s ? run_slice(S, act, P, st.s) : fail(COSIMS_E_HIP, "server: no stream");
+        const std::string msg = rc ? g_err : std::string();
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->st.busy_ms += S->now_ms() - t;
+            std::vector<std::shared_ptr<SJob>> keep;
+            for (auto &J : S->active) {
+                if (rc || J->finished) {
+                    J->state = rc ? 5 : 4;
+                    J->err = rc;
+                    J->errmsg = msg;
+                    J->t_done = S->now_ms();
+                    J->dsums.reset();
+                    if (J->mol) {
+                        cosims_mol_destroy(J->mol);
+                        J->mol = nullptr;
+                    }
+                    S->bytes_in_use -= J->bytes;
+                    if (rc) S->st.failed++;
+                    else S->st.done++;
+                } else {
+                    keep.push_back(J);
+                }
+            }
+            S->active.swap(keep);
+        }
+        S->done_cv.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cosims_server_create(const cosims_server_cfg *cfg, cosims_server **out)
+{
+    if (!out) return fail(COSIMS_E_INVALID, "cosims_server_create: null output");
+    cosims_server_cfg c{};
+    if (cfg) c = *cfg;
+    if (c.prec != COSIMS_PREC_FP64 && c.prec != COSIMS_PREC_MIXED) return fail(COSIMS_E_INVALID, "unknown precision");
+    if (c.lanes == 0) c.lanes = 16;
+    if (c.wg_size == 0) c.wg_size = 512;
+    if (c.lanes != 4 && c.lanes != 8 && c.lanes != 16 && c.lanes != 32 && c.lanes != 64)
+        return fail(COSIMS_E_INVALID, "server lanes must be 4, 8, 16, 32 or 64");
+    if (c.wg_size % 64 || c.wg_size > 1024 || c.wg_size % c.lanes)
+        return fail(COSIMS_E_INVALID, "server wg_size must be a multiple of 64, at most 1024");
+    if (c.max_active <= 0) c.max_active = 64;
+    if (c.block_traj <= 0) c.block_traj = 10000;
+    if (c.slice_traj <= 0) c.slice_traj = 262144;
+    if (c.min_blocks <= 0) c.min_blocks = 3;
+    if (c.max_blocks <= 0) c.max_blocks = 1000;
+    if (c.min_blocks > c.max_blocks) return fail(COSIMS_E_INVALID, "min_blocks > max_blocks");
+    if (c.setup_threads <= 0) c.setup_threads = 2;
+    Device d;
+    if (int rc = device_init(d)) return rc;
+    std::unique_ptr<cosims_server> S(new cosims_server());
+    S->cfg = c;
+    S->t0 = std::chrono::steady_clock::now();
+    if (c.hbm_budget > 0) {
+        S->budget = (size_t)c.hbm_budget;
+    } else {
+        size_t fr = 0, tot = 0;
+        HIP_CHECK(hipMemGetInfo(&fr, &tot));
+        S->budget = fr / 5 * 4;
+    }
+    cosims_server *raw = S.get();
+    for (int k = 0; k < c.setup_threads; k++) S->workers.emplace_back(setup_worker, raw);
+    S->sched = std::thread(scheduler, raw);
+    *out = S.release();
+    return 0;
+}
+
+int cosims_server_submit(cosims_server *S, const cosims_request *r, int64_t *ticket)
+{
+    if (!S || !r || !ticket) return fail(COSIMS_E_INVALID, "cosims_server_submit: null argument");
+    const cosims_atoms &a = r->atoms;
+    if (a.n <= 0 || !a.xyz || !a.mass || !a.sigma || !a.epsilon)
+        return fail(COSIMS_E_INVALID, "cosims_server_submit: empty molecule");
+    auto J = std::make_shared<SJob>();
+    J->params = r->params;
+    J->xyz.assign(a.xyz, a.xyz + 3 * (size_t)a.n);
+    J->mass.assign(a.mass, a.mass + a.n);
+    J->sigma.assign(a.sigma, a.sigma + a.n);
+    J->eps.assign(a.epsilon, a.epsilon + a.n);
+    J->has_charge = a.charge != nullptr;
+    if (a.charge) J->charge.assign(a.charge, a.charge + a.n);
+    J->sigma_max = a.sigma_max;
+    J->job = r->job;
+    J->target = r->target_rel_err > 0 ? r->target_rel_err : 0.005;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->stopping) return fail(COSIMS_E_INVALID, "cosims_server_submit: server is shutting down");
+        J->ticket = S->next_ticket++;
+        J->t_submit = S->now_ms();
+        S->jobs[J->ticket] = J;
+        S->queued.push_back(J);
+        S->st.submitted++;
+        *ticket = J->ticket;
+    }
+    S->cv.notify_all();
+    return 0;
+}
+
+int cosims_server_wait(cosims_server *S, int64_t ticket, double timeout_ms, cosims_result *out, double *omega,
+                       double *omega_err, int32_t cap)
+{
+    if (!S || !out) return fail(COSIMS_E_INVALID, "cosims_server_wait: null argument");
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(ticket);
+    if (it == S->jobs.end()) return fail(COSIMS_E_INVALID, "cosims_server_wait: unknown ticket");
+    std::shared_ptr<SJob> J = it->second;
+    auto doneq = [&] { return J->state >= 4; };
+    if (timeout_ms < 0) {
+        S->done_cv.wait(lk, doneq);
+    } else if (!S->done_cv.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), doneq)) {
+        std::memset(out, 0, sizeof(*out));
+        out->status = 1;
+        return fail(COSIMS_E_PENDING, "request not finished");
+    }
+    std::memset(out, 0, sizeof(*out));
+    out->status = J->state == 4 ? 0 : J->err;
+    out->blocks = J->used;
+    out->ccs = J->ccs;
+    out->rel_err = J->used > 0 && J->ccs.wmean != 0 ? J->ccs.werr / J->ccs.wmean : 0;
+    out->t_submit_ms = J->t_submit;
+    out->t_ready_ms = J->t_ready;
+    out->t_admit_ms = J->t_admit;
+    out->t_done_ms = J->t_done;
+    out->setup_ms = J->setup_ms;
+    out->block_traj = S->cfg.block_traj;
+    for (int k = 0; k < std::min<int>(cap, J->used); k++) {
+        if (omega) omega[k] = J->omega[k];
+        if (omega_err) omega_err[k] = J->omega_err[k];
+    }
+    if (J->state == 5) return fail(J->err, J->errmsg);
+    return 0;
+}
+
+int cosims_server_stats_get(cosims_server *S, cosims_server_stats *out)
+{
+    if (!S || !out) return fail(COSIMS_E_INVALID, "cosims_server_stats_get: null argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    *out = S->st;
+    out->queued = (int64_t)(S->queued.size() + S->ready.size()) + S->setups_running;
+    out->active = (int64_t)S->active.size();
+    out->bytes_in_use = (int64_t)S->bytes_in_use;
+    out->bytes_peak = (int64_t)S->bytes_peak;
+    out->budget = (int64_t)S->budget;
+    out->now_ms = S->now_ms();
+    return 0;
+}
+
+void cosims_server_destroy(cosims_server *S)
+{
+    if (!S) return;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stopping = true;
+    }
+    S->cv.notify_all();
+    for (auto &w : S->workers) w.join();
+    S->cv.notify_all();
+    S->sched.join();
+    delete S;
+}
+
+}  // extern "C"

@@ -27,7 +27,10 @@ EXPORTS = [
     "cosims_sample_reference",
     "cosims_verlet", "cosims_run", "cosims_run_batch", "cosims_tune_batch", "cosims_strerror",
     "cosims_last_error", "cosims_device_info", "cosims_device_sync",
+    "cosims_server_create", "cosims_server_submit", "cosims_server_wait", "cosims_server_stats_get",
+    "cosims_server_destroy",
 ]
+E_PENDING = -8
 
 
 class CosimsError(RuntimeError):
@@ -107,6 +110,29 @@ class BatchStats(C.Structure):
                                           "pair_evals")]
 
 
+class ServerCfg(C.Structure):
+    _fields_ = [("prec", C.c_int32), ("lanes", C.c_int32), ("wg_size", C.c_int32), ("hbm_budget", C.c_int64),
+                ("max_active", C.c_int32), ("min_blocks", C.c_int32), ("max_blocks", C.c_int32),
+                ("setup_threads", C.c_int32), ("block_traj", C.c_int64), ("slice_traj", C.c_int64)]
+
+
+class Request(C.Structure):
+    _fields_ = [("params", Params), ("atoms", Atoms), ("job", C.c_uint32), ("target_rel_err", C.c_double)]
+
+
+class Result(C.Structure):
+    _fields_ = [("status", C.c_int32), ("blocks", C.c_int32), ("ccs", Ccs), ("rel_err", C.c_double),
+                ("t_submit_ms", C.c_double), ("t_ready_ms", C.c_double), ("t_admit_ms", C.c_double),
+                ("t_done_ms", C.c_double), ("setup_ms", C.c_double), ("block_traj", C.c_int64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(k, C.c_int64) for k in ("submitted", "done", "failed", "queued", "active", "slices", "launches",
+                                          "traj", "pair_evals", "force_evals", "total_steps")] + \
+               [(k, C.c_double) for k in ("kernel_ms", "busy_ms", "now_ms")] + \
+               [(k, C.c_int64) for k in ("bytes_in_use", "bytes_peak", "budget")]
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -134,6 +160,12 @@ def lib():
         L.cosims_verlet.argtypes = [vp, C.c_int32, C.c_int64, vp, vp, vp, vp, vp, vp, vp]
         L.cosims_run.argtypes = [vp, C.POINTER(Params), C.POINTER(RunCfg), vp, vp, C.POINTER(Ccs)]
         L.cosims_run_batch.argtypes = [C.POINTER(BatchJob), C.c_int32, C.POINTER(BatchCfg), C.POINTER(BatchStats)]
+        L.cosims_server_create.argtypes = [C.POINTER(ServerCfg), C.POINTER(vp)]
+        L.cosims_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_int64)]
+        L.cosims_server_wait.argtypes = [vp, C.c_int64, C.c_double, C.POINTER(Result), vp, vp, C.c_int32]
+        L.cosims_server_stats_get.argtypes = [vp, C.POINTER(ServerStats)]
+        L.cosims_server_destroy.argtypes = [vp]
+        L.cosims_server_destroy.restype = None
         L.cosims_tune_batch.argtypes = [C.POINTER(BatchJob), C.c_int32, C.c_int32, C.POINTER(C.c_int32),
                                         C.POINTER(C.c_int32)]
         L.cosims_strerror.restype = C.c_char_p
@@ -391,3 +423,65 @@ def run_batch(jobs, prec=PREC_MIXED, lanes=0, wg_size=0, blocks_per_cu=0):
         d.update(omega=om, omega_err=err)
         out.append(d)
     return out, {k: getattr(st, k) for k, _ in BatchStats._fields_}
+
+
+class Server:
+    """CCS request server (cosims_server_*): continuous batching of requests
+    over persistent launches, admission by slots and device-memory budget."""
+
+    def __init__(self, prec=PREC_MIXED, lanes=16, wg_size=512, hbm_budget=0, max_active=64, min_blocks=3,
+                 max_blocks=1000, setup_threads=2, block_traj=10000, slice_traj=262144):
+        self.cfg = ServerCfg(prec=prec, lanes=lanes, wg_size=wg_size, hbm_budget=hbm_budget, max_active=max_active,
+                             min_blocks=min_blocks, max_blocks=max_blocks, setup_threads=setup_threads,
+                             block_traj=block_traj, slice_traj=slice_traj)
+        h = C.c_void_p()
+        _check(lib().cosims_server_create(C.byref(self.cfg), C.byref(h)))
+        self._h = h
+        self.max_blocks = max_blocks
+
+    def submit(self, mol: dict, params: Params, job=0, target=0.005) -> int:
+        """mol: a read_molfile() dict (xyz, mass, sigma, epsilon, charge, charged, sigma_max)."""
+        xyz = _f64(mol["xyz"]); mass = _f64(mol["mass"]); sig = _f64(mol["sigma"]); eps = _f64(mol["epsilon"])
+        q = _f64(mol["charge"]) if mol.get("charged") else None
+        a = Atoms(n=mass.size, xyz=xyz.ctypes.data, mass=mass.ctypes.data, sigma=sig.ctypes.data,
+                  epsilon=eps.ctypes.data, charge=None if q is None else q.ctypes.data,
+                  sigma_max=mol["sigma_max"])
+        p = Params()
+        C.pointer(p)[0] = params
+        p.charge = int(bool(mol.get("charged")))
+        r = Request(params=p, atoms=a, job=job, target_rel_err=target)
+        t = C.c_int64()
+        _check(lib().cosims_server_submit(self._h, C.byref(r), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1.0):
+        res = Result()
+        om = np.empty(self.max_blocks); err = np.empty(self.max_blocks)
+        _check(lib().cosims_server_wait(self._h, ticket, timeout_ms, C.byref(res), _p(om), _p(err), self.max_blocks))
+        out = {k: getattr(res, k) for k, _ in Result._fields_ if k != "ccs"}
+        out.update({k: getattr(res.ccs, k) for k in ("mean", "sd", "wmean", "werr", "max_ergy", "n_traj", "n_hit",
+                                                     "n_fail", "total_steps")})
+        out.update(omega=om[: res.blocks].copy(), omega_err=err[: res.blocks].copy())
+        return out
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().cosims_server_stats_get(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cosims_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -30,6 +30,7 @@ extern "C" {
 #define COSIMS_E_IO (-5)          /* file could not be read */
 #define COSIMS_E_UNSUPPORTED (-6) /* input the reference handles only by undefined behaviour */
 #define COSIMS_E_NODEVICE (-7)    /* no usable gfx950 device */
+#define COSIMS_E_PENDING (-8)     /* request not finished within the wait */
 
 #define COSIMS_RNG_PHILOX 0    /* counter-based Philox4x32-10 stream (default) */
 #define COSIMS_RNG_REFERENCE 1 /* the reference's own single-thread stream */
@@ -257,6 +258,68 @@ int cosims_run_batch(cosims_batch_job *jobs, int32_t njobs, const cosims_batch_c
 /* The launch shape cosims_run_batch would tune for these jobs. */
 int cosims_tune_batch(const cosims_batch_job *jobs, int32_t njobs, int32_t prec, int32_t *lanes,
                       int32_t *wg_size);
+
+/* ------------------------------------------------------------ serving
+ * A CCS request server; the reference runs one molecule per process
+ * (src/main.cpp:119-140) and has no counterpart.  A request (atoms, run
+ * parameters, target relative standard error of the weighted-mean CCS) is set
+ * up on a host worker (boundary marches on the GPU when params.setup_gpu),
+ * admitted while an active slot and the device-memory budget allow, and run
+ * in blocks: one CCS integral of block_traj trajectories each, Philox
+ * (params.seed, job) with iteration = block index.  Each slice is one
+ * persistent launch per force type over blocks of every active request;
+ * requests arriving meanwhile join the next slice.  A request retires at the
+ * first block, from min_blocks on, where werr / wmean <= target (or at
+ * max_blocks); its result is that of cosims_run with iter = blocks,
+ * traj = block_traj and the server's launch shape, whatever shared the
+ * launches.  Per-block sums stay on the device until read. */
+typedef struct cosims_server cosims_server;
+
+typedef struct cosims_server_cfg {
+    int32_t prec;             /* COSIMS_PREC_* */
+    int32_t lanes, wg_size;   /* launch shape, fixed for the server; 0 => 16, 512 */
+    int64_t hbm_budget;       /* device bytes for admitted requests; 0 => 80 % of free memory */
+    int32_t max_active;       /* admitted requests at once; 0 => 64 */
+    int32_t min_blocks;       /* blocks before the stopping rule applies; 0 => 3 */
+    int32_t max_blocks;       /* blocks at most per request; 0 => 1000 */
+    int32_t setup_threads;    /* host setup workers; 0 => 2 */
+    int64_t block_traj;       /* trajectories per block; 0 => 10000 */
+    int64_t slice_traj;       /* trajectories per slice to aim for; 0 => 262144 */
+} cosims_server_cfg;
+
+typedef struct cosims_request {
+    cosims_params params;     /* as for cosims_mol_create; seed keys the request's stream */
+    cosims_atoms atoms;       /* copied at submit */
+    uint32_t job;             /* Philox key extension */
+    double target_rel_err;    /* 0 => 0.005 */
+} cosims_request;
+
+typedef struct cosims_result {
+    int32_t status;           /* 0 done, 1 pending, < 0 error code */
+    int32_t blocks;           /* blocks in the result */
+    cosims_ccs ccs;           /* statistics over those blocks */
+    double rel_err;           /* werr / wmean */
+    double t_submit_ms, t_ready_ms, t_admit_ms, t_done_ms;  /* server clock */
+    double setup_ms;
+    int64_t block_traj;
+} cosims_result;
+
+typedef struct cosims_server_stats {
+    int64_t submitted, done, failed, queued, active;
+    int64_t slices, launches, traj, pair_evals, force_evals, total_steps;
+    double kernel_ms, busy_ms, now_ms;
+    int64_t bytes_in_use, bytes_peak, budget;
+} cosims_server_stats;
+
+int cosims_server_create(const cosims_server_cfg *cfg, cosims_server **out);
+int cosims_server_submit(cosims_server *s, const cosims_request *r, int64_t *ticket);
+/* Waits up to timeout_ms (< 0: no limit); COSIMS_E_PENDING if not done.
+ * omega/omega_err (may be NULL) receive up to cap per-block values. */
+int cosims_server_wait(cosims_server *s, int64_t ticket, double timeout_ms, cosims_result *out, double *omega,
+                       double *omega_err, int32_t cap);
+int cosims_server_stats_get(cosims_server *s, cosims_server_stats *out);
+/* Finishes every submitted request, then frees the server. */
+void cosims_server_destroy(cosims_server *s);
 
 const char *cosims_strerror(int code);
 const char *cosims_last_error(void);

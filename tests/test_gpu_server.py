@@ -1,0 +1,129 @@
+"""CCS request server (MI355X): continuous batching over persistent launches.
+
+A request's result must be exactly cosims_run over the same blocks (iter =
+blocks, traj = block_traj, the server's launch shape), and the same whether
+it ran alone or shared launches with other requests, whatever the admission
+order (bit-identical per-block CCS).  Tolerance: none (equality).
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+from cosims_amd import engine as E
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+SHAPE = dict(lanes=8, wg_size=256)
+BLOCK = 2000
+
+
+def _mol(name, atoms=None):
+    return E.read_molfile(os.path.join(GOLDEN, name), os.path.join(GOLDEN, atoms) if atoms else None)
+
+
+def _params(seed, **kw):
+    p = E.default_params(temp=298.0, seed=seed, setup_gpu=1, **kw)
+    return p
+
+
+REQUESTS = [  # (molecule file, params, job, target relative error)
+    ("ion128.mfj", dict(multipole=0, dispersion_cutoff=0), 1, 0.02),
+    ("acau.mfj", dict(), 2, 0.03),                                    # charged, clustered type 1
+    ("ion2000.mfj", dict(multipole=0, dispersion_cutoff=0), 3, 0.03),
+    ("acau.pdb", dict(multipole=0, dispersion_cutoff=0), 4, 0.05),    # neutral, exact type 6
+]
+
+
+def _server(**kw):
+    cfg = dict(prec=E.PREC_MIXED, block_traj=BLOCK, min_blocks=3, max_blocks=40, slice_traj=32768, **SHAPE)
+    cfg.update(kw)
+    return E.Server(**cfg)
+
+
+def _submit_all(srv, reqs):
+    return [srv.submit(_mol(f), _params(100 + j, **p), job=j, target=t) for f, p, j, t in reqs]
+
+
+@pytest.fixture(scope="module")
+def mixed_results():
+    with _server() as srv:
+        tickets = _submit_all(srv, REQUESTS)
+        res = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    return res, st
+
+
+def test_server_requests_finish_at_target(mixed_results):
+    res, st = mixed_results
+    assert st["done"] == len(REQUESTS) and st["failed"] == 0 and st["active"] == 0
+    for (f, p, j, t), r in zip(REQUESTS, res):
+        assert r["status"] == 0
+        assert 3 <= r["blocks"] <= 40
+        assert r["rel_err"] <= t or r["blocks"] == 40, (f, r["rel_err"])
+        assert r["t_submit_ms"] <= r["t_ready_ms"] <= r["t_admit_ms"] <= r["t_done_ms"]
+        assert r["n_traj"] == r["blocks"] * BLOCK
+    assert st["slices"] >= 1 and st["traj"] >= sum(r["n_traj"] for r in res)
+
+
+def test_server_result_equals_run(mixed_results):
+    """The blocks are CCS integrals 0..k-1 of the request's Philox stream."""
+    res, _ = mixed_results
+    for (f, p, j, t), r in zip(REQUESTS, res):
+        mol = _mol(f)
+        m = E.Molecule.from_file(os.path.join(GOLDEN, f), params=_params(100 + j, **p))
+        run = m.run(E.PREC_MIXED, job=j, iters=r["blocks"], traj=BLOCK, **SHAPE)
+        assert np.array_equal(r["omega"], run["omega"]), f
+        assert r["wmean"] == run["wmean"] and r["werr"] == run["werr"]
+        assert mol["charged"] == bool(m.info()["charged"])
+
+
+def test_server_alone_equals_mix(mixed_results):
+    res, _ = mixed_results
+    for k, req in enumerate(REQUESTS):
+        with _server() as srv:
+            (t,) = _submit_all(srv, [req])
+            alone = srv.wait(t)
+        assert alone["blocks"] == res[k]["blocks"], req[0]
+        assert np.array_equal(alone["omega"], res[k]["omega"]), req[0]
+
+
+def test_server_budget_serialises_admission(mixed_results):
+    """A one-byte device-memory budget admits one request at a time; the
+    results do not change."""
+    res, _ = mixed_results
+    with _server(hbm_budget=1) as srv:
+        tickets = _submit_all(srv, REQUESTS)
+        out = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    assert st["done"] == len(REQUESTS)
+    spans = sorted((r["t_admit_ms"], r["t_done_ms"]) for r in out)
+    for (a0, d0), (a1, d1) in zip(spans, spans[1:]):
+        assert a1 >= d0  # never two requests active at once
+    for r, ref in zip(out, res):
+        assert np.array_equal(r["omega"], ref["omega"])
+
+
+def test_server_continuous_admission():
+    """Requests submitted while others run are admitted at the next slice
+    (their admission precedes the earlier requests' completion)."""
+    with _server(min_blocks=100, max_blocks=100, slice_traj=16384) as srv:
+        first = srv.submit(_mol("ion2000.mfj"), _params(7, multipole=0, dispersion_cutoff=0), job=1, target=1e-9)
+        while srv.stats()["active"] == 0:
+            time.sleep(0.005)
+        late = srv.submit(_mol("ion128.mfj"), _params(8, multipole=0, dispersion_cutoff=0), job=2, target=1e-9)
+        a, b = srv.wait(first), srv.wait(late)
+    assert a["blocks"] == b["blocks"] == 100
+    assert b["t_admit_ms"] < a["t_done_ms"]
+
+
+def test_server_rejects_bad_requests():
+    with _server() as srv:
+        bad = dict(xyz=np.zeros((0, 3)), mass=[], sigma=[], epsilon=[], charged=0, sigma_max=3.5)
+        with pytest.raises(E.CosimsError, match="invalid argument"):
+            srv.submit(bad, _params(1))
+        with pytest.raises(E.CosimsError, match="unknown ticket"):
+            srv.wait(12345)
