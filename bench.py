@@ -68,6 +68,169 @@ def cpu_baseline(m, budget_s, threads):
                 sample=f"{done} trajectories of the 2,000-atom ion (FP64 oracle, OpenMP, {t_used:.1f} s)")
 
 
+def pct(a, q):
+    return float(np.percentile(np.asarray(a, dtype=float), q)) if len(a) else None
+
+
+def serving(E, synth, n_requests, rate, target, seed=5000, policy=0):
+    """Poisson request stream through the CCS server; latency = completion -
+    submission on the server's clock (setup, queueing and trajectories)."""
+    import tempfile
+    reqs = synth.serving_stream(count=n_requests, rate=rate, seed=seed)
+    d = tempfile.mkdtemp(prefix="cosims_serve_")
+    mols = []
+    for k, r in enumerate(reqs):
+        path = os.path.join(d, f"req{k}.mfj")
+        synth.write_mfj(path, f"req{k}", r["ion"])
+        mols.append(E.read_molfile(path))
+
+    def params(k):
+        return E.default_params(temp=reqs[k]["temp"], seed=seed + k, multipole=0, dispersion_cutoff=0, setup_gpu=1)
+    # launch shape: lanes per trajectory by molecule size (the size rule), fixed for every request
+    W = 512
+    log(f"serving: {n_requests} requests at {rate}/s, target {target}, lanes by size, wg {W}")
+    srv = E.Server(prec=E.PREC_MIXED, lanes=0, wg_size=W, block_traj=10000, min_blocks=3, max_blocks=1000,
+                   policy=policy)
+    t0 = time.perf_counter()
+    tickets = []
+    for k, r in enumerate(reqs):
+        delay = t0 + r["t"] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        tickets.append(srv.submit(mols[k], params(k), job=k, target=target))
+    res = []
+    for k, t in enumerate(tickets):
+        r = srv.wait(t)
+        res.append(r)
+        log(f"  request {k}: N={reqs[k]['n']} T={reqs[k]['temp']:.0f} K  {r['blocks']} blocks  "
+            f"CCS {r['wmean']:.1f} +- {100 * r['rel_err']:.2f}%  latency {r['t_done_ms'] - r['t_submit_ms']:.0f} ms")
+    st = srv.stats()
+    srv.close()
+    lat = [r["t_done_ms"] - r["t_submit_ms"] for r in res]
+    makespan = (max(r["t_done_ms"] for r in res) - min(r["t_submit_ms"] for r in res)) / 1e3
+    traj = sum(r["n_traj"] for r in res)
+    return {
+        "workload": f"Poisson {rate} req/s, {n_requests} requests, N log-uniform 100-20,000, "
+                    f"q round(N/800)+1 e, T 200-400 K, to {100 * target:.1f}% relative SE",
+        "requests": n_requests, "offered_rate": rate, "served_rate": n_requests / makespan,
+        "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "mean": float(np.mean(lat)), "max": max(lat)},
+        "setup_ms_p50": pct([r["setup_ms"] for r in res], 50),
+        "queue_ms_p50": pct([r["t_admit_ms"] - r["t_ready_ms"] for r in res], 50),
+        "trajectories_per_s": traj / makespan, "trajectories": traj, "makespan_s": makespan,
+        "rel_err_max": max(r["rel_err"] for r in res), "blocks_mean": float(np.mean([r["blocks"] for r in res])),
+        "slices": st["slices"], "kernel_s": st["kernel_ms"] / 1e3, "busy_s": st["busy_ms"] / 1e3,
+        "lanes_per_trajectory": "by size class (4 x 2^class)", "wg_size": W,
+        "policy": "shortest remaining work first" if policy == E.SERVE_SHORTEST_FIRST else "equal share",
+    }
+
+
+def batch_workload(E, synth, count, traj, steps):
+    """256 synthetic conformers (N log-uniform 50-5,000), 50,000 trajectories
+    each, all in shared launches (one per launch shape); per-size-class
+    launch shapes from the batch tuner, tuned before the timing."""
+    import tempfile
+    d = tempfile.mkdtemp(prefix="cosims_batch_")
+    mols = []
+    t0 = time.perf_counter()
+    for k, (n, ion) in enumerate(synth.uneven_conformers(count=count)):
+        path = os.path.join(d, f"c{k}.mfj")
+        synth.write_mfj(path, f"c{k}", ion)
+        p = E.default_params(temp=298.0, seed=7000 + k, multipole=0, dispersion_cutoff=0, setup_gpu=1,
+                             traj=traj, iter=1)
+        mols.append(E.Molecule.from_file(path, params=p))
+    log(f"batch: {count} conformers set up in {time.perf_counter() - t0:.1f} s")
+    jobs = [(m, dict(job=k, iters=1, traj=traj)) for k, m in enumerate(mols)]
+    CL, CW = E.tune_batch(jobs, E.PREC_MIXED)
+    log(f"batch: tuned lanes {CL} wg {CW}")
+    n_atoms = [m.info()["n"] for m in mols]
+    out = {}
+    for label, kw in (("tuned", dict(lanes=-1, class_lanes=CL, class_wg=CW)), ("size_rule", dict(lanes=0))):
+        times = []
+        for k in range(steps):
+            E.device_sync()
+            t1 = time.perf_counter()
+            res, st = E.run_batch(jobs, prec=E.PREC_MIXED, **kw)
+            E.device_sync()
+            times.append(time.perf_counter() - t1)
+            log(f"batch {label} {k}: {times[-1]:.2f} s, kernel {st['kernel_ms']:.0f} ms, {st['launches']} launches")
+        flops = st["pair_evals"] * FLOP_PAIR + st["force_evals"] * FLOP_EPI + st["total_steps"] * FLOP_VERLET
+        out[label] = {"trajectories_per_s": count * traj / float(np.median(times)), "s_per_step": float(np.median(times)),
+                      "kernel_s": st["kernel_ms"] / 1e3, "launches": st["launches"],
+                      "achieved_tflops": flops / (st["kernel_ms"] * 1e-3) / 1e12,
+                      "class_lanes": st["class_lanes"], "class_wg": st["class_wg"]}
+    out["workload"] = (f"{count} synthetic conformers, N log-uniform 50-5,000 (total {sum(n_atoms)} atoms), "
+                       f"q round(N/250)+1 e, He 298 K, {traj} trajectories each, mixed precision")
+    out["trajectories"] = count * traj
+    for m in mols:
+        m.close()
+    return out
+
+
+def clustered_workload(E, synth, traj, oracle_sample):
+    """The 20,000-atom ion (+25 e): clustered (force type 1, PCM clusters +
+    multipoles) against exact (type 4), mixed precision, and the exact FP64
+    GPU run on the same initial conditions; an FP64 oracle subsample pins the
+    FP64 run per trajectory."""
+    import tempfile
+    ion = synth.config_ion("ion20000")
+    d = tempfile.mkdtemp(prefix="cosims_c4_")
+    path = os.path.join(d, "ion20000.mfj")
+    synth.write_mfj(path, "ion20000", ion)
+    out = {"workload": f"20,000-atom synthetic ion, +25 e, He 298 K, {traj} trajectories, mixed precision"}
+    runs = {}
+    for label, mp in (("clustered", 1), ("exact", 0)):
+        p = E.default_params(temp=298.0, seed=4242, multipole=mp, dispersion_cutoff=mp, setup_gpu=1,
+                             traj=traj // 10, iter=10)
+        t0 = time.perf_counter()
+        m = E.Molecule.from_file(path, params=p)
+        setup = time.perf_counter() - t0
+        inf = m.info()
+        m.run(E.PREC_MIXED, job=99, iters=1, traj=4096)  # tunes outside the timing
+        E.device_sync()
+        t1 = time.perf_counter()
+        r = m.run(E.PREC_MIXED, job=0, keep=True)
+        dt = time.perf_counter() - t1
+        log(f"c4 {label}: type {inf['force_type']} ncl {inf['ncl']} setup {setup:.2f} s, run {dt:.2f} s, "
+            f"CCS {r['wmean']:.2f} +- {r['werr']:.2f}, lanes {r['lanes']}")
+        runs[label] = (m, r)
+        out[label] = {"force_type": inf["force_type"], "clusters": inf["ncl"], "setup_s": setup,
+                      "trajectories_per_s": traj / dt, "ccs": r["wmean"], "ccs_err": r["werr"],
+                      "lanes": r["lanes"], "wg_size": r["wg_size"], "kernel_s": r["traj_kernel_ms"][0] / 1e3}
+    m4, r4 = runs["exact"]
+    t1 = time.perf_counter()
+    f = m4.run(E.PREC_FP64, job=0, keep=True, lanes=64, wg_size=512)
+    dt = time.perf_counter() - t1
+    log(f"c4 exact FP64: {dt:.2f} s, CCS {f['wmean']:.2f} +- {f['werr']:.2f}")
+    out["exact_fp64"] = {"trajectories_per_s": traj / dt, "ccs": f["wmean"], "ccs_err": f["werr"],
+                         "lanes": f["lanes"]}
+    for label in ("clustered", "exact"):
+        r = runs[label][1]
+        dd = r["omega"] - f["omega"]
+        out[label]["rel_dev_vs_fp64"] = (r["wmean"] - f["wmean"]) / f["wmean"]
+        out[label]["paired_rel_dev_vs_fp64"] = float(dd.mean() / f["omega"].mean())
+        out[label]["paired_rel_se"] = float(dd.std(ddof=1) / np.sqrt(dd.size) / f["omega"].mean())
+    if oracle_sample > 0:
+        import oracle
+        A = m4.atoms()
+        inf = m4.info()
+        mo = oracle.Mol(A["xyz"], A["q"], A["sig6"], A["eps24m"], A["eps4"], inf["charged"], inf["mu"],
+                        inf["force_type"])
+        s = m4.sample(m4.params.seed, 0, 0, 0, oracle_sample)
+        h = s["hit"] == 1
+        t1 = time.perf_counter()
+        o = mo.verlet(m4.params.dt, s["p0"][h], s["v0"][h], (inf["a"], inf["b"], inf["c"]), nthreads=16)
+        dt = time.perf_counter() - t1
+        g = f["chi"][:oracle_sample][h]
+        ok = (o["ok"] == 1) & (f["status"][:oracle_sample][h] == 1)
+        dchi = np.abs(g[ok] - o["chi"][ok])
+        out["exact_fp64"]["oracle_sample"] = {"trajectories": int(h.sum()), "median_abs_dchi": float(np.median(dchi)),
+                                              "frac_dchi_le_1e-8": float(np.mean(dchi <= 1e-8)), "oracle_s": dt}
+        log(f"c4 oracle sample: {int(h.sum())} trajectories, median |dchi| {np.median(dchi):.2e}")
+    for m, _ in runs.values():
+        m.close()
+    return out
+
+
 def copy_probe(nbytes=256 << 20, reps=3):
     """A plain copy kernel of known size for calibrating the memory counters."""
     import ctypes as C
@@ -93,11 +256,32 @@ def main():
     ap.add_argument("--iters", type=int, default=100, help="CCS integrals per step")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-requests", type=int, default=40, help="requests of the serving stream (0: none)")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-target", type=float, default=0.005)
+    ap.add_argument("--serve-compare", type=int, default=1,
+                    help="also run the stream with equal-share slices (1) or not (0)")
+    ap.add_argument("--workload", choices=("headline", "batch", "clustered"), default="headline",
+                    help="headline: the 2,000-atom ion + serving stream (the JSON contract line); batch: 256 uneven "
+                         "conformers in shared launches; clustered: the 20,000-atom ion, clustered vs exact")
+    ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--batch-traj", type=int, default=50000)
+    ap.add_argument("--c4-traj", type=int, default=500000)
+    ap.add_argument("--oracle-sample", type=int, default=300)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 3 device-to-device copies of 256 MiB (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     a = ap.parse_args()
     if a.copy_probe:
         copy_probe()
+        return
+    if a.workload != "headline":
+        from cosims_amd import engine as E
+        from cosims_amd import synth
+        if a.workload == "batch":
+            res = batch_workload(E, synth, a.batch_count, a.batch_traj, max(1, a.steps))
+        else:
+            res = clustered_workload(E, synth, a.c4_traj, a.oracle_sample)
+        print(json.dumps({"workload": a.workload, **res}), flush=True)
         return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); trajectories shard "
@@ -183,6 +367,14 @@ def main():
     if a.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(m, a.cpu_seconds, threads)
     out["served_latency_ms"] = None
+    if a.serve_requests > 0:
+        sv = serving(E, synth, a.serve_requests, a.serve_rate, a.serve_target)
+        out["serving"] = sv
+        out["served_latency_ms"] = {"p50": sv["latency_ms"]["p50"], "p99": sv["latency_ms"]["p99"]}
+        if a.serve_compare:
+            eq = serving(E, synth, a.serve_requests, a.serve_rate, a.serve_target, policy=E.SERVE_EQUAL_SHARE)
+            out["serving_equal_share"] = {k: eq[k] for k in ("latency_ms", "served_rate", "trajectories_per_s",
+                                                             "makespan_s", "slices")}
     print(json.dumps(out), flush=True)
 
 

@@ -1071,6 +1071,11 @@ __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v)
     if (i < n) p[i] = v;
 }
 
+__global__ void k_copy_i32(const int32_t *src, int32_t *dst)
+{
+    if (threadIdx.x == 0) *dst = *src;
+}
+
 // Runs the attempt rounds over the hit list in B.items[0] (count in
 // counters[0]).  Returns per-round stats.
 struct RoundStats {
@@ -1497,9 +1502,29 @@ int autotune(const cosims_mol *M, int prec, int &L_out, int &wg_out)
 namespace {
 
 // ------------------------------------------------------------- batches ---
+constexpr int kMaxSegs = 64;      // launches of one batch (force type x launch shape)
+constexpr int kSegStreams = 4;    // streams the launches of a batch are spread over
+
+// Lanes per trajectory from the units one force evaluation sweeps (atoms,
+// or clusters): as many as leave each lane at least 48 units per step, from
+// 4 to 64.  Size class k = the molecules this gives 4 * 2^k lanes.
+int lanes_rule(int units)
+{
+    int L = 4;
+    while (L < 64 && units >= 48 * L) L *= 2;
+    return L;
+}
+int size_class(int units)
+{
+    int k = 0;
+    for (int L = lanes_rule(units); L > 4; L >>= 1) k++;
+    return k;
+}
+
 // A batch laid out on the device: per-trajectory buffers of all jobs back
 // to back, the molecule of each trajectory, the MolRec table, and the work
-// list ordered force type by force type, larger molecules first.
+// list ordered by launch (force type, lanes, workgroup size), larger
+// molecules first within a launch.
 struct BatchPlan {
     struct Job {
         const cosims_mol *mol;
@@ -1509,23 +1534,33 @@ struct BatchPlan {
         int64_t T, n, off;  // trajectories per integral, total, first slot
         int ft, units;
         uint32_t it0;       // Philox iteration of the first integral
+        int L, W;           // launch shape of this job's trajectories
     };
     std::vector<Job> jobs;
     std::vector<int> order;              // job indices in work-list order
     struct Seg {
-        int ft;
-        int64_t begin, end;              // work-list range of this force type
+        int ft, L, W;
+        int64_t begin, end;              // work-list range of this launch
+        double work;                     // trajectories x units, for the launch order
     };
     std::vector<Seg> segs;
     int64_t n = 0;
     int cus = 0;
     TrajBufs B;
-    DBuf tmol, table;
+    DBuf tmol, table, seg_ctr, seg_evals, seg_end;
     int64_t cap = 0, table_cap = 0;      // allocated trajectories / table entries (reused)
     float sample_ms = 0;
+    std::vector<hipStream_t> streams;    // default priority, made on first use
+    BatchPlan() = default;
+    BatchPlan(const BatchPlan &) = delete;
+    BatchPlan &operator=(const BatchPlan &) = delete;
+    ~BatchPlan()
+    {
+        for (hipStream_t x : streams) (void)hipStreamDestroy(x);
+    }
 };
 
-// Work-list order: force type by force type, larger molecules first.
+// Work-list order: by launch shape, larger molecules first.
 int order_plan(BatchPlan &P)
 {
     const int nj = (int)P.jobs.size();
@@ -1537,13 +1572,33 @@ int order_plan(BatchPlan &P)
     for (int k = 0; k < nj; k++) P.order[k] = k;
     std::stable_sort(P.order.begin(), P.order.end(), [&](int a, int b) {
         const auto &x = P.jobs[a], &y = P.jobs[b];
-        return x.ft != y.ft ? x.ft < y.ft : x.units > y.units;
+        if (x.ft != y.ft) return x.ft < y.ft;
+        if (x.L != y.L) return x.L < y.L;
+        if (x.W != y.W) return x.W < y.W;
+        return x.units > y.units;
     });
     return 0;
 }
 
+// Launch shape of one job: a fixed lane count, the size rule, or the tuned
+// table of its size class.
+struct ShapeTable {
+    int lanes = 0;                                   // > 0: every job
+    int wg = 512;
+    int class_lanes[COSIMS_SIZE_CLASSES] = {0};      // used when lanes == 0 and non-zero
+    int class_wg[COSIMS_SIZE_CLASSES] = {0};
+    void apply(BatchPlan::Job &b) const
+    {
+        const int c = size_class(b.units);
+        b.L = lanes > 0 ? lanes : (class_lanes[c] ? class_lanes[c] : lanes_rule(b.units));
+        b.W = (lanes <= 0 && class_wg[c]) ? class_wg[c] : wg;
+        if (b.W % b.L) b.W = std::max(b.W, b.L);
+    }
+};
+
 // Validates the jobs and computes the layout (host only).
-int plan_batch(const cosims_batch_job *jobs, int32_t njobs, int64_t cap_per_job, BatchPlan &P)
+int plan_batch(const cosims_batch_job *jobs, int32_t njobs, int64_t cap_per_job, const ShapeTable &shape,
+               BatchPlan &P)
 {
     if (!jobs || njobs <= 0) return fail(COSIMS_E_INVALID, "cosims_run_batch: no jobs");
     P.jobs.clear();
@@ -1567,6 +1622,7 @@ int plan_batch(const cosims_batch_job *jobs, int32_t njobs, int64_t cap_per_job,
         b.off = P.n;
         b.ft = J.mol->prep.force_type;
         b.units = group_units(J.mol->prep);
+        shape.apply(b);
         P.n += b.n;
         if (P.n > INT32_MAX / 4) return fail(COSIMS_E_INVALID, "cosims_run_batch: too many trajectories for one batch");
         P.jobs.push_back(b);
@@ -1587,6 +1643,7 @@ int stage_batch(BatchPlan &P, hipStream_t s)
         HIP_CHECK(P.table.alloc(sizeof(MolRec) * nj));
         P.table_cap = nj;
     }
+    if (!P.seg_end.p) HIP_CHECK(P.seg_end.alloc(sizeof(int32_t) * kMaxSegs));
     std::vector<MolRec> recs(nj);
     for (int k = 0; k < nj; k++) recs[k] = P.jobs[k].mol->rec;
     HIP_CHECK(hipMemcpyAsync(P.table.p, recs.data(), sizeof(MolRec) * nj, hipMemcpyHostToDevice, s));
@@ -1597,7 +1654,8 @@ int stage_batch(BatchPlan &P, hipStream_t s)
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, s));
     P.segs.clear();
-    int64_t seg_begin = 0;
+    std::vector<double> work;
+    double seg_work = 0;
     for (size_t i = 0; i < P.order.size(); i++) {
         const int k = P.order[i];
         const BatchPlan::Job &J = P.jobs[k];
@@ -1610,67 +1668,151 @@ int stage_batch(BatchPlan &P, hipStream_t s)
         A.items = P.B.items[0].as<int32_t>(); A.n_items = ctr; A.item_base = (int32_t)J.off;
         hipLaunchKernelGGL(k_sample, dim3(nb), dim3(256), 0, s, A);
         HIP_CHECK(hipGetLastError());
-        const bool last_of_type = i + 1 == P.order.size() || P.jobs[P.order[i + 1]].ft != J.ft;
-        if (last_of_type) {
-            int32_t end = 0;
-            HIP_CHECK(hipMemcpyAsync(&end, ctr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-            HIP_CHECK(hipStreamSynchronize(s));
-            P.segs.push_back({J.ft, seg_begin, (int64_t)end});
-            seg_begin = end;
+        seg_work += (double)J.n * J.units;
+        const bool last = i + 1 == P.order.size();
+        const BatchPlan::Job *nx = last ? nullptr : &P.jobs[P.order[i + 1]];
+        if (last || nx->ft != J.ft || nx->L != J.L || nx->W != J.W) {
+            // the launch's work list ends where the item counter stands now
+            if ((int)P.segs.size() == kMaxSegs) return fail(COSIMS_E_INVALID, "batch: too many distinct launch shapes");
+            hipLaunchKernelGGL(k_copy_i32, dim3(1), dim3(1), 0, s, ctr, P.seg_end.as<int32_t>() + P.segs.size());
+            P.segs.push_back({J.ft, J.L, J.W, 0, 0, seg_work});
+            seg_work = 0;
         }
     }
+    HIP_CHECK(hipGetLastError());
+    std::vector<int32_t> ends(P.segs.size());
+    HIP_CHECK(hipMemcpyAsync(ends.data(), P.seg_end.p, sizeof(int32_t) * ends.size(), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipEventRecord(e1, s));
     HIP_CHECK(hipEventSynchronize(e1));
     HIP_CHECK(hipEventElapsedTime(&P.sample_ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    for (size_t g = 0; g < P.segs.size(); g++) {
+        P.segs[g].begin = g ? ends[g - 1] : 0;
+        P.segs[g].end = ends[g];
+    }
     return 0;
 }
 
-// One table-kernel launch per force type over the staged work list.
-int launch_batch(BatchPlan &P, int prec, int L, int wg, int bpc, hipStream_t s, GroupRun &tot, int &launches)
+// The launches of a staged plan, spread over kSegStreams streams so that
+// one launch's tail overlaps the next; the biggest launch goes first.  tot.ms
+// is the time from the first launch to the last one's end.
+int launch_batch(BatchPlan &P, int prec, int bpc, hipStream_t s, GroupRun &tot, int &launches)
 {
     launches = 0;
-    for (const auto &g : P.segs) {
+    const int nseg = (int)P.segs.size();
+    if (!P.seg_ctr.p) {
+        HIP_CHECK(P.seg_ctr.alloc(sizeof(int32_t) * kMaxSegs));
+        HIP_CHECK(P.seg_evals.alloc(sizeof(unsigned long long) * 4 * kMaxSegs));
+    }
+    while ((int)P.streams.size() < std::min(nseg, kSegStreams)) {
+        hipStream_t x;
+        HIP_CHECK(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+        P.streams.push_back(x);
+    }
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, P.seg_ctr.as<int32_t>(), kMaxSegs, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(8), dim3(64), 0, s, P.seg_evals.as<int32_t>(), 8 * kMaxSegs, 0);
+    std::vector<int> order(nseg);
+    for (int i = 0; i < nseg; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return P.segs[a].work > P.segs[b].work; });
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    std::vector<hipEvent_t> joins;
+    HIP_CHECK(hipEventRecord(e0, s));
+    int rc = 0;
+    for (int q = 0; q < nseg && !rc; q++) {
+        const int i = order[q];
+        const auto &g = P.segs[i];
         const int cnt = (int)(g.end - g.begin);
         if (cnt <= 0) continue;
-        GroupRun R;
-        if (int rc = run_multi(P.table.as<MolRec>(), P.tmol.as<int32_t>(), g.ft, prec, P.cus,
-                               P.B.items[0].as<int32_t>() + g.begin, cnt, P.B, s, L, wg, bpc, R))
-            return rc;
-        tot.ms += R.ms;
-        tot.steps += R.steps;
-        tot.evals += R.evals;
-        tot.pots += R.pots;
-        tot.pairs += R.pairs;
+        MultiFn fn = multi_kernel(prec, g.ft, g.L);
+        if (!fn) {
+            rc = fail(COSIMS_E_INVALID, "no table kernel for this force type / lane count");
+            break;
+        }
+        hipStream_t x = P.streams[launches % P.streams.size()];
+        HIP_CHECK(hipStreamWaitEvent(x, e0, 0));
+        MultiArgs G{};
+        G.io = traj_io(P.B, P.B.items[0].as<int32_t>() + g.begin, cnt);
+        G.io.counter = P.seg_ctr.as<int32_t>() + i;
+        G.io.evals = P.seg_evals.as<unsigned long long>() + 4 * i;
+        G.mols = P.table.as<MolRec>();
+        G.tmol = P.tmol.as<int32_t>();
+        const int64_t grid = group_grid((const void *)fn, g.W, 0, bpc, P.cus, cnt, g.L);
+        hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(g.W), 0, x, G);
+        HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_chi, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, x, G.io.items, cnt, G.io.status,
+                           G.io.cchi, G.io.chi);
+        HIP_CHECK(hipGetLastError());
+        hipEvent_t j;
+        HIP_CHECK(hipEventCreateWithFlags(&j, hipEventDisableTiming));
+        joins.push_back(j);
+        HIP_CHECK(hipEventRecord(j, x));
+        HIP_CHECK(hipStreamWaitEvent(s, j, 0));
         launches++;
+    }
+    HIP_CHECK(hipEventRecord(e1, s));
+    std::vector<unsigned long long> ev(4 * kMaxSegs);
+    HIP_CHECK(hipMemcpyAsync(ev.data(), P.seg_evals.p, sizeof(unsigned long long) * ev.size(), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    for (hipEvent_t j : joins) (void)hipEventDestroy(j);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    tot.ms += ms;
+    for (int i = 0; i < nseg; i++) {
+        tot.steps += (int64_t)ev[4 * i];
+        tot.evals += (int64_t)ev[4 * i + 1];
+        tot.pots += (int64_t)ev[4 * i + 2];
+        tot.pairs += (int64_t)ev[4 * i + 3];
     }
     return 0;
 }
 
-int tune_batch(const cosims_batch_job *jobs, int32_t njobs, int prec, int &L_out, int &W_out)
+// Size-class autotuner of a batch: for every size class present, the
+// lanes x workgroup search of tune_shapes on that class's jobs alone.
+int tune_batch(const cosims_batch_job *jobs, int32_t njobs, int prec, ShapeTable &out)
 {
+    if (!jobs || njobs <= 0) return fail(COSIMS_E_INVALID, "cosims_tune_batch: no jobs");
     Stream st;
     HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
-    BatchPlan sets[2];
-    const int64_t caps[2] = {128, 512};
-    int units = 0;  // largest molecule: the lane counts worth trying
-    for (int k = 0; k < 2; k++) {
-        if (int rc = plan_batch(jobs, njobs, caps[k], sets[k])) return rc;
-        for (auto &J : sets[k].jobs) {
-            J.job = 0xfffffff0u - (uint32_t)k;  // a stream of its own
-            units = std::max(units, J.units);
-        }
-        if (int rc = stage_batch(sets[k], st.s)) return rc;
+    std::vector<cosims_batch_job> by_class[COSIMS_SIZE_CLASSES];
+    for (int k = 0; k < njobs; k++) {
+        if (!jobs[k].mol) return fail(COSIMS_E_INVALID, "cosims_tune_batch: job without molecule");
+        by_class[size_class(group_units(jobs[k].mol->prep))].push_back(jobs[k]);
     }
-    auto run = [&](int set, int L, int W, double &ms) -> int {
-        GroupRun R;
-        int launches = 0;
-        if (int rc = launch_batch(sets[set], prec, L, W, 0, st.s, R, launches)) return rc;
-        ms = R.ms;
-        return 0;
-    };
-    return tune_shapes(units, run, L_out, W_out);
+    out.lanes = 0;
+    for (int c = 0; c < COSIMS_SIZE_CLASSES; c++) {
+        if (by_class[c].empty()) continue;
+        int units = 0;  // largest molecule of the class: the lane counts worth trying
+        for (const auto &j : by_class[c]) units = std::max(units, group_units(j.mol->prep));
+        // tuning subsets: at most 64 jobs of the class, 128 / 512 trajectories each
+        std::vector<cosims_batch_job> sub(by_class[c].begin(),
+                                          by_class[c].begin() + std::min<size_t>(64, by_class[c].size()));
+        const int64_t caps[2] = {128, 512};
+        BatchPlan set;
+        auto run = [&](int k, int L, int W, double &ms) -> int {
+            ShapeTable fixed;
+            fixed.lanes = L;
+            fixed.wg = W;
+            if (int rc = plan_batch(sub.data(), (int32_t)sub.size(), caps[k], fixed, set)) return rc;
+            for (auto &J : set.jobs) J.job = 0xfffffff0u - (uint32_t)k;  // a stream of its own
+            if (int rc = stage_batch(set, st.s)) return rc;
+            GroupRun R;
+            int launches = 0;
+            if (int rc = launch_batch(set, prec, 0, st.s, R, launches)) return rc;
+            ms = R.ms;
+            return 0;
+        };
+        int L = 0, W = 0;
+        if (int rc = tune_shapes(units, run, L, W)) return rc;
+        out.class_lanes[c] = L;
+        out.class_wg[c] = W;
+    }
+    return 0;
 }
 
 }  // namespace
@@ -1884,14 +2026,17 @@ int cosims_run(const cosims_mol *M, const cosims_params *P, const cosims_run_cfg
     return 0;
 }
 
-int cosims_tune_batch(const cosims_batch_job *jobs, int32_t njobs, int32_t prec, int32_t *lanes, int32_t *wg_size)
+int cosims_tune_batch(const cosims_batch_job *jobs, int32_t njobs, int32_t prec, int32_t *class_lanes,
+                      int32_t *class_wg)
 {
-    if (!lanes || !wg_size) return fail(COSIMS_E_INVALID, "cosims_tune_batch: null output");
+    if (!class_lanes || !class_wg) return fail(COSIMS_E_INVALID, "cosims_tune_batch: null output");
     if (prec != COSIMS_PREC_FP64 && prec != COSIMS_PREC_MIXED) return fail(COSIMS_E_INVALID, "unknown precision");
-    int L = 0, W = 0;
-    if (int rc = tune_batch(jobs, njobs, prec, L, W)) return rc;
-    *lanes = L;
-    *wg_size = W;
+    ShapeTable T;
+    if (int rc = tune_batch(jobs, njobs, prec, T)) return rc;
+    for (int c = 0; c < COSIMS_SIZE_CLASSES; c++) {
+        class_lanes[c] = T.class_lanes[c];
+        class_wg[c] = T.class_wg[c];
+    }
     return 0;
 }
 
@@ -1903,32 +2048,41 @@ int cosims_run_batch(cosims_batch_job *jobs, int32_t njobs, const cosims_batch_c
     for (int k = 0; jobs && k < njobs; k++)
         if (!jobs[k].omega || !jobs[k].omega_err || !jobs[k].out)
             return fail(COSIMS_E_INVALID, "cosims_run_batch: job without outputs");
-    int L = cfg->lanes, wg = cfg->wg_size;
-    if (L != 0 && L != 4 && L != 8 && L != 16 && L != 32 && L != 64)
-        return fail(COSIMS_E_INVALID, "lanes must be 0 (tuned), 4, 8, 16, 32 or 64");
+    auto lanes_ok = [](int L) { return L == 4 || L == 8 || L == 16 || L == 32 || L == 64; };
+    auto wg_ok = [](int W) { return W > 0 && W % 64 == 0 && W <= 1024; };
+    ShapeTable shape;
+    shape.lanes = cfg->lanes > 0 ? cfg->lanes : 0;
+    shape.wg = cfg->wg_size > 0 ? cfg->wg_size : 512;
+    if ((cfg->lanes > 0 && !lanes_ok(cfg->lanes)) || cfg->lanes < -1)
+        return fail(COSIMS_E_INVALID, "lanes must be -1 (tuned per size class), 0 (size rule), 4, 8, 16, 32 or 64");
+    if (!wg_ok(shape.wg)) return fail(COSIMS_E_INVALID, "wg_size must be a multiple of 64, at most 1024");
+    for (int c = 0; c < COSIMS_SIZE_CLASSES; c++) {
+        if (cfg->class_lanes[c] && !lanes_ok(cfg->class_lanes[c])) return fail(COSIMS_E_INVALID, "bad class_lanes");
+        if (cfg->class_wg[c] && !wg_ok(cfg->class_wg[c])) return fail(COSIMS_E_INVALID, "bad class_wg");
+        shape.class_lanes[c] = cfg->class_lanes[c];
+        shape.class_wg[c] = cfg->class_wg[c];
+    }
     std::memset(stats, 0, sizeof(*stats));
     auto t0 = std::chrono::steady_clock::now();
-    BatchPlan P;
-    if (int rc = plan_batch(jobs, njobs, 0, P)) return rc;
-    if (L == 0) {
-        int tl = 0, tw = 0;
-        if (int rc = tune_batch(jobs, njobs, prec, tl, tw)) return rc;
-        L = tl;
-        if (wg <= 0) wg = tw;
+    if (cfg->lanes == -1) {
+        bool given = false;
+        for (int c = 0; c < COSIMS_SIZE_CLASSES; c++) given |= cfg->class_lanes[c] != 0;
+        if (!given)
+            if (int rc = tune_batch(jobs, njobs, prec, shape)) return rc;
     }
-    if (wg <= 0) wg = 512;
-    if (wg % 64 || wg > 1024) return fail(COSIMS_E_INVALID, "wg_size must be a multiple of 64, at most 1024");
+    BatchPlan P;
+    if (int rc = plan_batch(jobs, njobs, 0, shape, P)) return rc;
     Stream st;
     HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
     if (int rc = stage_batch(P, st.s)) return rc;
     GroupRun R;
     int launches = 0;
-    if (int rc = launch_batch(P, prec, L, wg, cfg->blocks_per_cu, st.s, R, launches)) return rc;
+    if (int rc = launch_batch(P, prec, cfg->blocks_per_cu, st.s, R, launches)) return rc;
     // per-integral sums of every job (k_reduce per job), then the host statistics
     int64_t total_iters = 0;
     for (const auto &J : P.jobs) total_iters += J.iters;
     DBuf red;
-    HIP_CHECK(red.alloc(sizeof(double) * 6 * total_iters));
+    HIP_CHECK(red.alloc(sizeof(double) * kSums * total_iters));
     int64_t it0 = 0;
     std::vector<int64_t> it_base(P.jobs.size());
     for (size_t k = 0; k < P.jobs.size(); k++) {
@@ -1937,12 +2091,12 @@ int cosims_run_batch(cosims_batch_job *jobs, int32_t njobs, const cosims_batch_c
         ReduceArgs RA{};
         RA.status = P.B.status.as<int8_t>() + J.off; RA.chi = P.B.chi.as<double>() + J.off;
         RA.ergy = P.B.ergy.as<double>() + J.off; RA.totsteps = P.B.tot.as<int32_t>() + J.off;
-        RA.per_iter = J.T; RA.ccs_const = J.mol->prep.ccs_const; RA.out = red.as<double>() + 6 * it0;
+        RA.per_iter = J.T; RA.ccs_const = J.mol->prep.ccs_const; RA.out = red.as<double>() + kSums * it0;
         hipLaunchKernelGGL(k_reduce, dim3(J.iters), dim3(256), 0, st.s, RA);
         HIP_CHECK(hipGetLastError());
         it0 += J.iters;
     }
-    std::vector<double> sums(6 * total_iters);
+    std::vector<double> sums(kSums * total_iters);
     std::vector<int8_t> status(P.n);
     HIP_CHECK(hipMemcpyAsync(sums.data(), red.p, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, st.s));
     HIP_CHECK(hipMemcpyAsync(status.data(), P.B.status.p, P.n, hipMemcpyDeviceToHost, st.s));
@@ -1951,15 +2105,19 @@ int cosims_run_batch(cosims_batch_job *jobs, int32_t njobs, const cosims_batch_c
         const auto &J = P.jobs[k];
         cosims_ccs *o = jobs[k].out;
         std::memset(o, 0, sizeof(*o));
-        finish_ccs(status.data() + J.off, sums.data() + 6 * it_base[k], J.iters, J.T, J.mol->params.print_rate,
+        finish_ccs(status.data() + J.off, sums.data() + kSums * it_base[k], J.iters, J.T, J.mol->params.print_rate,
                    jobs[k].omega, jobs[k].omega_err, o);
-        o->lanes = L;
-        o->wg_size = wg;
+        o->lanes = J.L;
+        o->wg_size = J.W;
         stats->n_hit += o->n_hit;
         stats->total_steps += o->total_steps;
     }
-    stats->lanes = L;
-    stats->wg_size = wg;
+    for (int c = 0; c < COSIMS_SIZE_CLASSES; c++) {
+        stats->class_lanes[c] = shape.class_lanes[c];
+        stats->class_wg[c] = shape.class_wg[c];
+    }
+    stats->lanes = shape.lanes;
+    stats->wg_size = shape.wg;
     stats->launches = launches;
     stats->kernel_ms = R.ms;
     stats->sample_ms = P.sample_ms;

@@ -65,6 +65,7 @@ struct cosims_server {
     std::vector<std::shared_ptr<SJob>> active;
     std::map<int64_t, std::shared_ptr<SJob>> jobs;
     size_t bytes_in_use = 0, bytes_peak = 0, budget = 0;
+    double units_per_ms = 1.0e9;      // force-loop work per kernel ms (learned from the slices)
     cosims_server_stats st{};
     std::vector<std::thread> workers;
     std::thread sched;
@@ -124,10 +125,9 @@ void setup_worker(cosims_server *S)
     }
 }
 
-// Blocks of job J in the next slice: enough to reach the target if the
-// standard error keeps falling as 1/sqrt(blocks), at most an equal share of
-// the slice, at least one.
-int blocks_for(const cosims_server *S, const SJob &J, size_t n_active)
+// Blocks job J still needs: min_blocks first, then as many as reach the
+// target if the standard error keeps falling as 1/sqrt(blocks); at least one.
+int blocks_wanted(const cosims_server *S, const SJob &J)
 {
     const auto &c = S->cfg;
     int want = 1;
@@ -140,9 +140,82 @@ int blocks_for(const cosims_server *S, const SJob &J, size_t n_active)
         const double need = J.used * (rel / J.target) * (rel / J.target);
         want = std::max(1, (int)std::ceil(need) - J.used);
     }
-    const int64_t share = std::max<int64_t>(1, c.slice_traj / ((int64_t)n_active * c.block_traj));
     // an active job always has blocks left: used == issued < max_blocks until it finishes
-    return (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)want, share, (int64_t)(c.max_blocks - J.issued)}));
+    return std::max(1, std::min(want, c.max_blocks - J.issued));
+}
+
+// Work of one block of J in force-loop units (integration steps x atoms, or
+// x clusters for the clustered types): the mean of its finished blocks, or
+// before any, straight chords through the ellipsoid at the mean speed for
+// 60 % of the trajectories.
+double block_cost(const cosims_server *S, const SJob &J)
+{
+    const Prepared &p = J.mol->prep;
+    const double units = (double)std::max(1, group_units(p));
+    if (J.used > 0) {
+        double steps = 0;
+        for (int k = 0; k < J.used; k++) steps += J.sums[kSums * k + 3];
+        return std::max(1.0, steps / J.used) * units;
+    }
+    const double chord = 2 * p.bmax / (p.vd.mean * J.mol->params.dt);
+    return 0.6 * (double)S->cfg.block_traj * chord * units;
+}
+
+struct SliceEntry {
+    size_t k;   // index into the active list
+    int nb;     // blocks of this slice
+};
+
+// Slice composition.  Shortest remaining work first: requests in order of
+// blocks wanted x block cost take blocks until the slice's work budget
+// (slice_ms at the measured throughput) is spent; the first always gets at
+// least one block.  Equal share: every request gets the same number of
+// blocks, up to slice_traj trajectories in all.
+std::vector<SliceEntry> plan_slice(cosims_server *S, const std::vector<std::shared_ptr<SJob>> &act)
+{
+    const auto &c = S->cfg;
+    std::vector<SliceEntry> out;
+    std::vector<int> want(act.size());
+    std::vector<double> cost(act.size());
+    for (size_t k = 0; k < act.size(); k++) {
+        want[k] = blocks_wanted(S, *act[k]);
+        cost[k] = block_cost(S, *act[k]);
+    }
+    int64_t traj_left = std::max<int64_t>(c.slice_traj, c.block_traj);
+    if (c.policy == COSIMS_SERVE_EQUAL_SHARE) {
+        const int64_t share = std::max<int64_t>(1, c.slice_traj / ((int64_t)act.size() * c.block_traj));
+        for (size_t k = 0; k < act.size(); k++) out.push_back({k, (int)std::min<int64_t>(want[k], share)});
+        return out;
+    }
+    std::vector<size_t> order(act.size());
+    for (size_t k = 0; k < order.size(); k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return want[a] * cost[a] < want[b] * cost[b]; });
+    double budget = c.slice_ms * S->units_per_ms;
+    std::vector<int> nb(act.size(), 0);
+    for (size_t k : order) {
+        int64_t b = std::min<int64_t>(want[k], (int64_t)(budget / cost[k]));
+        b = std::min<int64_t>(b, traj_left / c.block_traj);
+        if (b <= 0 && traj_left == std::max<int64_t>(c.slice_traj, c.block_traj)) b = 1;  // the first always runs
+        if (b <= 0) continue;
+        nb[k] = (int)b;
+        budget -= b * cost[k];
+        traj_left -= b * c.block_traj;
+    }
+    // fill: a slice with too few trajectories leaves the launch tail-bound;
+    // top up, still in order and within what each request wants
+    int64_t fill = c.min_slice_traj;
+    for (size_t k = 0; k < act.size(); k++) fill -= (int64_t)nb[k] * c.block_traj;
+    for (size_t k : order) {
+        if (fill <= 0) break;
+        const int64_t add = std::min<int64_t>(want[k] - nb[k], (fill + c.block_traj - 1) / c.block_traj);
+        if (add <= 0) continue;
+        nb[k] += (int)add;
+        fill -= add * c.block_traj;
+    }
+    for (size_t k : order)
+        if (nb[k] > 0) out.push_back({k, nb[k]});
+    return out;
 }
 
 // One slice over the active jobs; returns 0 or an error code (then every
@@ -150,24 +223,25 @@ int blocks_for(const cosims_server *S, const SJob &J, size_t n_active)
 int run_slice(cosims_server *S, std::vector<std::shared_ptr<SJob>> &act, BatchPlan &P, hipStream_t s)
 {
     const auto &c = S->cfg;
+    const std::vector<SliceEntry> plan = plan_slice(S, act);
     P.jobs.clear();
     P.n = 0;
-    std::vector<int> nb(act.size());
-    for (size_t k = 0; k < act.size(); k++) {
-        SJob &J = *act[k];
-        nb[k] = blocks_for(S, J, act.size());
-        if (J.issued + nb[k] > c.max_blocks) return fail(COSIMS_E_INVALID, "server: block count beyond max_blocks");
+    for (const SliceEntry &e : plan) {
+        SJob &J = *act[e.k];
+        if (J.issued + e.nb > c.max_blocks) return fail(COSIMS_E_INVALID, "server: block count beyond max_blocks");
         BatchPlan::Job b{};
         b.mol = J.mol;
         b.seed = J.params.seed;
         b.job = J.job;
-        b.iters = nb[k];
+        b.iters = e.nb;
         b.T = c.block_traj;
         b.n = (int64_t)b.iters * b.T;
         b.off = P.n;
         b.ft = J.mol->prep.force_type;
         b.units = group_units(J.mol->prep);
         b.it0 = (uint32_t)J.issued;
+        b.L = c.lanes > 0 ? c.lanes : lanes_rule(b.units);
+        b.W = std::max(c.wg_size, b.L);
         P.n += b.n;
         P.jobs.push_back(b);
     }
@@ -176,37 +250,41 @@ int run_slice(cosims_server *S, std::vector<std::shared_ptr<SJob>> &act, BatchPl
     if (int rc = stage_batch(P, s)) return rc;
     GroupRun R;
     int launches = 0;
-    if (int rc = launch_batch(P, c.prec, c.lanes, c.wg_size, 0, s, R, launches)) return rc;
-    for (size_t k = 0; k < act.size(); k++) {
-        const auto &J = P.jobs[k];
+    if (int rc = launch_batch(P, c.prec, 0, s, R, launches)) return rc;
+    for (size_t i = 0; i < plan.size(); i++) {
+        const auto &J = P.jobs[i];
+        SJob &Q = *act[plan[i].k];
         ReduceArgs RA{};
         RA.status = P.B.status.as<int8_t>() + J.off; RA.chi = P.B.chi.as<double>() + J.off;
         RA.ergy = P.B.ergy.as<double>() + J.off; RA.totsteps = P.B.tot.as<int32_t>() + J.off;
         RA.per_iter = J.T; RA.ccs_const = J.mol->prep.ccs_const;
-        RA.out = act[k]->dsums.as<double>() + kSums * act[k]->issued;
+        RA.out = Q.dsums.as<double>() + kSums * Q.issued;
         hipLaunchKernelGGL(k_reduce, dim3(J.iters), dim3(256), 0, s, RA);
         HIP_CHECK(hipGetLastError());
     }
     std::vector<int8_t> status(P.n);
-    std::vector<std::vector<double>> sums(act.size());
+    std::vector<std::vector<double>> sums(plan.size());
     HIP_CHECK(hipMemcpyAsync(status.data(), P.B.status.p, P.n, hipMemcpyDeviceToHost, s));
-    for (size_t k = 0; k < act.size(); k++) {
-        sums[k].resize((size_t)kSums * nb[k]);
-        HIP_CHECK(hipMemcpyAsync(sums[k].data(), act[k]->dsums.as<double>() + kSums * act[k]->issued,
-                                 sizeof(double) * kSums * nb[k], hipMemcpyDeviceToHost, s));
+    for (size_t i = 0; i < plan.size(); i++) {
+        SJob &Q = *act[plan[i].k];
+        sums[i].resize((size_t)kSums * plan[i].nb);
+        HIP_CHECK(hipMemcpyAsync(sums[i].data(), Q.dsums.as<double>() + kSums * Q.issued,
+                                 sizeof(double) * kSums * plan[i].nb, hipMemcpyDeviceToHost, s));
     }
     HIP_CHECK(hipStreamSynchronize(s));
     // stopping rule, block by block in block order
-    for (size_t k = 0; k < act.size(); k++) {
-        SJob &J = *act[k];
-        const auto &B = P.jobs[k];
-        for (int i = 0; i < nb[k] && !J.finished; i++) {
+    double units = 0;
+    for (size_t i = 0; i < plan.size(); i++) {
+        SJob &J = *act[plan[i].k];
+        const auto &B = P.jobs[i];
+        for (int b = 0; b < plan[i].nb; b++) units += sums[i][kSums * b + 3] * B.units;
+        for (int b = 0; b < plan[i].nb && !J.finished; b++) {
             double om = 0, er = 0;
-            integral_ccs(status.data() + B.off + (int64_t)i * B.T, sums[k].data() + kSums * i, B.T,
+            integral_ccs(status.data() + B.off + (int64_t)b * B.T, sums[i].data() + kSums * b, B.T,
                          J.mol->params.print_rate, om, er);
             J.omega.push_back(om);
             J.omega_err.push_back(er);
-            J.sums.insert(J.sums.end(), sums[k].begin() + kSums * i, sums[k].begin() + kSums * (i + 1));
+            J.sums.insert(J.sums.end(), sums[i].begin() + kSums * b, sums[i].begin() + kSums * (b + 1));
             J.used++;
             if (J.used >= c.min_blocks) {
                 cosims_ccs tmp{};
@@ -214,14 +292,15 @@ int run_slice(cosims_server *S, std::vector<std::shared_ptr<SJob>> &act, BatchPl
                 if (tmp.werr / tmp.wmean <= J.target || J.used >= c.max_blocks) {
                     J.finished = true;
                     J.ccs = tmp;
-                    J.ccs.lanes = c.lanes;
-                    J.ccs.wg_size = c.wg_size;
+                    J.ccs.lanes = B.L;
+                    J.ccs.wg_size = B.W;
                 }
             }
         }
-        J.issued += nb[k];
+        J.issued += plan[i].nb;
     }
     std::lock_guard<std::mutex> lk(S->mu);
+    if (R.ms > 0 && units > 0) S->units_per_ms = 0.5 * S->units_per_ms + 0.5 * units / R.ms;
     S->st.slices++;
     S->st.launches += launches;
     S->st.traj += P.n;
@@ -304,15 +383,18 @@ int cosims_server_create(const cosims_server_cfg *cfg, cosims_server **out)
     cosims_server_cfg c{};
     if (cfg) c = *cfg;
     if (c.prec != COSIMS_PREC_FP64 && c.prec != COSIMS_PREC_MIXED) return fail(COSIMS_E_INVALID, "unknown precision");
-    if (c.lanes == 0) c.lanes = 16;
     if (c.wg_size == 0) c.wg_size = 512;
-    if (c.lanes != 4 && c.lanes != 8 && c.lanes != 16 && c.lanes != 32 && c.lanes != 64)
-        return fail(COSIMS_E_INVALID, "server lanes must be 4, 8, 16, 32 or 64");
-    if (c.wg_size % 64 || c.wg_size > 1024 || c.wg_size % c.lanes)
+    if (c.lanes != 0 && c.lanes != 4 && c.lanes != 8 && c.lanes != 16 && c.lanes != 32 && c.lanes != 64)
+        return fail(COSIMS_E_INVALID, "server lanes must be 0 (size rule), 4, 8, 16, 32 or 64");
+    if (c.wg_size % 64 || c.wg_size > 1024)
         return fail(COSIMS_E_INVALID, "server wg_size must be a multiple of 64, at most 1024");
+    if (c.min_slice_traj <= 0) c.min_slice_traj = 65536;
     if (c.max_active <= 0) c.max_active = 64;
     if (c.block_traj <= 0) c.block_traj = 10000;
-    if (c.slice_traj <= 0) c.slice_traj = 262144;
+    if (c.slice_traj <= 0) c.slice_traj = 1 << 20;
+    if (c.slice_ms <= 0) c.slice_ms = 150;
+    if (c.policy != COSIMS_SERVE_SHORTEST_FIRST && c.policy != COSIMS_SERVE_EQUAL_SHARE)
+        return fail(COSIMS_E_INVALID, "unknown server policy");
     if (c.min_blocks <= 0) c.min_blocks = 3;
     if (c.max_blocks <= 0) c.max_blocks = 1000;
     if (c.min_blocks > c.max_blocks) return fail(COSIMS_E_INVALID, "min_blocks > max_blocks");

@@ -31,6 +31,8 @@ EXPORTS = [
     "cosims_server_destroy",
 ]
 E_PENDING = -8
+SERVE_SHORTEST_FIRST = 0
+SERVE_EQUAL_SHARE = 1
 
 
 class CosimsError(RuntimeError):
@@ -99,12 +101,17 @@ class BatchJob(C.Structure):
                 ("traj", C.c_int64), ("omega", C.c_void_p), ("omega_err", C.c_void_p), ("out", C.POINTER(Ccs))]
 
 
+SIZE_CLASSES = 5
+
+
 class BatchCfg(C.Structure):
-    _fields_ = [("prec", C.c_int32), ("lanes", C.c_int32), ("wg_size", C.c_int32), ("blocks_per_cu", C.c_int32)]
+    _fields_ = [("prec", C.c_int32), ("lanes", C.c_int32), ("wg_size", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("class_lanes", C.c_int32 * SIZE_CLASSES), ("class_wg", C.c_int32 * SIZE_CLASSES)]
 
 
 class BatchStats(C.Structure):
     _fields_ = [("lanes", C.c_int32), ("wg_size", C.c_int32), ("launches", C.c_int32),
+                ("class_lanes", C.c_int32 * SIZE_CLASSES), ("class_wg", C.c_int32 * SIZE_CLASSES),
                 ("kernel_ms", C.c_double), ("sample_ms", C.c_double), ("run_ms", C.c_double)] + \
                [(k, C.c_int64) for k in ("n_traj", "n_hit", "total_steps", "force_evals", "pot_evals",
                                           "pair_evals")]
@@ -113,7 +120,8 @@ class BatchStats(C.Structure):
 class ServerCfg(C.Structure):
     _fields_ = [("prec", C.c_int32), ("lanes", C.c_int32), ("wg_size", C.c_int32), ("hbm_budget", C.c_int64),
                 ("max_active", C.c_int32), ("min_blocks", C.c_int32), ("max_blocks", C.c_int32),
-                ("setup_threads", C.c_int32), ("block_traj", C.c_int64), ("slice_traj", C.c_int64)]
+                ("setup_threads", C.c_int32), ("block_traj", C.c_int64), ("slice_traj", C.c_int64),
+                ("slice_ms", C.c_double), ("policy", C.c_int32), ("min_slice_traj", C.c_int64)]
 
 
 class Request(C.Structure):
@@ -401,19 +409,32 @@ def _batch_jobs(jobs):
     return arr, keep
 
 
+def size_class(units):
+    """Size class of a molecule whose force loop sweeps `units` atoms (or clusters)."""
+    L, k = 4, 0
+    while L < 64 and units >= 48 * L:
+        L, k = 2 * L, k + 1
+    return k
+
+
 def tune_batch(jobs, prec=PREC_MIXED):
-    """Launch shape (lanes per trajectory, workgroup size) tuned on a batch."""
+    """(class_lanes, class_wg): the launch shape per size class tuned on a batch."""
     arr, _ = _batch_jobs(jobs)
-    L = C.c_int32(); W = C.c_int32()
-    _check(lib().cosims_tune_batch(arr, len(jobs), prec, C.byref(L), C.byref(W)))
-    return L.value, W.value
+    L = (C.c_int32 * SIZE_CLASSES)(); W = (C.c_int32 * SIZE_CLASSES)()
+    _check(lib().cosims_tune_batch(arr, len(jobs), prec, L, W))
+    return list(L), list(W)
 
 
-def run_batch(jobs, prec=PREC_MIXED, lanes=0, wg_size=0, blocks_per_cu=0):
-    """One persistent launch per force type over all jobs; returns (per-job
-    result dicts like Molecule.run, batch statistics)."""
+def run_batch(jobs, prec=PREC_MIXED, lanes=0, wg_size=0, blocks_per_cu=0, class_lanes=None, class_wg=None):
+    """Shared persistent launches (one per launch shape) over all jobs.
+    lanes > 0 fixes the lanes per trajectory, 0 applies the size rule, -1
+    uses class_lanes / class_wg (tuned on the batch when not given).
+    Returns (per-job result dicts like Molecule.run, batch statistics)."""
     arr, keep = _batch_jobs(jobs)
     cfg = BatchCfg(prec=prec, lanes=lanes, wg_size=wg_size, blocks_per_cu=blocks_per_cu)
+    for c in range(SIZE_CLASSES):
+        cfg.class_lanes[c] = (class_lanes or [0] * SIZE_CLASSES)[c]
+        cfg.class_wg[c] = (class_wg or [0] * SIZE_CLASSES)[c]
     st = BatchStats()
     _check(lib().cosims_run_batch(arr, len(jobs), C.byref(cfg), C.byref(st)))
     out = []
@@ -422,18 +443,23 @@ def run_batch(jobs, prec=PREC_MIXED, lanes=0, wg_size=0, blocks_per_cu=0):
                                                                      "round_evals", "round_items")}
         d.update(omega=om, omega_err=err)
         out.append(d)
-    return out, {k: getattr(st, k) for k, _ in BatchStats._fields_}
+    stats = {k: getattr(st, k) for k, _ in BatchStats._fields_}
+    stats["class_lanes"] = list(st.class_lanes)
+    stats["class_wg"] = list(st.class_wg)
+    return out, stats
 
 
 class Server:
     """CCS request server (cosims_server_*): continuous batching of requests
     over persistent launches, admission by slots and device-memory budget."""
 
-    def __init__(self, prec=PREC_MIXED, lanes=16, wg_size=512, hbm_budget=0, max_active=64, min_blocks=3,
-                 max_blocks=1000, setup_threads=2, block_traj=10000, slice_traj=262144):
+    def __init__(self, prec=PREC_MIXED, lanes=0, wg_size=512, hbm_budget=0, max_active=64, min_blocks=3,
+                 max_blocks=1000, setup_threads=2, block_traj=10000, slice_traj=0, slice_ms=0.0,
+                 policy=SERVE_SHORTEST_FIRST, min_slice_traj=0):
         self.cfg = ServerCfg(prec=prec, lanes=lanes, wg_size=wg_size, hbm_budget=hbm_budget, max_active=max_active,
                              min_blocks=min_blocks, max_blocks=max_blocks, setup_threads=setup_threads,
-                             block_traj=block_traj, slice_traj=slice_traj)
+                             block_traj=block_traj, slice_traj=slice_traj, slice_ms=slice_ms, policy=policy,
+                             min_slice_traj=min_slice_traj)
         h = C.c_void_p()
         _check(lib().cosims_server_create(C.byref(self.cfg), C.byref(h)))
         self._h = h

@@ -96,3 +96,34 @@ CONFIG_MOLECULES = {
 def config_ion(key: str):
     c = CONFIG_MOLECULES[key]
     return synthetic_ion(c["n"], c["radius"], c["sigma_q"], c["net_charge"], c["seed"])
+
+
+def uneven_conformers(count=256, n_min=50, n_max=5000, seed=3000):
+    """The batched uneven workload: atom counts log-uniform in [n_min, n_max],
+    radius 18 A x (N/2000)^(1/3), net charge round(N/250)+1 e, sigma_q 0.3 e.
+    Returns a list of (N, ion dict)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        ion = synthetic_ion(n, 18.0 * (n / 2000.0) ** (1 / 3), 0.3, round(n / 250) + 1, seed + 1 + k)
+        out.append((n, ion))
+    return out
+
+
+def serving_stream(count=40, rate=20.0, n_min=100, n_max=20000, seed=5000):
+    """The serving workload: Poisson arrivals at `rate` per second; atom counts
+    log-uniform in [n_min, n_max], radius 18 A x (N/2000)^(1/3), net charge
+    round(N/800)+1 e, sigma_q 0.3 e, He temperature uniform in 200-400 K.
+    Returns a list of dict(t (s after the first arrival), n, temp, ion)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for k in range(count):
+        if k:
+            t += rng.exponential(1.0 / rate)
+        n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        temp = float(rng.uniform(200.0, 400.0))
+        ion = synthetic_ion(n, 18.0 * (n / 2000.0) ** (1 / 3), 0.3, round(n / 800) + 1, seed + 1 + k)
+        out.append(dict(t=t, n=n, temp=temp, ion=ion))
+    return out

@@ -221,13 +221,22 @@ typedef struct cosims_ccs {
 int cosims_run(const cosims_mol *m, const cosims_params *p, const cosims_run_cfg *cfg, double *omega,
                double *omega_err, cosims_ccs *out);
 
-/* Many CCS runs in one persistent launch per force type.  The reference
- * runs one molecule per process (src/main.cpp:119-140, one
- * monteTraj::runTraj each); here molecules of uneven size share the
- * launch, trajectories of all of them on one work list (larger molecules
- * first), each group of lanes reading the atoms of its own trajectory's
- * molecule.  Each job's results are those cosims_run gives for the same
- * molecule, seed, job id and launch shape. */
+/* Many CCS runs sharing persistent launches.  The reference runs one
+ * molecule per process (src/main.cpp:119-140, one monteTraj::runTraj each);
+ * here molecules of uneven size share one work list per launch shape (force
+ * type x lanes per trajectory x workgroup size), larger molecules first,
+ * each group of lanes reading the atoms of its own trajectory's molecule;
+ * the launches of a batch run concurrently on separate streams.  Each job's
+ * results are those cosims_run gives for the same molecule, seed, job id
+ * and launch shape.
+ *
+ * Lanes per trajectory follow the molecule's size: size class k holds the
+ * molecules whose force loop sweeps n units (atoms, or clusters for the
+ * clustered types) with 48 * 4 * 2^(k-1) <= n < 48 * 4 * 2^k (class 0
+ * below 192, class 4 from 1536 up); the size rule gives class k 4 * 2^k
+ * lanes, the tuner times each class's choices. */
+#define COSIMS_SIZE_CLASSES 5
+
 typedef struct cosims_batch_job {
     const cosims_mol *mol;
     uint64_t seed;            /* Philox key (seed_ext) */
@@ -235,19 +244,23 @@ typedef struct cosims_batch_job {
     int32_t iter;             /* CCS integrals; 0 => the molecule's params */
     int64_t traj;             /* trajectories per integral; 0 => the molecule's params */
     double *omega, *omega_err;/* iter entries each (caller-owned) */
-    cosims_ccs *out;          /* per-job statistics */
+    cosims_ccs *out;          /* per-job statistics (lanes / wg_size: the job's launch shape) */
 } cosims_batch_job;
 
 typedef struct cosims_batch_cfg {
     int32_t prec;             /* COSIMS_PREC_* */
-    int32_t lanes;            /* lanes per trajectory 4..64; 0 => tuned on the batch */
-    int32_t wg_size;          /* 0 => tuned */
+    int32_t lanes;            /* > 0 (4..64): every job; 0: the size rule; -1: per size class,
+                                 from class_lanes / class_wg, or tuned on the batch when all zero */
+    int32_t wg_size;          /* workgroup size where no class value applies; 0 => 512 */
     int32_t blocks_per_cu;    /* 0 => occupancy limit */
+    int32_t class_lanes[COSIMS_SIZE_CLASSES];
+    int32_t class_wg[COSIMS_SIZE_CLASSES];
 } cosims_batch_cfg;
 
 typedef struct cosims_batch_stats {
     int32_t lanes, wg_size, launches;
-    double kernel_ms;         /* HIP-event time of the trajectory launches */
+    int32_t class_lanes[COSIMS_SIZE_CLASSES], class_wg[COSIMS_SIZE_CLASSES];
+    double kernel_ms;         /* first launch to last launch end (HIP events) */
     double sample_ms, run_ms; /* sampling launches; whole call */
     int64_t n_traj, n_hit, total_steps, force_evals, pot_evals;
     int64_t pair_evals;       /* atom pairs of the exact force types' force evaluations */
@@ -255,9 +268,10 @@ typedef struct cosims_batch_stats {
 
 int cosims_run_batch(cosims_batch_job *jobs, int32_t njobs, const cosims_batch_cfg *cfg,
                      cosims_batch_stats *stats);
-/* The launch shape cosims_run_batch would tune for these jobs. */
-int cosims_tune_batch(const cosims_batch_job *jobs, int32_t njobs, int32_t prec, int32_t *lanes,
-                      int32_t *wg_size);
+/* Per size class, the lanes and workgroup size the tuner picks for these
+ * jobs (0 for classes without jobs). */
+int cosims_tune_batch(const cosims_batch_job *jobs, int32_t njobs, int32_t prec, int32_t *class_lanes,
+                      int32_t *class_wg);
 
 /* ------------------------------------------------------------ serving
  * A CCS request server; the reference runs one molecule per process
@@ -271,21 +285,32 @@ int cosims_tune_batch(const cosims_batch_job *jobs, int32_t njobs, int32_t prec,
  * requests arriving meanwhile join the next slice.  A request retires at the
  * first block, from min_blocks on, where werr / wmean <= target (or at
  * max_blocks); its result is that of cosims_run with iter = blocks,
- * traj = block_traj and the server's launch shape, whatever shared the
- * launches.  Per-block sums stay on the device until read. */
+ * traj = block_traj and the request's launch shape (fixed lanes, or the size
+ * rule of cosims_run_batch), whatever shared the launches.  Per-block sums
+ * stay on the device until read. */
 typedef struct cosims_server cosims_server;
 
 typedef struct cosims_server_cfg {
     int32_t prec;             /* COSIMS_PREC_* */
-    int32_t lanes, wg_size;   /* launch shape, fixed for the server; 0 => 16, 512 */
+    int32_t lanes;            /* lanes per trajectory for every request; 0 => the size rule */
+    int32_t wg_size;          /* 0 => 512 */
     int64_t hbm_budget;       /* device bytes for admitted requests; 0 => 80 % of free memory */
     int32_t max_active;       /* admitted requests at once; 0 => 64 */
     int32_t min_blocks;       /* blocks before the stopping rule applies; 0 => 3 */
     int32_t max_blocks;       /* blocks at most per request; 0 => 1000 */
     int32_t setup_threads;    /* host setup workers; 0 => 2 */
     int64_t block_traj;       /* trajectories per block; 0 => 10000 */
-    int64_t slice_traj;       /* trajectories per slice to aim for; 0 => 262144 */
+    int64_t slice_traj;       /* trajectories per slice at most; 0 => 2^20 */
+    double slice_ms;          /* kernel time per slice to aim for; 0 => 150 */
+    int32_t policy;           /* COSIMS_SERVE_* */
+    int64_t min_slice_traj;   /* shortest-first slices are topped up to this many; 0 => 65536 */
 } cosims_server_cfg;
+
+/* Slice composition: shortest remaining work first (requests ordered by
+ * blocks still wanted x measured cost per block fill the slice's work
+ * budget), or an equal share of blocks for every active request. */
+#define COSIMS_SERVE_SHORTEST_FIRST 0
+#define COSIMS_SERVE_EQUAL_SHARE 1
 
 typedef struct cosims_request {
     cosims_params params;     /* as for cosims_mol_create; seed keys the request's stream */

@@ -109,17 +109,43 @@ def test_batch_alone_equals_mix(ion128, ion2000):
         assert mix[k]["wmean"] == alone["wmean"]
 
 
+def test_batch_size_rule_alone_equals_mix(ion128, ion2000):
+    """Lanes by molecule size: one launch per (force type, lanes), run
+    concurrently; each job's result is the same alone and in the mix."""
+    g = golden("acau_charged")
+    ac6 = golden_engine_mol(g, [0, 0, 0, 1])   # neutral exact, type 6, 252 atoms
+    jobs = [(ion128, dict(job=31, iters=1, traj=3000)), (ion2000, dict(job=32, iters=1, traj=3000)),
+            (ac6, dict(job=33, iters=1, traj=3000))]
+    mix, st = E.run_batch(jobs, prec=E.PREC_MIXED, lanes=0)
+    assert st["launches"] == 3
+    assert [r["lanes"] for r in mix] == [4, 64, 8]  # size classes 0, 4, 1
+    for k, job in enumerate(jobs):
+        (alone,), _ = E.run_batch([job], prec=E.PREC_MIXED, lanes=0)
+        assert np.array_equal(mix[k]["omega"], alone["omega"]), k
+        single = job[0].run(E.PREC_MIXED, job=job[1]["job"], iters=1, traj=3000, lanes=mix[k]["lanes"],
+                            wg_size=512)
+        assert np.array_equal(mix[k]["omega"], single["omega"]), k
+
+
 def test_batch_tuner_and_stats(ion128, ion2000):
     jobs = [(ion128, dict(job=21, iters=1, traj=2000)), (ion2000, dict(job=22, iters=1, traj=2000))]
-    L, W = E.tune_batch(jobs, E.PREC_MIXED)
-    assert L in (4, 8, 16, 32, 64) and W in (256, 512, 1024)
-    res, st = E.run_batch(jobs, prec=E.PREC_MIXED)  # tunes again: timing may pick another shape
-    assert st["lanes"] in (4, 8, 16, 32, 64) and st["wg_size"] in (256, 512, 1024)
+    CL, CW = E.tune_batch(jobs, E.PREC_MIXED)
+    present = {E.size_class(128), E.size_class(2000)}
+    for c in range(E.SIZE_CLASSES):
+        if c in present:
+            assert CL[c] in (4, 8, 16, 32, 64) and CW[c] in (256, 512, 1024), (c, CL, CW)
+        else:
+            assert CL[c] == 0 and CW[c] == 0
+    res, st = E.run_batch(jobs, prec=E.PREC_MIXED, lanes=-1, class_lanes=CL, class_wg=CW)
+    assert st["class_lanes"] == CL
+    assert [r["lanes"] for r in res] == [CL[E.size_class(128)], CL[E.size_class(2000)]]
     assert st["kernel_ms"] > 0 and st["force_evals"] > st["total_steps"] > 0
     # every force evaluation of these exact-type molecules is a full atom sweep
     assert st["pair_evals"] >= 128 * st["force_evals"]
     for r in res:
         assert r["n_traj"] == 2000 and 0 < r["n_hit"] <= 2000 and np.isfinite(r["wmean"])
+    _, st2 = E.run_batch(jobs, prec=E.PREC_MIXED, lanes=-1)  # tunes inside
+    assert all(st2["class_lanes"][c] for c in present)
 
 
 @pytest.fixture(scope="module")

@@ -91,6 +91,17 @@ def test_server_alone_equals_mix(mixed_results):
         assert np.array_equal(alone["omega"], res[k]["omega"]), req[0]
 
 
+def test_server_policy_does_not_change_results(mixed_results):
+    """Equal-share slices instead of shortest-remaining-work-first: other
+    slice compositions, the same results."""
+    res, _ = mixed_results
+    with _server(policy=E.SERVE_EQUAL_SHARE, slice_traj=8192) as srv:
+        tickets = _submit_all(srv, REQUESTS)
+        out = [srv.wait(t) for t in tickets]
+    for r, ref in zip(out, res):
+        assert np.array_equal(r["omega"], ref["omega"])
+
+
 def test_server_budget_serialises_admission(mixed_results):
     """A one-byte device-memory budget admits one request at a time; the
     results do not change."""
@@ -118,6 +129,21 @@ def test_server_continuous_admission():
         a, b = srv.wait(first), srv.wait(late)
     assert a["blocks"] == b["blocks"] == 100
     assert b["t_admit_ms"] < a["t_done_ms"]
+
+
+def test_server_size_rule_equals_run():
+    """lanes=0: every request runs with the lanes of its size class; the
+    result is cosims_run with those lanes."""
+    with _server(lanes=0, wg_size=512) as srv:
+        tickets = _submit_all(srv, REQUESTS)
+        res = [srv.wait(t) for t in tickets]
+    for (f, p, j, t), r in zip(REQUESTS, res):
+        m = E.Molecule.from_file(os.path.join(GOLDEN, f), params=_params(100 + j, **p))
+        inf = m.info()
+        units = inf["n"] if inf["force_type"] in (4, 6) else inf["ncl"]
+        L = 4 << E.size_class(units)
+        run = m.run(E.PREC_MIXED, job=j, iters=r["blocks"], traj=BLOCK, lanes=L, wg_size=512)
+        assert np.array_equal(r["omega"], run["omega"]), (f, L)
 
 
 def test_server_rejects_bad_requests():
