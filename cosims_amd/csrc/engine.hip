@@ -747,12 +747,14 @@ int upload(const cosims_mol *M)
         soa[5 * npad + i] = real ? p.eps24m[i] : 0.0;
         soa[6 * npad + i] = real ? p.eps4[i] : 0.0;
     }
-    // FP32 SoA of the same (x, y, z, q | sigma^6, 24 eps/mu)
+    // FP32 SoA of the group kernels (x, y, z, q | LJ constants A, B)
     auto *xq = (float4 *)(host.data() + oxq);
     auto *se = (float2 *)(host.data() + ose);
     for (int i = 0; i < npad; i++) {
         xq[i] = make_float4((float)soa[i], (float)soa[npad + i], (float)soa[2 * npad + i], (float)soa[3 * npad + i]);
-        se[i] = make_float2((float)soa[4 * npad + i], (float)soa[5 * npad + i]);
+        float A, B;
+        lj_constants(soa[4 * npad + i], soa[5 * npad + i], A, B);
+        se[i] = make_float2(A, B);
     }
     HIP_CHECK(hipMalloc(&M->dmem, total));
     M->dbytes = total;
@@ -1218,6 +1220,11 @@ MultiFn multi_fn(int ft, int L)
 
 MultiFn multi_kernel(int prec, int ft, int L) { return prec == COSIMS_PREC_FP64 ? multi_fn<0>(ft, L) : multi_fn<1>(ft, L); }
 
+size_t multi_lds_bytes(int ft, int L, int wg)
+{
+    return multi_uses_lists(ft, L) ? (size_t)(wg / 64) * kListInts * sizeof(int) : 0;
+}
+
 size_t group_lds_bytes(const cosims_mol *M, int prec)
 {
     return (size_t)M->npad * (prec == COSIMS_PREC_FP64 ? 6 * sizeof(double) : sizeof(float4) + sizeof(float2));
@@ -1318,8 +1325,9 @@ int run_multi(const MolRec *mols, const int32_t *tmol, int ft, int prec, int cus
     G.io = traj_io(B, items, n_items);
     G.mols = mols;
     G.tmol = tmol;
-    const int64_t grid = group_grid((const void *)fn, wg, 0, bpc, cus, n_items, L);
-    return timed_group_launch(fn, G, grid, wg, 0, s, B, n_items, out);
+    const size_t lds = multi_lds_bytes(ft, L, wg);
+    const int64_t grid = group_grid((const void *)fn, wg, lds, bpc, cus, n_items, L);
+    return timed_group_launch(fn, G, grid, wg, lds, s, B, n_items, out);
 }
 
 // One persistent group launch of one molecule over the item list B.items[0].
@@ -1430,7 +1438,8 @@ void finish_ccs(const int8_t *status, const double *sums, int iters, int64_t T, 
 // Launch-shape search shared by the single-molecule and batch tuners:
 // every lanes-per-trajectory L (the atom tile each lane strides over) x
 // workgroup size W (W/L trajectories per workgroup) is timed on a small
-// item set; the three fastest are re-timed on a set large enough that the
+// item set; the three fastest are re-timed twice (the faster run counts:
+// launch times vary by a few per cent) on a set large enough that the
 // end-of-launch tail is amortised.  run(set, L, W, ms) times one launch.
 template <class RunFn>
 int tune_shapes(int units, RunFn run, int &L_out, int &W_out)
@@ -1452,8 +1461,12 @@ int tune_shapes(int units, RunFn run, int &L_out, int &W_out)
     }
     std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
     if (cands.size() > 3) cands.resize(3);
-    for (Cand &c : cands)
-        if (int rc = run(1, c.L, c.W, c.ms)) return rc;
+    for (Cand &c : cands) {
+        double a = 0, b = 0;
+        if (int rc = run(1, c.L, c.W, a)) return rc;
+        if (int rc = run(1, c.L, c.W, b)) return rc;
+        c.ms = std::min(a, b);
+    }
     std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
     L_out = cands[0].L;
     W_out = cands[0].W;
@@ -1473,7 +1486,7 @@ int autotune(const cosims_mol *M, int prec, int &L_out, int &wg_out)
     }
     Stream st;
     HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
-    const int64_t nset[2] = {16384, std::max<int64_t>(16384, (int64_t)M->dev.cus * 512)};
+    const int64_t nset[2] = {16384, std::max<int64_t>(16384, (int64_t)M->dev.cus * 1024)};
     TrajBufs B;
     if (int rc = B.alloc(nset[1])) return rc;
     int cur = -1;
@@ -1739,8 +1752,9 @@ int launch_batch(BatchPlan &P, int prec, int bpc, hipStream_t s, GroupRun &tot, 
         G.io.evals = P.seg_evals.as<unsigned long long>() + 4 * i;
         G.mols = P.table.as<MolRec>();
         G.tmol = P.tmol.as<int32_t>();
-        const int64_t grid = group_grid((const void *)fn, g.W, 0, bpc, P.cus, cnt, g.L);
-        hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(g.W), 0, x, G);
+        const size_t lds = multi_lds_bytes(g.ft, g.L, g.W);
+        const int64_t grid = group_grid((const void *)fn, g.W, lds, bpc, P.cus, cnt, g.L);
+        hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(g.W), lds, x, G);
         HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_chi, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, x, G.io.items, cnt, G.io.status,
                            G.io.cchi, G.io.chi);

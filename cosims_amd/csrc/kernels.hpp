@@ -196,6 +196,22 @@ __device__ __forceinline__ void cl_lj(const MolDev &m, int s, int e, T fs6, T fe
     }
 }
 
+// The same two per-atom loops with the variant chosen at run time (used
+// where lanes handle clusters of different kinds side by side).
+template <typename T>
+__device__ __forceinline__ void cl_exact_rt(const MolDev &m, int s, int e, bool lj, T px, T py, T pz, ClAcc<T> &A)
+{
+    if (lj) cl_exact<T, true>(m, s, e, px, py, pz, A);
+    else cl_exact<T, false>(m, s, e, px, py, pz, A);
+}
+template <typename T>
+__device__ __forceinline__ void cl_lj_rt(const MolDev &m, int s, int e, bool fixed, T fs6, T fe24, T px, T py, T pz,
+                                         ClAcc<T> &A)
+{
+    if (fixed) cl_lj<T, true>(m, s, e, fs6, fe24, px, py, pz, A);
+    else cl_lj<T, false>(m, s, e, T(0), T(0), px, py, pz, A);
+}
+
 // far-field multipole of one cluster, src/forceAccel.cpp:258-337
 template <typename T>
 __device__ __forceinline__ void cl_far(const MolDev &m, const CAS ClusterT<T> &H, T dx0, T dx1, T dx2, T r2,

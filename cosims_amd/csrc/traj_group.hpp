@@ -12,8 +12,8 @@
 //
 // Two kernels share the trajectory state machine (group_loop):
 //   k_traj_grp    one molecule, staged once per workgroup in LDS (SoA: FP32
-//                 x,y,z,q | sigma^6, 24 eps/mu as float4 + float2, or the FP64
-//                 equivalents), exact force types 4 and 6;
+//                 x,y,z,q | LJ constants A, B as float4 + float2, or the FP64
+//                 x,y,z,q,sigma^6,24 eps/mu), exact force types 4 and 6;
 //   k_traj_multi  any number of molecules of one force type, read from HBM
 //                 through L2 (a table of MolRec, one molecule per trajectory):
 //                 molecules of uneven size in one launch, molecules too large
@@ -67,7 +67,7 @@ struct GroupArgs {
 struct MolRec {
     MolDev m;
     const float4 *xq;                      // FP32 x, y, z, q (npad)
-    const float2 *se;                      // FP32 sigma^6, 24 eps/mu (npad)
+    const float2 *se;                      // FP32 LJ constants A, B of grp_force32 (npad)
     const double *soa;                     // FP64 SoA as in GroupArgs
     int32_t npad, pad_;
     double dt0;
@@ -100,46 +100,57 @@ __device__ __forceinline__ T gsum(T v)
     return v;
 }
 
-// FP32 pair kernel of the exact-sum types over atoms j, j+L, ... < npad.
+// FP32 pair kernel of the exact-sum types over atoms j, j+L, ... < npad,
+// same physics as src/forceAccel.cpp:764-802 rearranged for the VALU (30
+// instructions a pair): per atom x, y, z, q and the LJ constants
+// A = 2 (24 eps/mu) sigma^12, B = (24 eps/mu) sigma^6, so the LJ factor is
+// r^-8 (A r^-6 - B) (src/forceAccel.cpp:785-788: 24 eps/mu (2 t^2 - t) / r^2
+// with t = sigma^6 r^-6); one v_rsq_f32; the field gradient accumulated as
+// S_ab = sum q d_a d_b r^-5 (xx, yy, xy, xz, yz) and its trace sum q r^-3,
+// with T = 3 S - tr(S) formed once after the loop (the reference sums
+// (3 d_a d_b - r^2 delta_ab) q r^-5 term by term, :790-797).
 #pragma clang fp contract(fast)
 template <int FT, int L, typename P4, typename P2>
-__device__ __forceinline__ void grp_force32(P4 xq, P2 se, int npad, int j, float px, float py, float pz, double ds,
+__device__ __forceinline__ void grp_force32(P4 xq, P2 ab, int npad, int j, float px, float py, float pz, double ds,
                                             double &ox, double &oy, double &oz)
 {
-    float fx = 0, fy = 0, fz = 0, ex = 0, ey = 0, ez = 0, sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0;
-#pragma unroll 2
-    for (int k = j; k < npad; k += L) {
+    float fx = 0, fy = 0, fz = 0, ex = 0, ey = 0, ez = 0, sxx = 0, syy = 0, sxy = 0, sxz = 0, syz = 0, tr = 0;
+#pragma unroll 4
+    for (int kk = 0; kk < npad; kk += L) {
+        const int k = kk + j;
         const f4n a = xq[k];
-        const f2n b = se[k];
-        float dx = px - a.x, dy = py - a.y, dz = pz - a.z;
-        float r2 = dx * dx + dy * dy + dz * dz;
+        const f2n c = ab[k];
+        const float dx = px - a.x, dy = py - a.y, dz = pz - a.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        float ir2, ir;
         if constexpr (FT == 4) {
-            float ir = __builtin_amdgcn_rsqf(r2);
-            float ir2 = ir * ir;
-            float t2 = b.x * ir2 * ir2 * ir2;
-            float lj = b.y * ir2 * t2 * (2.0f * t2 - 1.0f);
-            fx += lj * dx; fy += lj * dy; fz += lj * dz;
-            float qr3 = a.w * ir2 * ir;
-            ex += qr3 * dx; ey += qr3 * dy; ez += qr3 * dz;
-            float qr5 = qr3 * ir2;
-            float u = qr5 * dx, v = qr5 * dy, w = qr5 * dz;
-            sxx += u * dx; syy += v * dy; szz += w * dz;
-            sxy += u * dy; sxz += u * dz; syz += v * dz;
+            ir = __builtin_amdgcn_rsqf(r2);
+            ir2 = ir * ir;
         } else {
-            float ir2 = __builtin_amdgcn_rcpf(r2);
-            float t2 = b.x * ir2 * ir2 * ir2;
-            float lj = b.y * ir2 * t2 * (2.0f * t2 - 1.0f);
-            fx += lj * dx; fy += lj * dy; fz += lj * dz;
+            ir2 = __builtin_amdgcn_rcpf(r2);
+        }
+        const float ir4 = ir2 * ir2;
+        const float ir6 = ir4 * ir2;
+        const float lj = (ir4 * ir4) * (c.x * ir6 - c.y);
+        fx += lj * dx; fy += lj * dy; fz += lj * dz;
+        if constexpr (FT == 4) {
+            const float qr3 = a.w * ir2 * ir;
+            ex += qr3 * dx; ey += qr3 * dy; ez += qr3 * dz;
+            tr += qr3;
+            const float qr5 = qr3 * ir2;
+            const float u = qr5 * dx, v = qr5 * dy;
+            sxx += u * dx; syy += v * dy;
+            sxy += u * dy; sxz += u * dz; syz += v * dz;
         }
     }
     fx = gsum<L>(fx); fy = gsum<L>(fy); fz = gsum<L>(fz);
     if constexpr (FT == 4) {
         ex = gsum<L>(ex); ey = gsum<L>(ey); ez = gsum<L>(ez);
-        sxx = gsum<L>(sxx); syy = gsum<L>(syy); szz = gsum<L>(szz);
+        sxx = gsum<L>(sxx); syy = gsum<L>(syy); tr = gsum<L>(tr);
         sxy = gsum<L>(sxy); sxz = gsum<L>(sxz); syz = gsum<L>(syz);
-        double tr = (double)sxx + (double)syy + (double)szz;
-        double Txx = 3.0 * sxx - tr, Tyy = 3.0 * syy - tr, Tzz = 3.0 * szz - tr;
-        double Txy = 3.0 * sxy, Txz = 3.0 * sxz, Tyz = 3.0 * syz;
+        const double t = tr, Sxx = sxx, Syy = syy;
+        const double Txx = 3.0 * Sxx - t, Tyy = 3.0 * Syy - t, Tzz = 2.0 * t - 3.0 * Sxx - 3.0 * Syy;
+        const double Txy = 3.0 * sxy, Txz = 3.0 * sxz, Tyz = 3.0 * syz;
         ox = (double)fx - ds * (Txx * ex + Txy * ey + Txz * ez);
         oy = (double)fy - ds * (Txy * ex + Tyy * ey + Tyz * ez);
         oz = (double)fz - ds * (Txz * ex + Tyz * ey + Tzz * ez);
@@ -148,6 +159,14 @@ __device__ __forceinline__ void grp_force32(P4 xq, P2 se, int npad, int j, float
     }
 }
 #pragma clang fp contract(off)
+
+// FP32 LJ constants of one atom for grp_force32 (identical on host and device:
+// IEEE double products, no contraction, then one rounding to float).
+__host__ __device__ inline void lj_constants(double s6, double e24, float &A, float &B)
+{
+    A = (float)(2.0 * e24 * s6 * s6);
+    B = (float)(e24 * s6);
+}
 
 // FP64 reference-form pair terms (src/forceAccel.cpp:764-797, 865-880),
 // partial sums of this lane, then the group butterfly and the epilogue
@@ -213,6 +232,88 @@ __device__ __forceinline__ void grp_clustered(const MolDev &m, int j, T px, T py
     }
     if constexpr (FT == 1 || FT == 2) {
         A.aXY = gsum<L>(A.aXY); A.aXZ = gsum<L>(A.aXZ); A.aYZ = gsum<L>(A.aYZ);
+    }
+    T a, b, c;
+    cl_epilogue<T, FT>(m, A, a, b, c);
+    ox = a; oy = b; oz = c;
+}
+
+// Clustered force types with one group per wavefront (L = 64).  Lanes
+// classify clusters j, j+64, ... in chunks of 64 x kChunk: a far cluster's
+// multipole terms are added at once (most clusters are far, so most lanes
+// are busy), while near clusters (exact charges, with or without LJ) and LJ
+// clusters (band or per-atom LJ) are appended to two lists in LDS by ballot
+// and then taken lane by lane.  Every lane of a list loop runs the same kind
+// of per-atom loop, instead of every lane stepping through all three
+// branches of whichever clusters its neighbours hold.  Terms and
+// accumulators are those of cl_one (reference per-cluster arithmetic).
+constexpr int kChunk = 4;
+constexpr int kListInts = 2 * 64 * kChunk;  // LDS ints per wavefront
+
+template <typename T, int FT>
+__device__ __forceinline__ void wave_clustered(const MolDev &m, int ncl, int lane, T px, T py, T pz, int *lists,
+                                               double &ox, double &oy, double &oz)
+{
+    ClAcc<T> A;
+    const auto H = Layout<T>::heads(m);
+    int *ex_list = lists, *lj_list = lists + 64 * kChunk;
+    const unsigned long long below = (1ull << lane) - 1;
+    for (int base = 0; base < ncl; base += 64 * kChunk) {
+        int nex = 0, nlj = 0;
+#pragma unroll
+        for (int q = 0; q < kChunk; q++) {
+            const int c = base + q * 64 + lane;
+            int kex = 0, klj = 0;  // exact: 1 charges, 2 charges + LJ; LJ: 1 per-atom, 2 fixed parameters
+            if (c < ncl) {
+                const CAS ClusterT<T> &h = H[c];
+                const T dx0 = px - h.px, dx1 = py - h.py, dx2 = pz - h.pz;
+                const T r2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+                if constexpr (FT == 1 || FT == 2) {
+                    if (r2 > h.mpr) {
+                        cl_far<T>(m, h, dx0, dx1, dx2, r2, A);
+                        if (FT == 2) klj = 1;
+                        else if (!(r2 > h.dpr)) klj = 2;
+                    } else {
+                        kex = (FT == 2 || !(r2 > h.dpr)) ? 2 : 1;
+                    }
+                } else if constexpr (FT == 3) {
+                    kex = !(r2 > h.dpr) ? 2 : 1;
+                } else {
+                    if (!(r2 > h.dpr)) klj = 1;
+                }
+            }
+            const unsigned long long me = __ballot(kex != 0), ml = __ballot(klj != 0);
+            if (kex) ex_list[nex + __popcll(me & below)] = 4 * c + kex;
+            if (klj) lj_list[nlj + __popcll(ml & below)] = 4 * c + klj;
+            nex += __popcll(me);
+            nlj += __popcll(ml);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int e = lane; e < nex; e += 64) {
+            const int v = ex_list[e];
+            const CAS ClusterT<T> &h = H[v >> 2];
+            cl_exact_rt<T>(m, h.start, h.end, (v & 3) == 2, px, py, pz, A);
+        }
+        for (int e = lane; e < nlj; e += 64) {
+            const int v = lj_list[e];
+            const CAS ClusterT<T> &h = H[v >> 2];
+            cl_lj_rt<T>(m, h.start, h.end, (v & 3) == 2, h.qs6, h.qe24, px, py, pz, A);
+        }
+        // the next chunk overwrites the lists: every lane is past its reads
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    A.sumX = gsum<64>(A.sumX); A.sumY = gsum<64>(A.sumY); A.sumZ = gsum<64>(A.sumZ);
+    if constexpr (FT != 5) {
+        A.fX = gsum<64>(A.fX); A.fY = gsum<64>(A.fY); A.fZ = gsum<64>(A.fZ);
+        A.dXX = gsum<64>(A.dXX); A.dYY = gsum<64>(A.dYY); A.dZZ = gsum<64>(A.dZZ);
+        A.sXY = gsum<64>(A.sXY); A.sXZ = gsum<64>(A.sXZ); A.sYZ = gsum<64>(A.sYZ);
+    }
+    if constexpr (FT == 1 || FT == 2) {
+        A.aXY = gsum<64>(A.aXY); A.aXZ = gsum<64>(A.aXZ); A.aYZ = gsum<64>(A.aYZ);
     }
     T a, b, c;
     cl_epilogue<T, FT>(m, A, a, b, c);
@@ -286,6 +387,7 @@ struct LdsSource {
 template <int PREC, int FT, int L>
 struct TableSource {
     const MultiArgs &A;
+    int *lists;  // this wavefront's cluster lists in LDS (clustered types, L = 64)
     int mi = 0;
     __device__ void bind(int t) { mi = A.tmol ? A.tmol[t] : 0; }
     __device__ const MolRec &rec() const { return A.mols[mi]; }
@@ -310,6 +412,12 @@ struct TableSource {
                                    (float)ez, r.m.dipole_scalar, fx, fy, fz);
             else
                 grp_force64<FT, L>(glb(r.soa), npad, j, ex, ey, ez, r.m.dipole_scalar, fx, fy, fz);
+        } else if constexpr (L == 64) {
+            const int ncl = active ? r.m.ncl : 0;
+            if constexpr (PREC == 1)
+                wave_clustered<float, FT>(r.m, ncl, j, (float)ex, (float)ey, (float)ez, lists, fx, fy, fz);
+            else
+                wave_clustered<double, FT>(r.m, ncl, j, ex, ey, ez, lists, fx, fy, fz);
         } else {
             MolDev m = r.m;
             if (!active) m.ncl = 0;
@@ -507,7 +615,9 @@ __global__ void __launch_bounds__(1024) k_traj_grp(GroupArgs A)
         float2 *se = (float2 *)(xq + npad);
         for (int i = threadIdx.x; i < npad; i += blockDim.x) {
             xq[i] = make_float4((float)S[i], (float)S[npad + i], (float)S[2 * npad + i], (float)S[3 * npad + i]);
-            se[i] = make_float2((float)S[4 * npad + i], (float)S[5 * npad + i]);
+            float a, b;
+            lj_constants(S[4 * npad + i], S[5 * npad + i], a, b);
+            se[i] = make_float2(a, b);
         }
     } else {
         double *X = (double *)smem;
@@ -518,10 +628,16 @@ __global__ void __launch_bounds__(1024) k_traj_grp(GroupArgs A)
     group_loop<L>(A.io, src);
 }
 
+// Dynamic LDS of k_traj_multi: the cluster lists of each wavefront when
+// clustered types run one group per wavefront, else none.
+__host__ __device__ constexpr bool multi_uses_lists(int ft, int L) { return ft != 4 && ft != 6 && L == 64; }
+
 template <int PREC, int FT, int L>
 __global__ void __launch_bounds__(1024) k_traj_multi(MultiArgs A)
 {
-    TableSource<PREC, FT, L> src{A};
+    extern __shared__ int lds_lists[];
+    int *lists = multi_uses_lists(FT, L) ? lds_lists + (threadIdx.x >> 6) * kListInts : nullptr;
+    TableSource<PREC, FT, L> src{A, lists};
     group_loop<L>(A.io, src);
 }
 

@@ -43,7 +43,7 @@ def test_clustered_group_fp64_matches_lane_kernel(case, flags):
     assert ft in (1, 2, 3, 5)
     kw = dict(job=3, iters=2, traj=2000, keep=True)
     lane = m.run(E.PREC_FP64, lanes=1, **kw)
-    for L in (4, 16):
+    for L in (4, 16, 64):  # 64: one group per wavefront, clusters sorted into LDS lists
         grp = m.run(E.PREC_FP64, lanes=L, wg_size=256, **kw)
         assert grp["lanes"] == L
         assert np.array_equal(grp["status"] != 0, lane["status"] != 0)
@@ -59,11 +59,12 @@ def test_clustered_group_mixed_close_to_fp64(case, flags):
     m = golden_engine_mol(golden(case), flags)
     kw = dict(job=4, iters=2, traj=4000, keep=True)
     f64 = m.run(E.PREC_FP64, lanes=1, **kw)
-    mix = m.run(E.PREC_MIXED, **kw)
-    assert mix["lanes"] >= 4
-    assert np.array_equal(mix["status"] != 0, f64["status"] != 0)
-    dev = abs(mix["wmean"] - f64["wmean"]) / f64["wmean"]
-    assert dev <= 1e-3, dev
+    for lanes in (0, 64):
+        mix = m.run(E.PREC_MIXED, lanes=lanes, **kw)
+        assert mix["lanes"] >= 4
+        assert np.array_equal(mix["status"] != 0, f64["status"] != 0)
+        dev = abs(mix["wmean"] - f64["wmean"]) / f64["wmean"]
+        assert dev <= 1e-3, (lanes, dev)
 
 
 @pytest.fixture(scope="module")
