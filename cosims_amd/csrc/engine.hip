@@ -1486,7 +1486,9 @@ int autotune(const cosims_mol *M, int prec, int &L_out, int &wg_out)
     }
     Stream st;
     HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
-    const int64_t nset[2] = {16384, std::max<int64_t>(16384, (int64_t)M->dev.cus * 1024)};
+    // stage 1 holds several rounds even at 4 lanes per trajectory (16 trajectories per wavefront)
+    const int64_t nset[2] = {std::max<int64_t>(16384, (int64_t)M->dev.cus * 256),
+                             std::max<int64_t>(16384, (int64_t)M->dev.cus * 1024)};
     TrajBufs B;
     if (int rc = B.alloc(nset[1])) return rc;
     int cur = -1;

@@ -92,11 +92,58 @@ __global__ void __launch_bounds__(256) k_chi(const int32_t *items, int n, const 
     chi[t] = (t3 > 1) ? 0.0 : ((t3 < -1) ? 3.14159265358979323846 : acos(t3));
 }
 
+// Sum over the L lanes of a group (aligned, L a power of two <= 64), the
+// result in every lane, on the VALU: DPP quad permutations, half-row and row
+// mirrors (within 16 lanes), then the gfx950 row and half-wave swaps.  Each
+// step adds two operands that every lane of the group sees in the same
+// order, so all lanes hold bitwise-identical sums (every lane carries the
+// same Verlet state).  Needs all 64 lanes active.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_step(float v)
+{
+    return v + __builtin_bit_cast(float, dpp_u32<CTRL>(__builtin_bit_cast(uint32_t, v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_step(double v)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint64_t w = (uint64_t)dpp_u32<CTRL>((uint32_t)u) | ((uint64_t)dpp_u32<CTRL>((uint32_t)(u >> 32)) << 32);
+    return v + __builtin_bit_cast(double, w);
+}
+__device__ __forceinline__ float swap_step(float v, bool half_wave)
+{
+    const uint32_t u = __builtin_bit_cast(uint32_t, v);
+    const auto r = half_wave ? __builtin_amdgcn_permlane32_swap(u, u, false, false)
+                             : __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    return __builtin_bit_cast(float, (uint32_t)r[0]) + __builtin_bit_cast(float, (uint32_t)r[1]);
+}
+__device__ __forceinline__ double swap_step(double v, bool half_wave)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)u, hi = (uint32_t)(u >> 32);
+    const auto a = half_wave ? __builtin_amdgcn_permlane32_swap(lo, lo, false, false)
+                             : __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto b = half_wave ? __builtin_amdgcn_permlane32_swap(hi, hi, false, false)
+                             : __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    const double x = __builtin_bit_cast(double, (uint64_t)(uint32_t)a[0] | ((uint64_t)(uint32_t)b[0] << 32));
+    const double y = __builtin_bit_cast(double, (uint64_t)(uint32_t)a[1] | ((uint64_t)(uint32_t)b[1] << 32));
+    return x + y;
+}
 template <int L, typename T>
 __device__ __forceinline__ T gsum(T v)
 {
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    static_assert(L >= 4 && L <= 64 && (L & (L - 1)) == 0, "group size");
+    v = dpp_step<0xB1>(v);                       // quad_perm [1,0,3,2]
+    v = dpp_step<0x4E>(v);                       // quad_perm [2,3,0,1]
+    if constexpr (L >= 8) v = dpp_step<0x141>(v);   // row_half_mirror
+    if constexpr (L >= 16) v = dpp_step<0x140>(v);  // row_mirror
+    if constexpr (L >= 32) v = swap_step(v, false); // rows 2k <-> 2k+1
+    if constexpr (L >= 64) v = swap_step(v, true);  // half-waves
     return v;
 }
 
