@@ -28,6 +28,9 @@ METRIC = "He trajectories/s at 2k atoms (mixed vs FP64, ≤0.1% CCS); p50/p99 CC
 FLOP_PAIR, FLOP_EPI, FLOP_VERLET = 58, 21, 50   # algorithmic FLOPs, type-4 pair / epilogue / step
 PEAK_FP32 = 157.3          # TFLOP/s, MI355X FP32 vector (packed) peak
 PEAK_FP32_UNPACKED = 78.6  # TFLOP/s, FP32 vector without packed math (= FP64 vector)
+# HBM bytes per launch of the timed mixed kernel (rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0,
+# scale factors from a 256 MiB device copy), from profiles/traj_grp_mixed_ion2000.md
+TRAFFIC_BYTES = (114758 * 2 + 129265) * 1024
 
 
 def log(*a):
@@ -321,7 +324,7 @@ def main():
 
     fp64 = None
     if not a.no_fp64:
-        m.run(E.PREC_FP64, job=1, iters=1, traj=2000)   # tunes the FP64 launch shape outside the timing
+        m.run(E.PREC_FP64, job=1, iters=1, traj=cus * 1024)  # tunes the FP64 launch shape outside the timing
         E.device_sync()
         t1 = time.perf_counter()
         fp64 = m.run(E.PREC_FP64, job=0)
@@ -329,6 +332,7 @@ def main():
         log(f"fp64: {fp64_s * 1e3:.1f} ms, CCS {fp64['wmean']:.3f} +- {fp64['werr']:.3f}, "
             f"lanes {fp64['lanes']} wg {fp64['wg_size']}")
 
+    fma = E.device_fma_rate()  # the non-packed FP32 FMA ceiling of this build, measured
     # roofline of the dominant kernel (first trajectory launch of each step)
     kms = np.array([r["traj_kernel_ms"][0] for r in runs])
     flops = np.array([kernel_flops(inf["n"], r) for r in runs])
@@ -344,7 +348,9 @@ def main():
                    "precision": "mixed"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32, "frac_unpacked_fp32": achieved / PEAK_FP32_UNPACKED,
-                     "traffic": None, "kernel": "k_traj<mixed,type4>",
+                     "fma_rate_measured": fma, "frac_of_measured_fma": achieved / fma,
+                     "traffic": TRAFFIC_BYTES, "traffic_source": "profiles/traj_grp_mixed_ion2000.md",
+                     "kernel": f"k_traj_grp<mixed,type4,{mixed['lanes']}>",
                      "kernel_ms": float(kms.mean())},
         "ccs": {"mixed_wmean": mixed["wmean"], "mixed_werr": mixed["werr"]},
         "launch": {"lanes_per_trajectory": mixed["lanes"], "wg_size": mixed["wg_size"],

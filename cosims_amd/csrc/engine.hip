@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -444,6 +445,23 @@ __global__ void __launch_bounds__(256) k_potential(MolDev m, int64_t n, const do
     if (i < n) u[i] = potential(m, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
 }
 
+// FP32 FMA issue rate of the device as this build compiles it (packed FP32
+// off): 16 independent v_fma_f32 chains per lane, every result stored.
+__global__ void __launch_bounds__(256) k_fma_rate(float *out, int iters, float a, float b)
+{
+    float v[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) v[i] = threadIdx.x * 1e-3f + i;
+    for (int k = 0; k < iters; k++) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) v[i] = __builtin_fmaf(v[i], a, b);
+    }
+    float s = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) s += v[i];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
 // ------------------------------------------------- kernel selection ---
 using TrajFn = void (*)(TrajArgs);
 using AccFn = void (*)(MolDev, int64_t, const double *, double *);
@@ -658,9 +676,10 @@ struct cosims_mol {
     mutable size_t dbytes = 0;            // device bytes of the molecule
     mutable MolRec rec{};                 // table record of this molecule (device pointers)
     mutable const MolRec *rec_dev = nullptr;  // the same record in device memory
-    // autotuned launch shape per precision: lanes per trajectory, workgroup size
+    // autotuned launch shapes, keyed by precision and run-size bucket:
+    // lanes per trajectory, workgroup size
     mutable std::mutex tune_mu;
-    mutable int tuned_lanes[2] = {0, 0}, tuned_wg[2] = {0, 0};
+    mutable std::map<int, std::pair<int, int>> tuned;
     ~cosims_mol() { if (dmem) (void)hipFree(dmem); }
 };
 
@@ -840,6 +859,36 @@ const char *cosims_last_error(void) { return g_err.c_str(); }
 int cosims_device_sync(void)
 {
     HIP_CHECK(hipDeviceSynchronize());
+    return 0;
+}
+
+int cosims_device_fma_rate(double *tflops)
+{
+    if (!tflops) return fail(COSIMS_E_INVALID, "cosims_device_fma_rate: null output");
+    Device d;
+    if (int rc = device_init(d)) return rc;
+    const int blocks = d.cus * 32, threads = 256, iters = 20000;
+    DBuf buf;
+    HIP_CHECK(buf.alloc(sizeof(float) * blocks * threads));
+    Stream st;
+    HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    hipLaunchKernelGGL(k_fma_rate, dim3(blocks), dim3(threads), 0, st.s, buf.as<float>(), 100, 0.999f, 0.001f);
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; rep++) {
+        HIP_CHECK(hipEventRecord(e0, st.s));
+        hipLaunchKernelGGL(k_fma_rate, dim3(blocks), dim3(threads), 0, st.s, buf.as<float>(), iters, 0.999f, 0.001f);
+        HIP_CHECK(hipEventRecord(e1, st.s));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        best = std::min(best, ms);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *tflops = 2.0 * 16 * (double)iters * blocks * threads / (best * 1e-3) / 1e12;
     return 0;
 }
 
@@ -1077,6 +1126,8 @@ __global__ void k_copy_i32(const int32_t *src, int32_t *dst)
 {
     if (threadIdx.x == 0) *dst = *src;
 }
+
+
 
 // Runs the attempt rounds over the hit list in B.items[0] (count in
 // counters[0]).  Returns per-round stats.
@@ -1438,9 +1489,11 @@ void finish_ccs(const int8_t *status, const double *sums, int iters, int64_t T, 
 // Launch-shape search shared by the single-molecule and batch tuners:
 // every lanes-per-trajectory L (the atom tile each lane strides over) x
 // workgroup size W (W/L trajectories per workgroup) is timed on a small
-// item set; the three fastest are re-timed twice (the faster run counts:
-// launch times vary by a few per cent) on a set large enough that the
-// end-of-launch tail is amortised.  run(set, L, W, ms) times one launch.
+// item set; the best W of each of the three fastest L is re-timed twice
+// (the faster run counts: launch times vary by a few per cent) on the large
+// set, where the end-of-launch tail weighs as in the real run (fewer lanes
+// per trajectory mean slower trajectories, hence longer tails, which a small
+// set over-weights).  run(set, L, W, ms) times one launch.
 template <class RunFn>
 int tune_shapes(int units, RunFn run, int &L_out, int &W_out)
 {
@@ -1459,8 +1512,15 @@ int tune_shapes(int units, RunFn run, int &L_out, int &W_out)
             cands.push_back({L, W, ms});
         }
     }
+    // finalists: the best workgroup size of each of the three best lane counts
     std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
-    if (cands.size() > 3) cands.resize(3);
+    std::vector<Cand> fin;
+    for (const Cand &c : cands) {
+        bool seen = false;
+        for (const Cand &f : fin) seen |= f.L == c.L;
+        if (!seen && fin.size() < 3) fin.push_back(c);
+    }
+    cands.swap(fin);
     for (Cand &c : cands) {
         double a = 0, b = 0;
         if (int rc = run(1, c.L, c.W, a)) return rc;
@@ -1474,21 +1534,29 @@ int tune_shapes(int units, RunFn run, int &L_out, int &W_out)
 }
 
 // Single-molecule autotuner, timed on the molecule itself (fixed Philox
-// stream); the result is cached on the handle per precision.
-int autotune(const cosims_mol *M, int prec, int &L_out, int &wg_out)
+// stream).  The final stage runs on as many trajectories as the run it tunes
+// for (capped: 4096 per CU mixed, 1024 FP64), because the end-of-launch tail
+// weighs differently on small and large launches; results are cached on the
+// handle per precision and power-of-two size of that set.
+int autotune(const cosims_mol *M, int prec, int64_t n_hint, int &L_out, int &wg_out)
 {
     std::lock_guard<std::mutex> lk(M->tune_mu);
     const int pi = prec == COSIMS_PREC_FP64 ? 0 : 1;
-    if (M->tuned_lanes[pi]) {
-        L_out = M->tuned_lanes[pi];
-        wg_out = M->tuned_wg[pi];
+    const int64_t cap = (int64_t)M->dev.cus * (pi ? 4096 : 1024);
+    const int64_t n2 = std::max<int64_t>(16384, std::min<int64_t>(n_hint, cap));
+    int bucket = 0;
+    while ((int64_t(1) << (bucket + 1)) <= n2) bucket++;
+    const int key = 64 * pi + bucket;
+    auto it = M->tuned.find(key);
+    if (it != M->tuned.end()) {
+        L_out = it->second.first;
+        wg_out = it->second.second;
         return 0;
     }
     Stream st;
     HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
     // stage 1 holds several rounds even at 4 lanes per trajectory (16 trajectories per wavefront)
-    const int64_t nset[2] = {std::max<int64_t>(16384, (int64_t)M->dev.cus * 256),
-                             std::max<int64_t>(16384, (int64_t)M->dev.cus * 1024)};
+    const int64_t nset[2] = {std::min<int64_t>(n2, std::max<int64_t>(16384, (int64_t)M->dev.cus * 256)), n2};
     TrajBufs B;
     if (int rc = B.alloc(nset[1])) return rc;
     int cur = -1;
@@ -1507,8 +1575,7 @@ int autotune(const cosims_mol *M, int prec, int &L_out, int &wg_out)
         return 0;
     };
     if (int rc = tune_shapes(group_units(M->prep), run, L_out, wg_out)) return rc;
-    M->tuned_lanes[pi] = L_out;
-    M->tuned_wg[pi] = wg_out;
+    M->tuned[key] = {L_out, wg_out};
     return 0;
 }
 
@@ -1883,7 +1950,7 @@ int cosims_verlet(const cosims_mol *M, int32_t prec, int64_t n, const double *p0
     // to the reference; mixed precision runs the tuned group kernel.
     int L = 1, wg = 256;
     if (prec == COSIMS_PREC_MIXED)
-        if (int rc = autotune(M, prec, L, wg)) return rc;
+        if (int rc = autotune(M, prec, n, L, wg)) return rc;
     Stream st;
     HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
     TrajBufs B;
@@ -1943,7 +2010,7 @@ int cosims_run(const cosims_mol *M, const cosims_params *P, const cosims_run_cfg
     int L = c.lanes, wg = c.wg_size;
     if (L == 0) {
         int tl = 0, tw = 0;
-        if (int rc = autotune(M, prec, tl, tw)) return rc;
+        if (int rc = autotune(M, prec, n, tl, tw)) return rc;
         L = tl;
         if (wg <= 0) wg = tw;
     }

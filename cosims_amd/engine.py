@@ -26,7 +26,7 @@ EXPORTS = [
     "cosims_mol_atoms", "cosims_mol_clusters", "cosims_accel", "cosims_potential", "cosims_sample",
     "cosims_sample_reference",
     "cosims_verlet", "cosims_run", "cosims_run_batch", "cosims_tune_batch", "cosims_strerror",
-    "cosims_last_error", "cosims_device_info", "cosims_device_sync",
+    "cosims_last_error", "cosims_device_info", "cosims_device_sync", "cosims_device_fma_rate",
     "cosims_server_create", "cosims_server_submit", "cosims_server_wait", "cosims_server_stats_get",
     "cosims_server_destroy",
 ]
@@ -180,6 +180,7 @@ def lib():
         L.cosims_strerror.argtypes = [C.c_int]
         L.cosims_last_error.restype = C.c_char_p
         L.cosims_device_info.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_int32)]
+        L.cosims_device_fma_rate.argtypes = [C.POINTER(C.c_double)]
         _LIB = L
     return _LIB
 
@@ -234,6 +235,13 @@ def read_molfile(mol_path, atoms_path=None) -> dict:
 
 def device_sync():
     _check(lib().cosims_device_sync())
+
+
+def device_fma_rate() -> float:
+    """Measured FP32 FMA rate (TFLOP/s) of the device for code built like the engine."""
+    t = C.c_double()
+    _check(lib().cosims_device_fma_rate(C.byref(t)))
+    return t.value
 
 
 def device_info():

@@ -350,6 +350,10 @@ const char *cosims_strerror(int code);
 const char *cosims_last_error(void);
 /* Device name and compute-unit count of the current device. */
 int cosims_device_info(char *name, size_t cap, int32_t *cus);
+/* Measured FP32 FMA rate of the current device for code built as this
+ * library is (no packed FP32), in TFLOP/s: the vector ceiling of the mixed
+ * force sum. */
+int cosims_device_fma_rate(double *tflops);
 /* Waits for all work on the current device (benchmark brackets). */
 int cosims_device_sync(void);
 

@@ -88,3 +88,10 @@ def test_device_sqrt_div_correctly_rounded():
     mo = oracle.Mol(np.zeros((1, 3)), [0.7], [3.043 ** 6], [0.3], [0.5], 1, 3.99, 4)
     assert np.array_equal(m.potential(pts), mo.potential(pts))
     assert np.array_equal(m.accel(pts, E.PREC_FP64), mo.accel(pts))
+
+
+def test_device_fma_rate_is_plausible():
+    """The measured non-packed FP32 FMA ceiling lies between a quarter of and
+    the published packed peak (157.3 TFLOP/s on MI355X)."""
+    t = E.device_fma_rate()
+    assert 157.3 / 4 < t <= 157.3, t
