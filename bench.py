@@ -75,7 +75,7 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a, dtype=float), q)) if len(a) else None
 
 
-def serving(E, synth, n_requests, rate, target, seed=5000, policy=0):
+def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, **server_kw):
     """Poisson request stream through the CCS server; latency = completion -
     submission on the server's clock (setup, queueing and trajectories)."""
     import tempfile
@@ -93,7 +93,7 @@ def serving(E, synth, n_requests, rate, target, seed=5000, policy=0):
     W = 512
     log(f"serving: {n_requests} requests at {rate}/s, target {target}, lanes by size, wg {W}")
     srv = E.Server(prec=E.PREC_MIXED, lanes=0, wg_size=W, block_traj=10000, min_blocks=3, max_blocks=1000,
-                   policy=policy)
+                   policy=policy, **server_kw)
     t0 = time.perf_counter()
     tickets = []
     for k, r in enumerate(reqs):
@@ -120,6 +120,7 @@ def serving(E, synth, n_requests, rate, target, seed=5000, policy=0):
         "setup_ms_p50": pct([r["setup_ms"] for r in res], 50),
         "queue_ms_p50": pct([r["t_admit_ms"] - r["t_ready_ms"] for r in res], 50),
         "trajectories_per_s": traj / makespan, "trajectories": traj, "makespan_s": makespan,
+        "trajectories_issued": st["traj"],
         "rel_err_max": max(r["rel_err"] for r in res), "blocks_mean": float(np.mean([r["blocks"] for r in res])),
         "slices": st["slices"], "kernel_s": st["kernel_ms"] / 1e3, "busy_s": st["busy_ms"] / 1e3,
         "lanes_per_trajectory": "by size class (4 x 2^class)", "wg_size": W,
@@ -380,7 +381,7 @@ def main():
         if a.serve_compare:
             eq = serving(E, synth, a.serve_requests, a.serve_rate, a.serve_target, policy=E.SERVE_EQUAL_SHARE)
             out["serving_equal_share"] = {k: eq[k] for k in ("latency_ms", "served_rate", "trajectories_per_s",
-                                                             "makespan_s", "slices")}
+                                                             "makespan_s", "slices", "trajectories_issued")}
     print(json.dumps(out), flush=True)
 
 

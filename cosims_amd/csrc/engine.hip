@@ -548,25 +548,16 @@ struct Stream {
 }  // namespace
 
 // ------------------------------------------------- boundary marches ---
-struct MarchArgs {
-    MolDev m;          // atoms (a64, reference order), n, charged, dipole_scalar
-    const March *w;
-    double mk;
-    int64_t max_points;
-    MarchHit *out;
-};
-
 // One wavefront per march (impactParam::setBoundry, src/impactParam.cpp:
 // 602-736): 64 consecutive points per pass, lane j rebuilding p_{k+j} by the
 // same sequential FP64 additions as the host loop and evaluating getPotential
 // there in reference atom order (scalar atom loads shared by the wave); the
 // first lane that crosses (or passes the axis) ends the march.
-__global__ void __launch_bounds__(64) k_march(MarchArgs A)
+__device__ void march_wave(const MolDev &m, const March &w, double mk, int64_t max_points, MarchHit &out)
 {
-    const March &w = A.w[blockIdx.x];
     const int lane = threadIdx.x;
-    const double e0 = potential(A.m, w.start[0], w.start[1], w.start[2]);
-    const double sl = (e0 + A.mk < 0) ? -1.0 : 1.0;
+    const double e0 = potential(m, w.start[0], w.start[1], w.start[2]);
+    const double sl = (e0 + mk < 0) ? -1.0 : 1.0;
     const double vx = w.dir[0] * sl, vy = w.dir[1] * sl, vz = w.dir[2] * sl;
     const double sx = w.start[0], sy = w.start[1];
     double bx = w.start[0], by = w.start[1], bz = w.start[2];  // point kb
@@ -576,14 +567,14 @@ __global__ void __launch_bounds__(64) k_march(MarchArgs A)
         bx = bx + vx; by = by + vy; bz = bz + vz;
         kb = 1;
     }
-    for (; kb < A.max_points; kb += 64) {
+    for (; kb < max_points; kb += 64) {
         double px = bx, py = by, pz = bz;
         for (int q = 0; q < lane; q++) {
             px = px + vx; py = py + vy; pz = pz + vz;
         }
-        const double en = potential(A.m, px, py, pz);
-        const bool inr = kb + lane < A.max_points;
-        const bool cross = inr && (en + A.mk) * sl < 0;
+        const double en = potential(m, px, py, pz);
+        const bool inr = kb + lane < max_points;
+        const bool cross = inr && (en + mk) * sl < 0;
         const bool axis = inr && w.axis && ((sx == 0 && sy / py <= 0) || (sy == 0 && sx / px <= 0) ||
                                             (sx / px <= 0 && sy / py <= 0));
         const int src = (lane + 63) & 63;
@@ -592,47 +583,176 @@ __global__ void __launch_bounds__(64) k_march(MarchArgs A)
         if (mask) {
             const int k = __ffsll(mask) - 1;
             if (lane == k) {
-                MarchHit &h = A.out[blockIdx.x];
-                h.found = cross ? 1 : 0;
-                h.p[0] = px; h.p[1] = py; h.p[2] = pz;
-                h.prev[0] = k ? nx : qx; h.prev[1] = k ? ny : qy; h.prev[2] = k ? nz : qz;
+                out.found = cross ? 1 : 0;
+                out.p[0] = px; out.p[1] = py; out.p[2] = pz;
+                out.prev[0] = k ? nx : qx; out.prev[1] = k ? ny : qy; out.prev[2] = k ? nz : qz;
             }
             return;
         }
         qx = __shfl(px, 63); qy = __shfl(py, 63); qz = __shfl(pz, 63);
         bx = qx + vx; by = qy + vy; bz = qz + vz;
     }
-    if (lane == 0) A.out[blockIdx.x].found = -1;
+    if (lane == 0) out.found = -1;
+}
+
+struct MarchArgs {
+    MolDev m;          // atoms (a64, reference order), n, charged, dipole_scalar
+    const March *w;
+    double mk;
+    int64_t max_points;
+    MarchHit *out;
+};
+
+__global__ void __launch_bounds__(64) k_march(MarchArgs A)
+{
+    march_wave(A.m, A.w[blockIdx.x], A.mk, A.max_points, A.out[blockIdx.x]);
+}
+
+// The whole boundary search as one chain of launches: the z-axis walk, the
+// ray geometry it implies, and every ray (blocks past the ray count exit).
+constexpr int kChainMaxRays = 1024 * kBoundaryRot;
+struct ChainState {
+    MarchHit z;
+    RayGeometry g;
+    int32_t count;     // rays of the geometry; > kChainMaxRays: not run here
+};
+struct ChainArgs {
+    MolDev m;
+    BoundaryPlan bp;
+    double mk;
+    int64_t max_points;
+    ChainState *st;
+    MarchHit *hits;
+};
+
+__global__ void __launch_bounds__(64) k_chain_zwalk(ChainArgs A)
+{
+    // z axis: pos.z -= 0.5 * slope, i.e. the step (0, 0, -0.5) * slope
+    March w;
+    w.start[0] = 0; w.start[1] = 0; w.start[2] = A.bp.z0;
+    w.dir[0] = 0; w.dir[1] = 0; w.dir[2] = -0.5;
+    w.prev0[0] = 0; w.prev0[1] = 0; w.prev0[2] = A.bp.z0;
+    w.first = 1;
+    w.axis = 0;
+    march_wave(A.m, w, A.mk, A.max_points, A.st->z);
+}
+
+// ray_geometry (host) in the same IEEE operations
+__global__ void k_chain_geometry(ChainArgs A)
+{
+    if (threadIdx.x != 0) return;
+    ChainState &S = *A.st;
+    S.count = 0;
+    if (S.z.found != 1) return;
+    RayGeometry g;
+    double zMax = (S.z.p[2] < S.z.prev[2]) ? S.z.prev[2] : S.z.p[2];
+    zMax *= 1.3;
+    g.zmax = zMax;
+    g.posmag = A.bp.posmag0 + fabs(zMax - A.bp.maxz);
+    const int cz = (int)ceil(zMax);
+    g.levels = (int)ceil((double)(25 < cz ? cz : 25));
+    g.zinc = 2 * zMax / (g.levels);
+    S.g = g;
+    S.count = (g.levels + 1) * kBoundaryRot;
+}
+
+__global__ void __launch_bounds__(64) k_chain_rays(ChainArgs A)
+{
+    const ChainState &S = *A.st;
+    const int r = blockIdx.x;
+    if (S.count > kChainMaxRays || r >= S.count) return;
+    const int i = r / kBoundaryRot, n = r % kBoundaryRot;
+    // ray_march (host) in the same IEEE operations
+    March w;
+    w.start[0] = S.g.posmag * A.bp.cos_t[n];
+    w.start[1] = S.g.posmag * A.bp.sin_t[n];
+    w.start[2] = -S.g.zmax + i * S.g.zinc;
+    w.dir[0] = A.bp.dir_x[n]; w.dir[1] = A.bp.dir_y[n]; w.dir[2] = 0.0;
+    w.prev0[0] = 0; w.prev0[1] = 0; w.prev0[2] = 0;
+    w.first = 0;
+    w.axis = 1;
+    march_wave(A.m, w, A.mk, A.max_points, A.hits[r]);
 }
 
 // Boundary marches on the GPU for prepare(): the molecule's atoms are
-// uploaded once per setup, one k_march wavefront per march.
+// uploaded once, then the chain of launches runs with one wait at its end.
 struct GpuMarcher : BoundaryMarcher {
     void *atoms = nullptr;
     ~GpuMarcher() override
     {
         if (atoms) (void)hipFree(atoms);
     }
-    int run(const Prepared &p, double mk, const std::vector<March> &w, std::vector<MarchHit> &out,
-            std::string &err) override
+    int boundary(const Prepared &p, double mk, const BoundaryPlan &bp, MarchHit &zhit, std::vector<MarchHit> &rays,
+                 std::string &err) override
     {
-        int rc = run_(p, mk, w, out);
+        int rc = boundary_(p, mk, bp, zhit, rays);
         if (rc) err = g_err;
         return rc;
     }
-    int run_(const Prepared &p, double mk, const std::vector<March> &w, std::vector<MarchHit> &out)
+    MolDev mol_dev(const Prepared &p) const
     {
-        if (!atoms) {
-            Device d;
-            if (int rc = device_init(d)) return rc;
-            std::vector<AtomF64> a(p.n);
-            for (int i = 0; i < p.n; i++) {
-                a[i].x = p.xyz[3 * i]; a[i].y = p.xyz[3 * i + 1]; a[i].z = p.xyz[3 * i + 2];
-                a[i].q = p.q[i]; a[i].s6 = p.sig6[i]; a[i].e24 = p.eps24m[i]; a[i].e4 = p.eps4[i]; a[i].pad = 0;
-            }
-            HIP_CHECK(hipMalloc(&atoms, sizeof(AtomF64) * p.n));
-            HIP_CHECK(hipMemcpy(atoms, a.data(), sizeof(AtomF64) * p.n, hipMemcpyHostToDevice));
+        MolDev m{};
+        m.n = p.n;
+        m.charged = p.charged ? 1 : 0;
+        m.dipole_scalar = p.dipole_scalar;
+        m.a64 = (const AtomF64 *)atoms;
+        return m;
+    }
+    int upload_atoms(const Prepared &p)
+    {
+        if (atoms) return 0;
+        Device d;
+        if (int rc = device_init(d)) return rc;
+        std::vector<AtomF64> a(p.n);
+        for (int i = 0; i < p.n; i++) {
+            a[i].x = p.xyz[3 * i]; a[i].y = p.xyz[3 * i + 1]; a[i].z = p.xyz[3 * i + 2];
+            a[i].q = p.q[i]; a[i].s6 = p.sig6[i]; a[i].e24 = p.eps24m[i]; a[i].e4 = p.eps4[i]; a[i].pad = 0;
         }
+        HIP_CHECK(hipMalloc(&atoms, sizeof(AtomF64) * p.n));
+        HIP_CHECK(hipMemcpy(atoms, a.data(), sizeof(AtomF64) * p.n, hipMemcpyHostToDevice));
+        return 0;
+    }
+    int boundary_(const Prepared &p, double mk, const BoundaryPlan &bp, MarchHit &zhit, std::vector<MarchHit> &rays)
+    {
+        if (int rc = upload_atoms(p)) return rc;
+        Stream st;
+        HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+        DBuf dst, dh;
+        HIP_CHECK(dst.alloc(sizeof(ChainState)));
+        HIP_CHECK(dh.alloc(sizeof(MarchHit) * kChainMaxRays));
+        ChainArgs A{};
+        A.m = mol_dev(p);
+        A.bp = bp;
+        A.mk = mk;
+        A.max_points = kMaxMarchPoints;
+        A.st = dst.as<ChainState>();
+        A.hits = dh.as<MarchHit>();
+        hipLaunchKernelGGL(k_chain_zwalk, dim3(1), dim3(64), 0, st.s, A);
+        hipLaunchKernelGGL(k_chain_geometry, dim3(1), dim3(64), 0, st.s, A);
+        hipLaunchKernelGGL(k_chain_rays, dim3(kChainMaxRays), dim3(64), 0, st.s, A);
+        HIP_CHECK(hipGetLastError());
+        ChainState S{};
+        HIP_CHECK(hipMemcpyAsync(&S, dst.p, sizeof(S), hipMemcpyDeviceToHost, st.s));
+        rays.resize(kChainMaxRays);
+        HIP_CHECK(hipMemcpyAsync(rays.data(), dh.p, sizeof(MarchHit) * kChainMaxRays, hipMemcpyDeviceToHost, st.s));
+        HIP_CHECK(hipStreamSynchronize(st.s));
+        zhit = S.z;
+        if (S.z.found != 1) {
+            rays.clear();
+            return 0;
+        }
+        if (S.count <= kChainMaxRays) {
+            rays.resize(S.count);
+            return 0;
+        }
+        // more rays than the chain runs: the same marches from a host-built list
+        std::vector<March> w((size_t)S.count);
+        for (int i = 0; i <= S.g.levels; i++)
+            for (int n = 0; n < kBoundaryRot; n++) w[(size_t)i * kBoundaryRot + n] = ray_march(bp, S.g, i, n);
+        return run_list(p, mk, w, rays);
+    }
+    int run_list(const Prepared &p, double mk, const std::vector<March> &w, std::vector<MarchHit> &out)
+    {
         Stream st;
         HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
         DBuf dw, dh;
@@ -640,10 +760,7 @@ struct GpuMarcher : BoundaryMarcher {
         HIP_CHECK(dh.alloc(sizeof(MarchHit) * w.size()));
         HIP_CHECK(hipMemcpyAsync(dw.p, w.data(), sizeof(March) * w.size(), hipMemcpyHostToDevice, st.s));
         MarchArgs A{};
-        A.m.n = p.n;
-        A.m.charged = p.charged ? 1 : 0;
-        A.m.dipole_scalar = p.dipole_scalar;
-        A.m.a64 = (const AtomF64 *)atoms;
+        A.m = mol_dev(p);
         A.w = dw.as<March>();
         A.mk = mk;
         A.max_points = kMaxMarchPoints;
@@ -1347,7 +1464,7 @@ int timed_group_launch(void (*fn)(Args), const Args &args, int64_t grid, int wg,
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(e1, s));
     hipLaunchKernelGGL(k_chi, dim3((unsigned)((n_items + 255) / 256)), dim3(256), 0, s, args.io.items, n_items,
-                       args.io.status, args.io.cchi, args.io.chi);
+                       args.io.status, args.io.cchi, args.io.chi, (const int32_t *)nullptr, (const int32_t *)nullptr);
     HIP_CHECK(hipGetLastError());
     unsigned long long ev[4];
     HIP_CHECK(hipMemcpyAsync(ev, B.evals.p, sizeof(ev), hipMemcpyDeviceToHost, s));
@@ -1622,7 +1739,7 @@ struct BatchPlan {
     std::vector<int> order;              // job indices in work-list order
     struct Seg {
         int ft, L, W;
-        int64_t begin, end;              // work-list range of this launch
+        int64_t reserved, traj;          // trajectories of this launch (its hits: on the device, seg_end)
         double work;                     // trajectories x units, for the launch order
     };
     std::vector<Seg> segs;
@@ -1632,12 +1749,19 @@ struct BatchPlan {
     DBuf tmol, table, seg_ctr, seg_evals, seg_end;
     int64_t cap = 0, table_cap = 0;      // allocated trajectories / table entries (reused)
     float sample_ms = 0;
+    std::vector<MolRec> recs;            // host side of the table upload
     std::vector<hipStream_t> streams;    // default priority, made on first use
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // first launch start / last launch end
+    hipEvent_t es0 = nullptr, es1 = nullptr;  // sampling start / end
+    std::vector<hipEvent_t> joins;
     BatchPlan() = default;
     BatchPlan(const BatchPlan &) = delete;
     BatchPlan &operator=(const BatchPlan &) = delete;
     ~BatchPlan()
     {
+        for (hipEvent_t j : joins) (void)hipEventDestroy(j);
+        for (hipEvent_t e : {ev0, ev1, es0, es1})
+            if (e) (void)hipEventDestroy(e);
         for (hipStream_t x : streams) (void)hipStreamDestroy(x);
     }
 };
@@ -1726,18 +1850,20 @@ int stage_batch(BatchPlan &P, hipStream_t s)
         P.table_cap = nj;
     }
     if (!P.seg_end.p) HIP_CHECK(P.seg_end.alloc(sizeof(int32_t) * kMaxSegs));
-    std::vector<MolRec> recs(nj);
-    for (int k = 0; k < nj; k++) recs[k] = P.jobs[k].mol->rec;
-    HIP_CHECK(hipMemcpyAsync(P.table.p, recs.data(), sizeof(MolRec) * nj, hipMemcpyHostToDevice, s));
+    // the host copy lives on the plan: the upload is asynchronous
+    P.recs.resize(nj);
+    for (int k = 0; k < nj; k++) P.recs[k] = P.jobs[k].mol->rec;
+    HIP_CHECK(hipMemcpyAsync(P.table.p, P.recs.data(), sizeof(MolRec) * nj, hipMemcpyHostToDevice, s));
     int32_t *ctr = P.B.counters.as<int32_t>();
+    if (!P.es0) {
+        HIP_CHECK(hipEventCreate(&P.es0));
+        HIP_CHECK(hipEventCreate(&P.es1));
+    }
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ctr, 64, 0);
-    hipEvent_t e0, e1;
-    HIP_CHECK(hipEventCreate(&e0));
-    HIP_CHECK(hipEventCreate(&e1));
-    HIP_CHECK(hipEventRecord(e0, s));
+    HIP_CHECK(hipEventRecord(P.es0, s));
     P.segs.clear();
-    std::vector<double> work;
     double seg_work = 0;
+    int64_t seg_traj = 0;
     for (size_t i = 0; i < P.order.size(); i++) {
         const int k = P.order[i];
         const BatchPlan::Job &J = P.jobs[k];
@@ -1751,35 +1877,30 @@ int stage_batch(BatchPlan &P, hipStream_t s)
         hipLaunchKernelGGL(k_sample, dim3(nb), dim3(256), 0, s, A);
         HIP_CHECK(hipGetLastError());
         seg_work += (double)J.n * J.units;
+        seg_traj += J.n;
         const bool last = i + 1 == P.order.size();
         const BatchPlan::Job *nx = last ? nullptr : &P.jobs[P.order[i + 1]];
         if (last || nx->ft != J.ft || nx->L != J.L || nx->W != J.W) {
-            // the launch's work list ends where the item counter stands now
+            // the launch's work list ends where the item counter stands now;
+            // the bound stays on the device (no host wait here)
             if ((int)P.segs.size() == kMaxSegs) return fail(COSIMS_E_INVALID, "batch: too many distinct launch shapes");
             hipLaunchKernelGGL(k_copy_i32, dim3(1), dim3(1), 0, s, ctr, P.seg_end.as<int32_t>() + P.segs.size());
-            P.segs.push_back({J.ft, J.L, J.W, 0, 0, seg_work});
+            P.segs.push_back({J.ft, J.L, J.W, 0, seg_traj, seg_work});
             seg_work = 0;
+            seg_traj = 0;
         }
     }
     HIP_CHECK(hipGetLastError());
-    std::vector<int32_t> ends(P.segs.size());
-    HIP_CHECK(hipMemcpyAsync(ends.data(), P.seg_end.p, sizeof(int32_t) * ends.size(), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipEventRecord(e1, s));
-    HIP_CHECK(hipEventSynchronize(e1));
-    HIP_CHECK(hipEventElapsedTime(&P.sample_ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    for (size_t g = 0; g < P.segs.size(); g++) {
-        P.segs[g].begin = g ? ends[g - 1] : 0;
-        P.segs[g].end = ends[g];
-    }
+    HIP_CHECK(hipEventRecord(P.es1, s));
     return 0;
 }
 
 // The launches of a staged plan, spread over kSegStreams streams so that
-// one launch's tail overlaps the next; the biggest launch goes first.  tot.ms
-// is the time from the first launch to the last one's end.
-int launch_batch(BatchPlan &P, int prec, int bpc, hipStream_t s, GroupRun &tot, int &launches)
+// one launch's tail overlaps the next; the biggest launch goes first.
+// launch_enqueue only enqueues (the stream s carries the join; kernels run
+// on the plan's own streams); launch_finish waits for them and adds their
+// time (first launch to last one's end) and counters to tot.
+int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches)
 {
     launches = 0;
     const int nseg = (int)P.segs.size();
@@ -1792,60 +1913,63 @@ int launch_batch(BatchPlan &P, int prec, int bpc, hipStream_t s, GroupRun &tot, 
         HIP_CHECK(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
         P.streams.push_back(x);
     }
+    if (!P.ev0) {
+        HIP_CHECK(hipEventCreate(&P.ev0));
+        HIP_CHECK(hipEventCreate(&P.ev1));
+    }
+    for (hipEvent_t j : P.joins) (void)hipEventDestroy(j);
+    P.joins.clear();
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, P.seg_ctr.as<int32_t>(), kMaxSegs, 0);
     hipLaunchKernelGGL(k_fill_i32, dim3(8), dim3(64), 0, s, P.seg_evals.as<int32_t>(), 8 * kMaxSegs, 0);
     std::vector<int> order(nseg);
     for (int i = 0; i < nseg; i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return P.segs[a].work > P.segs[b].work; });
-    hipEvent_t e0, e1;
-    HIP_CHECK(hipEventCreate(&e0));
-    HIP_CHECK(hipEventCreate(&e1));
-    std::vector<hipEvent_t> joins;
-    HIP_CHECK(hipEventRecord(e0, s));
-    int rc = 0;
-    for (int q = 0; q < nseg && !rc; q++) {
+    HIP_CHECK(hipEventRecord(P.ev0, s));
+    for (int q = 0; q < nseg; q++) {
         const int i = order[q];
         const auto &g = P.segs[i];
-        const int cnt = (int)(g.end - g.begin);
+        const int cnt = (int)g.traj;  // upper bound of the launch's work list (its hits)
         if (cnt <= 0) continue;
         MultiFn fn = multi_kernel(prec, g.ft, g.L);
-        if (!fn) {
-            rc = fail(COSIMS_E_INVALID, "no table kernel for this force type / lane count");
-            break;
-        }
+        if (!fn) return fail(COSIMS_E_INVALID, "no table kernel for this force type / lane count");
         hipStream_t x = P.streams[launches % P.streams.size()];
-        HIP_CHECK(hipStreamWaitEvent(x, e0, 0));
+        HIP_CHECK(hipStreamWaitEvent(x, P.ev0, 0));
         MultiArgs G{};
-        G.io = traj_io(P.B, P.B.items[0].as<int32_t>() + g.begin, cnt);
+        G.io = traj_io(P.B, P.B.items[0].as<int32_t>(), cnt);
         G.io.counter = P.seg_ctr.as<int32_t>() + i;
         G.io.evals = P.seg_evals.as<unsigned long long>() + 4 * i;
         G.mols = P.table.as<MolRec>();
         G.tmol = P.tmol.as<int32_t>();
+        G.first = i ? P.seg_end.as<int32_t>() + i - 1 : nullptr;
+        G.last = P.seg_end.as<int32_t>() + i;
         const size_t lds = multi_lds_bytes(g.ft, g.L, g.W);
         const int64_t grid = group_grid((const void *)fn, g.W, lds, bpc, P.cus, cnt, g.L);
         hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(g.W), lds, x, G);
         HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_chi, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, x, G.io.items, cnt, G.io.status,
-                           G.io.cchi, G.io.chi);
+                           G.io.cchi, G.io.chi, G.first, G.last);
         HIP_CHECK(hipGetLastError());
         hipEvent_t j;
         HIP_CHECK(hipEventCreateWithFlags(&j, hipEventDisableTiming));
-        joins.push_back(j);
+        P.joins.push_back(j);
         HIP_CHECK(hipEventRecord(j, x));
         HIP_CHECK(hipStreamWaitEvent(s, j, 0));
         launches++;
     }
-    HIP_CHECK(hipEventRecord(e1, s));
+    HIP_CHECK(hipEventRecord(P.ev1, s));
+    return 0;
+}
+
+int launch_finish(BatchPlan &P, hipStream_t s, GroupRun &tot)
+{
+    const int nseg = (int)P.segs.size();
     std::vector<unsigned long long> ev(4 * kMaxSegs);
     HIP_CHECK(hipMemcpyAsync(ev.data(), P.seg_evals.p, sizeof(unsigned long long) * ev.size(), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     float ms = 0;
-    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-    for (hipEvent_t j : joins) (void)hipEventDestroy(j);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (rc) return rc;
+    HIP_CHECK(hipEventElapsedTime(&ms, P.ev0, P.ev1));
     tot.ms += ms;
+    HIP_CHECK(hipEventElapsedTime(&P.sample_ms, P.es0, P.es1));
     for (int i = 0; i < nseg; i++) {
         tot.steps += (int64_t)ev[4 * i];
         tot.evals += (int64_t)ev[4 * i + 1];
@@ -1853,6 +1977,15 @@ int launch_batch(BatchPlan &P, int prec, int bpc, hipStream_t s, GroupRun &tot, 
         tot.pairs += (int64_t)ev[4 * i + 3];
     }
     return 0;
+}
+
+int launch_batch(BatchPlan &P, int prec, int bpc, hipStream_t s, GroupRun &tot, int &launches)
+{
+    if (int rc = launch_enqueue(P, prec, bpc, s, launches)) {
+        (void)hipStreamSynchronize(s);
+        return rc;
+    }
+    return launch_finish(P, s, tot);
 }
 
 // Size-class autotuner of a batch: for every size class present, the

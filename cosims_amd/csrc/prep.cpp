@@ -413,6 +413,35 @@ void march_all(const Prepared &m, double mk, const std::vector<March> &w, std::v
     for (auto &th : pool) th.join();
 }
 
+}  // namespace
+
+RayGeometry ray_geometry(const BoundaryPlan &bp, const MarchHit &zhit)
+{
+    RayGeometry g;
+    double zMax = std::max(zhit.p[2], zhit.prev[2]);
+    zMax *= 1.3;
+    g.zmax = zMax;
+    g.posmag = bp.posmag0 + std::fabs(zMax - bp.maxz);
+    g.levels = (int)std::ceil((double)std::max(25, (int)std::ceil(zMax)));
+    g.zinc = 2 * zMax / (g.levels);
+    return g;
+}
+
+March ray_march(const BoundaryPlan &bp, const RayGeometry &g, int level, int n)
+{
+    March w;
+    w.start[0] = g.posmag * bp.cos_t[n];
+    w.start[1] = g.posmag * bp.sin_t[n];
+    w.start[2] = -g.zmax + level * g.zinc;
+    w.dir[0] = bp.dir_x[n]; w.dir[1] = bp.dir_y[n]; w.dir[2] = 0.0;
+    w.prev0[0] = w.prev0[1] = w.prev0[2] = 0;
+    w.first = 0;
+    w.axis = 1;
+    return w;
+}
+
+namespace {
+
 // Boundary points of impactParam::setBoundry (src/impactParam.cpp:602-736):
 // a march down the z axis in 0.5 A steps finds zMax; then (levels+1) x 20
 // rays at heights -zMax..zMax march towards the z axis in 0.1 A steps, each
@@ -420,44 +449,39 @@ void march_all(const Prepared &m, double mk, const std::vector<March> &w, std::v
 int boundary_points(const Prepared &m, double min_kin, int threads, BoundaryMarcher *bm, std::vector<V3> &out,
                     std::string &err)
 {
-    const int numRot = 20;
     const double velMag = 0.1;
-    double posMag = std::fabs(std::max(std::fabs(m.max_xyz[0]), std::fabs(m.max_xyz[1])));
     const double mk = 0.01 * min_kin;
-    auto run = [&](const std::vector<March> &w, std::vector<MarchHit> &h) -> int {
-        if (bm) return bm->run(m, mk, w, h, err);
-        march_all(m, mk, w, h, threads);
-        return 0;
-    };
-    // z axis: pos.z -= 0.5 * slope, i.e. the step (0, 0, -0.5) * slope
-    std::vector<March> zw(1);
-    zw[0] = March{{0, 0, std::fabs(m.max_xyz[2]) + 20}, {0, 0, -0.5}, {0, 0, 0}, 1, 0};
-    std::memcpy(zw[0].prev0, zw[0].start, sizeof(zw[0].start));
-    std::vector<MarchHit> zh;
-    if (int rc = run(zw, zh)) return rc;
-    if (zh[0].found != 1) {
+    BoundaryPlan bp;
+    bp.z0 = std::fabs(m.max_xyz[2]) + 20;
+    bp.posmag0 = std::fabs(std::max(std::fabs(m.max_xyz[0]), std::fabs(m.max_xyz[1])));
+    bp.maxz = std::fabs(m.max_xyz[2]);
+    for (int n = 0; n < kBoundaryRot; n++) {
+        double theta = n * 2 * kPi / ((double)kBoundaryRot);
+        bp.cos_t[n] = std::cos(theta);
+        bp.sin_t[n] = std::sin(theta);
+        bp.dir_x[n] = -velMag * std::cos(theta);
+        bp.dir_y[n] = -velMag * std::sin(theta);
+    }
+    MarchHit zh{};
+    std::vector<MarchHit> hits;
+    if (bm) {
+        if (int rc = bm->boundary(m, mk, bp, zh, hits, err)) return rc;
+    } else {
+        // z axis: pos.z -= 0.5 * slope, i.e. the step (0, 0, -0.5) * slope
+        March zw{{0, 0, bp.z0}, {0, 0, -0.5}, {0, 0, bp.z0}, 1, 0};
+        zh = march_host(m, mk, zw);
+        if (zh.found == 1) {
+            const RayGeometry g = ray_geometry(bp, zh);
+            std::vector<March> rays((size_t)(g.levels + 1) * kBoundaryRot);
+            for (int i = 0; i <= g.levels; i++)
+                for (int n = 0; n < kBoundaryRot; n++) rays[(size_t)i * kBoundaryRot + n] = ray_march(bp, g, i, n);
+            march_all(m, mk, rays, hits, threads);
+        }
+    }
+    if (zh.found != 1) {
         err = "boundary search: no potential crossing on the z axis";
         return COSIMS_E_SETUP;
     }
-    double zMax = std::max(zh[0].p[2], zh[0].prev[2]);
-    zMax *= 1.3;
-    posMag += std::fabs(zMax - std::fabs(m.max_xyz[2]));
-    int levels = (int)std::ceil((double)std::max(25, (int)std::ceil(zMax)));
-    double zInc = 2 * zMax / (levels);
-    std::vector<March> rays((size_t)(levels + 1) * numRot);
-    for (int i = 0; i <= levels; i++)
-        for (int n = 0; n < numRot; n++) {
-            double zPos = -zMax + i * zInc;
-            double theta = n * 2 * kPi / ((double)numRot);
-            March &w = rays[(size_t)i * numRot + n];
-            w.start[0] = posMag * std::cos(theta); w.start[1] = posMag * std::sin(theta); w.start[2] = zPos;
-            w.dir[0] = -velMag * std::cos(theta); w.dir[1] = -velMag * std::sin(theta); w.dir[2] = 0.0;
-            w.prev0[0] = w.prev0[1] = w.prev0[2] = 0;
-            w.first = 0;
-            w.axis = 1;
-        }
-    std::vector<MarchHit> hits;
-    if (int rc = run(rays, hits)) return rc;
     out.clear();
     for (const MarchHit &h : hits) {
         if (h.found < 0) {

@@ -101,13 +101,33 @@ struct MarchHit {
 };
 constexpr int64_t kMaxMarchPoints = int64_t(1) << 22;
 
-// Evaluates marches somewhere other than the host loop (the GPU).  Must give
-// the same points as the host loop: the same FP64 additions and the
-// reference-order getPotential.
+// Inputs of the boundary search fixed before any march: the z-axis walk's
+// start, and what the rays' geometry needs once zMax is known
+// (src/impactParam.cpp:602-665).
+constexpr int kBoundaryRot = 20;
+struct BoundaryPlan {
+    double z0;                          // z-axis walk start, |maxZ| + 20
+    double posmag0;                     // |max(|maxX|, |maxY|)|
+    double maxz;                        // |maxZ|
+    double cos_t[kBoundaryRot], sin_t[kBoundaryRot];      // cos, sin of n 2 pi / 20
+    double dir_x[kBoundaryRot], dir_y[kBoundaryRot];      // -0.1 cos, -0.1 sin
+};
+// Ray geometry from the z-axis crossing (src/impactParam.cpp:632-665).
+struct RayGeometry {
+    double zmax, posmag, zinc;
+    int levels;
+};
+RayGeometry ray_geometry(const BoundaryPlan &bp, const MarchHit &zhit);
+March ray_march(const BoundaryPlan &bp, const RayGeometry &g, int level, int n);
+
+// Evaluates the boundary marches somewhere other than the host loop (the
+// GPU).  Must give the same points as the host loop: the same FP64
+// additions and the reference-order getPotential.  `boundary` runs the
+// z-axis walk and every ray (levels + 1) x 20 of the geometry it implies.
 struct BoundaryMarcher {
     virtual ~BoundaryMarcher() = default;
-    virtual int run(const Prepared &m, double mk, const std::vector<March> &w, std::vector<MarchHit> &out,
-                    std::string &err) = 0;
+    virtual int boundary(const Prepared &m, double mk, const BoundaryPlan &bp, MarchHit &zhit,
+                         std::vector<MarchHit> &rays, std::string &err) = 0;
 };
 
 // Full setup from raw atoms.  Returns 0, or a negative engine error code with

@@ -125,23 +125,23 @@ void setup_worker(cosims_server *S)
     }
 }
 
-// Blocks job J still needs: min_blocks first, then as many as reach the
-// target if the standard error keeps falling as 1/sqrt(blocks); at least one.
+// Blocks job J still needs beyond those already issued: min_blocks first,
+// then as many as reach the target if the standard error keeps falling as
+// 1/sqrt(blocks).  0 while the issued blocks may suffice.
 int blocks_wanted(const cosims_server *S, const SJob &J)
 {
     const auto &c = S->cfg;
-    int want = 1;
+    int want;
     if (J.used < c.min_blocks) {
-        want = c.min_blocks - J.used;
-    } else if (!J.omega.empty()) {
+        want = c.min_blocks - J.issued;
+    } else {
         cosims_ccs tmp{};
         final_stats(J.omega.data(), J.omega_err.data(), J.sums.data(), J.used, c.block_traj, &tmp);
         const double rel = tmp.werr / tmp.wmean;
         const double need = J.used * (rel / J.target) * (rel / J.target);
-        want = std::max(1, (int)std::ceil(need) - J.used);
+        want = std::max(J.issued == J.used ? 1 : 0, (int)std::ceil(need) - J.issued);
     }
-    // an active job always has blocks left: used == issued < max_blocks until it finishes
-    return std::max(1, std::min(want, c.max_blocks - J.issued));
+    return std::max(0, std::min(want, c.max_blocks - J.issued));
 }
 
 // Work of one block of J in force-loop units (integration steps x atoms, or
@@ -162,73 +162,92 @@ double block_cost(const cosims_server *S, const SJob &J)
 }
 
 struct SliceEntry {
-    size_t k;   // index into the active list
-    int nb;     // blocks of this slice
+    std::shared_ptr<SJob> job;  // keeps the molecule alive while the slice is in flight
+    int it0, nb;                // first block and block count of this slice
 };
 
 // Slice composition.  Shortest remaining work first: requests in order of
 // blocks wanted x block cost take blocks until the slice's work budget
-// (slice_ms at the measured throughput) is spent; the first always gets at
-// least one block.  Equal share: every request gets the same number of
-// blocks, up to slice_traj trajectories in all.
+// (slice_ms at the measured throughput) is spent, then the slice is topped
+// up to min_slice_traj trajectories in the same order; the first request
+// always gets its block.  Equal share: every request gets the same number
+// of blocks, up to slice_traj trajectories in all.  Requests already
+// holding every block they may need get none.
 std::vector<SliceEntry> plan_slice(cosims_server *S, const std::vector<std::shared_ptr<SJob>> &act)
 {
     const auto &c = S->cfg;
     std::vector<SliceEntry> out;
-    std::vector<int> want(act.size());
+    std::vector<int> want(act.size()), nb(act.size(), 0);
     std::vector<double> cost(act.size());
+    std::vector<size_t> order;
     for (size_t k = 0; k < act.size(); k++) {
         want[k] = blocks_wanted(S, *act[k]);
+        // the estimate of what is still needed is noisy until many blocks are
+        // in: issue at most half of it (rounded up) per slice, so blocks past
+        // the stopping point stay few
+        if (act[k]->used >= c.min_blocks) want[k] = (want[k] + 1) / 2;
         cost[k] = block_cost(S, *act[k]);
+        if (want[k] > 0) order.push_back(k);
     }
-    int64_t traj_left = std::max<int64_t>(c.slice_traj, c.block_traj);
+    if (order.empty()) return out;
     if (c.policy == COSIMS_SERVE_EQUAL_SHARE) {
-        const int64_t share = std::max<int64_t>(1, c.slice_traj / ((int64_t)act.size() * c.block_traj));
-        for (size_t k = 0; k < act.size(); k++) out.push_back({k, (int)std::min<int64_t>(want[k], share)});
-        return out;
-    }
-    std::vector<size_t> order(act.size());
-    for (size_t k = 0; k < order.size(); k++) order[k] = k;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](size_t a, size_t b) { return want[a] * cost[a] < want[b] * cost[b]; });
-    double budget = c.slice_ms * S->units_per_ms;
-    std::vector<int> nb(act.size(), 0);
-    for (size_t k : order) {
-        int64_t b = std::min<int64_t>(want[k], (int64_t)(budget / cost[k]));
-        b = std::min<int64_t>(b, traj_left / c.block_traj);
-        if (b <= 0 && traj_left == std::max<int64_t>(c.slice_traj, c.block_traj)) b = 1;  // the first always runs
-        if (b <= 0) continue;
-        nb[k] = (int)b;
-        budget -= b * cost[k];
-        traj_left -= b * c.block_traj;
-    }
-    // fill: a slice with too few trajectories leaves the launch tail-bound;
-    // top up, still in order and within what each request wants
-    int64_t fill = c.min_slice_traj;
-    for (size_t k = 0; k < act.size(); k++) fill -= (int64_t)nb[k] * c.block_traj;
-    for (size_t k : order) {
-        if (fill <= 0) break;
-        const int64_t add = std::min<int64_t>(want[k] - nb[k], (fill + c.block_traj - 1) / c.block_traj);
-        if (add <= 0) continue;
-        nb[k] += (int)add;
-        fill -= add * c.block_traj;
+        const int64_t share = std::max<int64_t>(1, c.slice_traj / ((int64_t)order.size() * c.block_traj));
+        for (size_t k : order) nb[k] = (int)std::min<int64_t>(want[k], share);
+    } else {
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t a, size_t b) { return want[a] * cost[a] < want[b] * cost[b]; });
+        double budget = c.slice_ms * S->units_per_ms;
+        int64_t traj_left = std::max<int64_t>(c.slice_traj, c.block_traj);
+        bool first = true;
+        for (size_t k : order) {
+            int64_t b = std::min<int64_t>(want[k], (int64_t)(budget / cost[k]));
+            b = std::min<int64_t>(b, traj_left / c.block_traj);
+            if (b <= 0 && first) b = 1;
+            first = false;
+            if (b <= 0) continue;
+            nb[k] = (int)b;
+            budget -= b * cost[k];
+            traj_left -= b * c.block_traj;
+        }
+        // fill: a slice with too few trajectories leaves its launches tail-bound
+        int64_t fill = c.min_slice_traj;
+        for (size_t k : order) fill -= (int64_t)nb[k] * c.block_traj;
+        for (size_t k : order) {
+            if (fill <= 0) break;
+            const int64_t add = std::min<int64_t>(want[k] - nb[k], (fill + c.block_traj - 1) / c.block_traj);
+            if (add <= 0) continue;
+            nb[k] += (int)add;
+            fill -= add * c.block_traj;
+        }
     }
     for (size_t k : order)
-        if (nb[k] > 0) out.push_back({k, nb[k]});
+        if (nb[k] > 0) out.push_back({act[k], act[k]->issued, nb[k]});
     return out;
 }
 
-// One slice over the active jobs; returns 0 or an error code (then every
-// active job fails with it).
-int run_slice(cosims_server *S, std::vector<std::shared_ptr<SJob>> &act, BatchPlan &P, hipStream_t s)
+// A slice in flight: its plan, device buffers and stream.
+struct Slice {
+    std::vector<SliceEntry> e;
+    BatchPlan *P = nullptr;
+    hipStream_t s = nullptr;
+    int launches = 0;
+    double t_issue = 0;
+    bool live() const { return !e.empty(); }
+};
+
+// Issues one slice (sampling, launches, per-block reduction into each
+// request's device sums) without waiting for the launches.
+int issue_slice(cosims_server *S, Slice &X)
 {
     const auto &c = S->cfg;
-    const std::vector<SliceEntry> plan = plan_slice(S, act);
+    BatchPlan &P = *X.P;
     P.jobs.clear();
     P.n = 0;
-    for (const SliceEntry &e : plan) {
-        SJob &J = *act[e.k];
-        if (J.issued + e.nb > c.max_blocks) return fail(COSIMS_E_INVALID, "server: block count beyond max_blocks");
+    for (SliceEntry &e : X.e) {
+        SJob &J = *e.job;
+        if (e.it0 + e.nb > c.max_blocks) return fail(COSIMS_E_INVALID, "server: block count beyond max_blocks");
+        if (!J.dsums.p && J.dsums.alloc(sizeof(double) * kSums * c.max_blocks) != hipSuccess)
+            return fail(COSIMS_E_NOMEM, "server: per-request sums");
         BatchPlan::Job b{};
         b.mol = J.mol;
         b.seed = J.params.seed;
@@ -239,52 +258,62 @@ int run_slice(cosims_server *S, std::vector<std::shared_ptr<SJob>> &act, BatchPl
         b.off = P.n;
         b.ft = J.mol->prep.force_type;
         b.units = group_units(J.mol->prep);
-        b.it0 = (uint32_t)J.issued;
+        b.it0 = (uint32_t)e.it0;
         b.L = c.lanes > 0 ? c.lanes : lanes_rule(b.units);
         b.W = std::max(c.wg_size, b.L);
         P.n += b.n;
         P.jobs.push_back(b);
+        J.issued += e.nb;
     }
     if (P.n > INT32_MAX / 4) return fail(COSIMS_E_INVALID, "server: slice too large");
     if (int rc = order_plan(P)) return rc;
-    if (int rc = stage_batch(P, s)) return rc;
-    GroupRun R;
-    int launches = 0;
-    if (int rc = launch_batch(P, c.prec, 0, s, R, launches)) return rc;
-    for (size_t i = 0; i < plan.size(); i++) {
+    if (int rc = stage_batch(P, X.s)) return rc;
+    if (int rc = launch_enqueue(P, c.prec, 0, X.s, X.launches)) return rc;
+    for (size_t i = 0; i < X.e.size(); i++) {
         const auto &J = P.jobs[i];
-        SJob &Q = *act[plan[i].k];
         ReduceArgs RA{};
         RA.status = P.B.status.as<int8_t>() + J.off; RA.chi = P.B.chi.as<double>() + J.off;
         RA.ergy = P.B.ergy.as<double>() + J.off; RA.totsteps = P.B.tot.as<int32_t>() + J.off;
         RA.per_iter = J.T; RA.ccs_const = J.mol->prep.ccs_const;
-        RA.out = Q.dsums.as<double>() + kSums * Q.issued;
-        hipLaunchKernelGGL(k_reduce, dim3(J.iters), dim3(256), 0, s, RA);
+        RA.out = X.e[i].job->dsums.as<double>() + kSums * X.e[i].it0;
+        hipLaunchKernelGGL(k_reduce, dim3(J.iters), dim3(256), 0, X.s, RA);
         HIP_CHECK(hipGetLastError());
     }
+    return 0;
+}
+
+// Waits for a slice and applies the stopping rule block by block in block
+// order to every request in it (blocks of finished requests are ignored).
+int collect_slice(cosims_server *S, Slice &X)
+{
+    const auto &c = S->cfg;
+    BatchPlan &P = *X.P;
+    GroupRun R;
+    if (int rc = launch_finish(P, X.s, R)) return rc;
+    // copies on the slice's own stream: the next slice keeps running meanwhile
     std::vector<int8_t> status(P.n);
-    std::vector<std::vector<double>> sums(plan.size());
-    HIP_CHECK(hipMemcpyAsync(status.data(), P.B.status.p, P.n, hipMemcpyDeviceToHost, s));
-    for (size_t i = 0; i < plan.size(); i++) {
-        SJob &Q = *act[plan[i].k];
-        sums[i].resize((size_t)kSums * plan[i].nb);
-        HIP_CHECK(hipMemcpyAsync(sums[i].data(), Q.dsums.as<double>() + kSums * Q.issued,
-                                 sizeof(double) * kSums * plan[i].nb, hipMemcpyDeviceToHost, s));
+    std::vector<std::vector<double>> all(X.e.size());
+    HIP_CHECK(hipMemcpyAsync(status.data(), P.B.status.p, P.n, hipMemcpyDeviceToHost, X.s));
+    for (size_t i = 0; i < X.e.size(); i++) {
+        all[i].resize((size_t)kSums * X.e[i].nb);
+        HIP_CHECK(hipMemcpyAsync(all[i].data(), X.e[i].job->dsums.as<double>() + kSums * X.e[i].it0,
+                                 sizeof(double) * all[i].size(), hipMemcpyDeviceToHost, X.s));
     }
-    HIP_CHECK(hipStreamSynchronize(s));
-    // stopping rule, block by block in block order
+    HIP_CHECK(hipStreamSynchronize(X.s));
     double units = 0;
-    for (size_t i = 0; i < plan.size(); i++) {
-        SJob &J = *act[plan[i].k];
+    for (size_t i = 0; i < X.e.size(); i++) {
+        SJob &J = *X.e[i].job;
         const auto &B = P.jobs[i];
-        for (int b = 0; b < plan[i].nb; b++) units += sums[i][kSums * b + 3] * B.units;
-        for (int b = 0; b < plan[i].nb && !J.finished; b++) {
+        const std::vector<double> &sums = all[i];
+        for (int b = 0; b < X.e[i].nb; b++) units += sums[kSums * b + 3] * B.units;
+        for (int b = 0; b < X.e[i].nb && !J.finished; b++) {
+            if (X.e[i].it0 + b != J.used) return fail(COSIMS_E_INVALID, "server: blocks out of order");
             double om = 0, er = 0;
-            integral_ccs(status.data() + B.off + (int64_t)b * B.T, sums[i].data() + kSums * b, B.T,
+            integral_ccs(status.data() + B.off + (int64_t)b * B.T, sums.data() + kSums * b, B.T,
                          J.mol->params.print_rate, om, er);
             J.omega.push_back(om);
             J.omega_err.push_back(er);
-            J.sums.insert(J.sums.end(), sums[i].begin() + kSums * b, sums[i].begin() + kSums * (b + 1));
+            J.sums.insert(J.sums.end(), sums.begin() + kSums * b, sums.begin() + kSums * (b + 1));
             J.used++;
             if (J.used >= c.min_blocks) {
                 cosims_ccs tmp{};
@@ -297,12 +326,11 @@ int run_slice(cosims_server *S, std::vector<std::shared_ptr<SJob>> &act, BatchPl
                 }
             }
         }
-        J.issued += plan[i].nb;
     }
     std::lock_guard<std::mutex> lk(S->mu);
     if (R.ms > 0 && units > 0) S->units_per_ms = 0.5 * S->units_per_ms + 0.5 * units / R.ms;
     S->st.slices++;
-    S->st.launches += launches;
+    S->st.launches += X.launches;
     S->st.traj += P.n;
     S->st.kernel_ms += R.ms;
     S->st.pair_evals += R.pairs;
@@ -311,13 +339,27 @@ int run_slice(cosims_server *S, std::vector<std::shared_ptr<SJob>> &act, BatchPl
     return 0;
 }
 
+// The scheduler keeps up to pipeline_depth slices in flight: whenever a place is
+// free and some request wants blocks, the next slice (planned from the
+// results known so far and the blocks already issued) is sampled and
+// launched on its own streams, so one slice's tail overlaps the next ones.
+// Slices are collected in issue order (a request's blocks are taken in
+// block order), polling the oldest so that new slices can be issued while
+// it drains.
+constexpr int kMaxDepth = 8;
+
 void scheduler(cosims_server *S)
 {
-    Stream st;
-    if (hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking) != hipSuccess) st.s = nullptr;
-    BatchPlan P;
-    std::vector<std::shared_ptr<SJob>> act;
+    const int kDepth = S->cfg.pipeline_depth;
+    BatchPlan plans[kMaxDepth];
+    Stream streams[kMaxDepth];
+    bool ok = true;
+    for (auto &x : streams) ok &= hipStreamCreateWithFlags(&x.s, hipStreamNonBlocking) == hipSuccess;
+    Slice fly[kMaxDepth];
+    int head = 0, inflight = 0;  // fly[head] is the oldest slice
+    double last_end = 0;         // busy time: union of the slices' issue-to-collect spans
     for (;;) {
+        std::vector<std::shared_ptr<SJob>> act;
         {
             std::unique_lock<std::mutex> lk(S->mu);
             for (;;) {
@@ -332,22 +374,53 @@ void scheduler(cosims_server *S)
                     S->bytes_peak = std::max(S->bytes_peak, S->bytes_in_use);
                     S->active.push_back(J);
                 }
-                if (!S->active.empty()) break;
+                if (!S->active.empty() || inflight > 0) break;
                 if (S->stopping && S->queued.empty() && S->ready.empty() && S->setups_running == 0) return;
                 S->cv.wait(lk);
             }
             act = S->active;
         }
-        int rc = 0;
-        for (auto &J : act)
-            if (!J->dsums.p && J->dsums.alloc(sizeof(double) * kSums * S->cfg.max_blocks) != hipSuccess)
-                rc = fail(COSIMS_E_NOMEM, "server: per-request sums");
-        const double t = S->now_ms();
-        if (!rc) rc = st.s ? run_slice(S, act, P, st.s) : fail(COSIMS_E_HIP, "server: no stream");
+        int rc = ok ? 0 : fail(COSIMS_E_HIP, "server: no stream");
+        // fill the free places of the pipeline
+        while (!rc && inflight < kDepth) {
+            const int q = (head + inflight) % kDepth;
+            Slice &X = fly[q];
+            X.e = plan_slice(S, act);
+            if (!X.live()) break;
+            X.P = &plans[q];
+            X.s = streams[q].s;
+            X.t_issue = S->now_ms();
+            rc = issue_slice(S, X);
+            inflight++;
+        }
+        Slice &X = fly[head];
+        bool collected = false;
+        if (!rc && inflight > 0) {
+            const hipError_t q = hipStreamQuery(X.s);
+            if (q == hipSuccess || inflight == kDepth) {
+                rc = collect_slice(S, X);
+                collected = true;
+            } else if (q == hipErrorNotReady) {
+                // the oldest slice still runs and a place is free: wait a little
+                // for it, or for a new request to plan into the free place
+                std::unique_lock<std::mutex> lk(S->mu);
+                S->cv.wait_for(lk, std::chrono::milliseconds(1));
+                continue;
+            } else {
+                rc = fail(COSIMS_E_HIP, std::string("server: ") + hipGetErrorString(q));
+            }
+        }
         const std::string msg = rc ? g_err : std::string();
+        if (rc) {  // drain what is in flight before failing the requests
+            for (auto &x : streams) (void)hipStreamSynchronize(x.s);
+        }
         {
             std::lock_guard<std::mutex> lk(S->mu);
-            S->st.busy_ms += S->now_ms() - t;
+            if (collected) {
+                const double now = S->now_ms();
+                S->st.busy_ms += now - std::max(X.t_issue, last_end);
+                last_end = now;
+            }
             std::vector<std::shared_ptr<SJob>> keep;
             for (auto &J : S->active) {
                 if (rc || J->finished) {
@@ -355,11 +428,6 @@ void scheduler(cosims_server *S)
                     J->err = rc;
                     J->errmsg = msg;
                     J->t_done = S->now_ms();
-                    J->dsums.reset();
-                    if (J->mol) {
-                        cosims_mol_destroy(J->mol);
-                        J->mol = nullptr;
-                    }
                     S->bytes_in_use -= J->bytes;
                     if (rc) S->st.failed++;
                     else S->st.done++;
@@ -370,6 +438,16 @@ void scheduler(cosims_server *S)
             S->active.swap(keep);
         }
         S->done_cv.notify_all();
+        if (rc) {
+            for (auto &f : fly) f = Slice();
+            inflight = 0;
+            continue;
+        }
+        if (collected) {
+            X = Slice();  // releases its requests (molecules freed once no slice holds them)
+            head = (head + 1) % kDepth;
+            inflight--;
+        }
     }
 }
 
@@ -389,16 +467,18 @@ int cosims_server_create(const cosims_server_cfg *cfg, cosims_server **out)
     if (c.wg_size % 64 || c.wg_size > 1024)
         return fail(COSIMS_E_INVALID, "server wg_size must be a multiple of 64, at most 1024");
     if (c.min_slice_traj <= 0) c.min_slice_traj = 65536;
+    if (c.pipeline_depth <= 0) c.pipeline_depth = 3;
+    if (c.pipeline_depth > kMaxDepth) return fail(COSIMS_E_INVALID, "pipeline_depth at most 8");
     if (c.max_active <= 0) c.max_active = 64;
     if (c.block_traj <= 0) c.block_traj = 10000;
     if (c.slice_traj <= 0) c.slice_traj = 1 << 20;
-    if (c.slice_ms <= 0) c.slice_ms = 150;
+    if (c.slice_ms <= 0) c.slice_ms = 400;
     if (c.policy != COSIMS_SERVE_SHORTEST_FIRST && c.policy != COSIMS_SERVE_EQUAL_SHARE)
         return fail(COSIMS_E_INVALID, "unknown server policy");
     if (c.min_blocks <= 0) c.min_blocks = 3;
     if (c.max_blocks <= 0) c.max_blocks = 1000;
     if (c.min_blocks > c.max_blocks) return fail(COSIMS_E_INVALID, "min_blocks > max_blocks");
-    if (c.setup_threads <= 0) c.setup_threads = 2;
+    if (c.setup_threads <= 0) c.setup_threads = 8;
     Device d;
     if (int rc = device_init(d)) return rc;
     std::unique_ptr<cosims_server> S(new cosims_server());

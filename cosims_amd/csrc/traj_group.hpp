@@ -77,13 +77,31 @@ struct MultiArgs {
     TrajIO io;
     const MolRec *mols;
     const int32_t *tmol;                   // molecule of each trajectory (NULL: all molecule 0)
+    // when set, the work list is io.items[*first .. *last) (first NULL: from 0),
+    // bounds written on the device by the sampler before the launch runs
+    const int32_t *first, *last;
 };
 
-// chi = vector3D::Angle of the accepted exit (src/vector3D.h:134-143) from
-// the stored cosine, for the items of one group launch.
-__global__ void __launch_bounds__(256) k_chi(const int32_t *items, int n, const int8_t *status, const double *cchi,
-                                             double *chi)
+// Work-list bounds held on the device (see MultiArgs::first / last).
+__device__ __forceinline__ void resolve_range(TrajIO &io, const int32_t *first, const int32_t *last)
 {
+    if (!last) return;
+    const int b = first ? *first : 0;
+    io.items += b;
+    io.n_items = *last - b;
+}
+
+// chi = vector3D::Angle of the accepted exit (src/vector3D.h:134-143) from
+// the stored cosine, for the items of one group launch (items[0..n), or
+// items[*first .. *last) when last is set).
+__global__ void __launch_bounds__(256) k_chi(const int32_t *items, int n, const int8_t *status, const double *cchi,
+                                             double *chi, const int32_t *first, const int32_t *last)
+{
+    if (last) {
+        const int b = first ? *first : 0;
+        items += b;
+        n = *last - b;
+    }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int t = items[i];
@@ -684,8 +702,10 @@ __global__ void __launch_bounds__(1024) k_traj_multi(MultiArgs A)
 {
     extern __shared__ int lds_lists[];
     int *lists = multi_uses_lists(FT, L) ? lds_lists + (threadIdx.x >> 6) * kListInts : nullptr;
+    TrajIO io = A.io;
+    resolve_range(io, A.first, A.last);
     TableSource<PREC, FT, L> src{A, lists};
-    group_loop<L>(A.io, src);
+    group_loop<L>(io, src);
 }
 
 }  // namespace cosims
