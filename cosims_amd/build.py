@@ -1,7 +1,9 @@
-"""Build libcosims_engine.so (HIP kernels for gfx950 + host setup + C ABI).
+"""Build libcosims_engine.so (HIP kernels for gfx950 + host setup + C ABI)
+and the command-line program that drives it.
 
 Compiles in-tree: objects under cosims_amd/_build/, the library at
-cosims_amd/libcosims_engine.so.  Run ``python -m cosims_amd.build``.
+cosims_amd/libcosims_engine.so, the program at cosims_amd/cosims.  Run
+``python -m cosims_amd.build``.
 """
 from __future__ import annotations
 
@@ -15,6 +17,7 @@ ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcosims_engine.so")
+CLI = os.path.join(HERE, "cosims")
 ARCH = os.environ.get("COSIMS_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -53,6 +56,12 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"], verbose)
     os.replace(tmp, LIB)
+    # the command-line program: host code over the C ABI, found next to the library
+    o = os.path.join(BUILD, "cli.o")
+    _run([CXX, *CXX_FLAGS, *inc, "-c", os.path.join(SRC, "cli.cpp"), "-o", o], verbose)
+    tmp = CLI + ".tmp"
+    _run([HIPCC, "-o", tmp, o, "-L" + HERE, "-lcosims_engine", "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(tmp, CLI)
     return LIB
 
 

@@ -126,6 +126,25 @@ int cosims_molfile_read(const char *mol_path, const char *atoms_path, cosims_mol
     for (const auto &r : table) smax = std::max(smax, r.sigma);
     f->sigma_max = smax;
     std::snprintf(f->name, sizeof(f->name), "%s", mf.name.c_str());
+    // fieldData rows with their atom counts (src/molecule2.cpp:820, 920-927)
+    f->ntypes = (int32_t)table.size();
+    f->types = new cosims_atom_type[table.size()];
+    for (size_t k = 0; k < table.size(); k++) {
+        cosims_atom_type &t = f->types[k];
+        std::memset(&t, 0, sizeof(t));
+        std::snprintf(t.name, sizeof(t.name), "%s", table[k].name.c_str());
+        std::snprintf(t.symbol, sizeof(t.symbol), "%s", table[k].symbol.c_str());
+        t.mass = table[k].mass;
+        t.sigma = table[k].sigma;
+        t.epsilon = table[k].eps;
+        t.int_mass = table[k].intmass;
+    }
+    for (const auto &s : mf.symbols)  // first matching row, as the assignment loop
+        for (size_t k = 0; k < table.size(); k++)
+            if (s == table[k].symbol) {
+                f->types[k].count++;
+                break;
+            }
     *out = f;
     return COSIMS_OK;
 }
@@ -138,6 +157,7 @@ void cosims_molfile_free(cosims_molfile *f)
     delete[] f->sigma;
     delete[] f->epsilon;
     delete[] f->charge;
+    delete[] f->types;
     delete f;
 }
 

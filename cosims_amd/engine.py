@@ -49,11 +49,17 @@ class Params(C.Structure):
                 ("setup_gpu", C.c_int32)]
 
 
+class AtomType(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("symbol", C.c_char * 8), ("mass", C.c_double),
+                ("sigma", C.c_double), ("epsilon", C.c_double), ("int_mass", C.c_int32), ("count", C.c_int32)]
+
+
 class MolFile(C.Structure):
     _fields_ = [("n", C.c_int32), ("xyz", C.POINTER(C.c_double)), ("mass", C.POINTER(C.c_double)),
                 ("sigma", C.POINTER(C.c_double)), ("epsilon", C.POINTER(C.c_double)),
                 ("charge", C.POINTER(C.c_double)), ("charged", C.c_int32), ("total_charge", C.c_double),
-                ("sigma_max", C.c_double), ("name", C.c_char * 128)]
+                ("sigma_max", C.c_double), ("name", C.c_char * 128), ("ntypes", C.c_int32),
+                ("types", C.POINTER(AtomType))]
 
 
 class Atoms(C.Structure):
@@ -229,7 +235,10 @@ def read_molfile(mol_path, atoms_path=None) -> dict:
                epsilon=np.ctypeslib.as_array(f.epsilon, (n,)).copy(),
                charge=np.ctypeslib.as_array(f.charge, (n,)).copy(),
                charged=bool(f.charged), total_charge=f.total_charge, sigma_max=f.sigma_max,
-               name=f.name.decode())
+               name=f.name.decode(),
+               types=[dict(name=t.name.decode(), symbol=t.symbol.decode(), mass=t.mass, sigma=t.sigma,
+                           epsilon=t.epsilon, int_mass=t.int_mass, count=t.count)
+                      for t in (f.types[k] for k in range(f.ntypes))])
     lib().cosims_molfile_free(ptr)
     return out
 

@@ -75,6 +75,14 @@ int cosims_params_read(const char *input_path, cosims_params *p);
  * molecule2::assignDefaultFF/importFile/importMFJ/importPDB
  * (src/molecule2.cpp:43-192, 442-768), types resolved as in
  * setMolProperties (:782-839). */
+typedef struct cosims_atom_type {   /* one row of molecule2::fieldData (src/molecule2.h) */
+    char name[32];        /* lower case when it came from atoms.dat */
+    char symbol[8];
+    double mass, sigma, epsilon;  /* amu, A, kcal/mol */
+    int32_t int_mass;
+    int32_t count;        /* atoms of this type in the molecule (fieldData::atomCount) */
+} cosims_atom_type;
+
 typedef struct cosims_molfile {
     int32_t n;
     double *xyz;          /* 3n, file order */
@@ -86,6 +94,8 @@ typedef struct cosims_molfile {
     double total_charge;  /* totalCharge_ext as the reader sets it */
     double sigma_max;     /* largest sigma of the whole type table */
     char name[128];       /* MFJ title line (empty for PDB) */
+    int32_t ntypes;       /* the type table after atoms.dat, in table order */
+    cosims_atom_type *types;
 } cosims_molfile;
 int cosims_molfile_read(const char *mol_path, const char *atoms_dat_path, cosims_molfile **out);
 void cosims_molfile_free(cosims_molfile *f);

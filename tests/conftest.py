@@ -82,3 +82,27 @@ def golden_engine_setup(g, threads=0, setup_gpu=0):
 def oracle_lib():
     import oracle
     return oracle.lib()
+
+
+CLI = os.path.join(ROOT, "cosims_amd", "cosims")
+LOGS = json.load(open(os.path.join(GOLDEN, "example_logs.json")))
+
+
+def run_cli_example(entry, work, extra=(), timeout=600):
+    """Runs the command-line program on one of the reference's example runs
+    (its molecule, input file, atoms.dat and seed) in directory `work`.
+    Returns (exit code, console text, path of the omega log it named)."""
+    import shutil
+    import subprocess
+    shutil.copy(os.path.join(GOLDEN, entry["mol"]), work)
+    args = [CLI, "-m", entry["mol"]]
+    if entry["input"]:
+        with open(os.path.join(work, "input.txt"), "w") as f:
+            f.write(entry["input"])
+        args += ["-i", "input.txt"]
+    if entry["atoms_dat"]:
+        shutil.copy(os.path.join(GOLDEN, entry["atoms_dat"]), work)
+        args += ["-a", entry["atoms_dat"]]
+    args += ["-s", str(entry["seed"]), *extra]
+    r = subprocess.run(args, cwd=work, capture_output=True, text=True, timeout=timeout)
+    return r.returncode, r.stdout + r.stderr, os.path.join(work, os.path.basename(entry["log"]))

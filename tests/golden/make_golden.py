@@ -10,7 +10,10 @@ are what the reference's own classes return for them:
   * forceAccel::getReducedVec / getPotential at seeded probe points for each
     force type the case exercises;
   * diffEqSolver::verletVel on seeded initial conditions (chi, success,
-    steps, energy deviation).
+    steps, energy deviation);
+  * the example runs' logs (examples/*/omega_*.txt): settings, the CCS
+    integrals with their standard errors, the final statistics, and the
+    lines logfilter.log_lines keeps.
 
 Floats are written with repr() (round-trip exact).
 """
@@ -30,6 +33,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 import oracle  # noqa: E402
+import logfilter  # noqa: E402  (tests/golden/logfilter.py)
 from cosims_amd import synth  # noqa: E402
 
 REFLIB = os.path.join(ROOT, "oracle", "_ref", "libcosims_ref.so")
@@ -208,7 +212,8 @@ def example_logs():
                (("mean", "Mean Cross-section:"), ("sd", "Standard Dev:"), ("wmean", "Weighted CCS Mean:"),
                 ("werr", "Weighted Error:"), ("max_ergy", "Max Energy Deviation:"))}
         out.append(dict(log=f"examples/{d}/{log}", mol=mol, input=inp, atoms_dat=atoms, seed=seed, iters=iters,
-                        traj=traj, omega=[o[0] for o in om], omega_err=[o[1] for o in om], final=fin))
+                        traj=traj, omega=[o[0] for o in om], omega_err=[o[1] for o in om], final=fin,
+                        lines=logfilter.log_lines(text)))
     return out
 
 
