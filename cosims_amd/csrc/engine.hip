@@ -848,7 +848,11 @@ int upload(const cosims_mol *M)
            oc32 = align_up(oc64 + sizeof(AtomF64) * n), oh64 = align_up(oc32 + sizeof(AtomF32) * n),
            oh32 = align_up(oh64 + sizeof(ClusterF64) * ncl), osoa = align_up(oh32 + sizeof(ClusterF32) * ncl),
            oxq = align_up(osoa + sizeof(double) * 7 * npad), ose = align_up(oxq + sizeof(float4) * npad),
-           orec = align_up(ose + sizeof(float2) * npad), total = align_up(orec + sizeof(MolRec));
+           orec = align_up(ose + sizeof(float2) * npad), og0 = align_up(orec + sizeof(MolRec)),
+           og1 = align_up(og0 + sizeof(float4) * ncl), og2 = align_up(og1 + sizeof(float4) * ncl),
+           ocse = align_up(og2 + sizeof(float4) * ncl), ocxq = align_up(ocse + sizeof(int2) * ncl),
+           ocab = align_up(ocxq + sizeof(float4) * (ncl > 0 ? n : 0)),
+           total = align_up(ocab + sizeof(float2) * (ncl > 0 ? n : 0));
     std::vector<char> host(total, 0);
     auto *a64 = (AtomF64 *)(host.data() + o64);
     auto *a32 = (AtomF32 *)(host.data() + o32);
@@ -870,6 +874,34 @@ int upload(const cosims_mol *M)
     for (int c = 0; c < ncl; c++) {
         fill_head(h64[c], p, c);
         fill_head(h32[c], p, c);
+    }
+    // FP32 cluster SoA of the mixed-precision clustered kernel
+    auto *g0 = (float4 *)(host.data() + og0);
+    auto *g1 = (float4 *)(host.data() + og1);
+    auto *g2 = (float4 *)(host.data() + og2);
+    auto *cse = (int2 *)(host.data() + ocse);
+    auto *cxq = (float4 *)(host.data() + ocxq);
+    auto *cab = (float2 *)(host.data() + ocab);
+    float dpr_max = 0.0f;
+    for (int c = 0; c < ncl; c++) {
+        const ClusterF32 &h = h32[c];
+        dpr_max = std::max(dpr_max, h.dpr);
+        const bool dip = p.multipole_order >= 1 && !h.rep;
+        g0[c] = make_float4(h.px, h.py, h.pz, h.mpr);
+        g1[c] = make_float4(h.q, dip ? h.d0 : 0.0f, dip ? h.d1 : 0.0f, dip ? h.d2 : 0.0f);
+        float A, B;
+        const int qi = c < n ? c : 0;
+        lj_constants(p.sig6[qi], p.eps24m[qi], A, B);
+        g2[c] = make_float4(h.dpr, A, B, 0.0f);
+        cse[c] = make_int2(h.start, h.end);
+    }
+    for (int k = 0; k < (ncl > 0 ? n : 0); k++) {
+        const AtomF32 &a = c32[k];
+        const int i = p.cl_atoms[k];
+        cxq[k] = make_float4(a.x, a.y, a.z, a.q);
+        float A, B;
+        lj_constants(p.sig6[i], p.eps24m[i], A, B);
+        cab[k] = make_float2(A, B);
     }
     // FP64 SoA in atom order, padded with inert atoms (no charge, no LJ, far away)
     double *soa = (double *)(host.data() + osoa);
@@ -903,6 +935,9 @@ int upload(const cosims_mol *M)
     m.a64 = (const AtomF64 *)(d + o64); m.a32 = (const AtomF32 *)(d + o32);
     m.c64 = (const AtomF64 *)(d + oc64); m.c32 = (const AtomF32 *)(d + oc32);
     m.h64 = (const ClusterF64 *)(d + oh64); m.h32 = (const ClusterF32 *)(d + oh32);
+    m.cg0 = (const float4 *)(d + og0); m.cg1 = (const float4 *)(d + og1); m.cg2 = (const float4 *)(d + og2);
+    m.cse = (const int2 *)(d + ocse); m.cxq = (const float4 *)(d + ocxq); m.cab = (const float2 *)(d + ocab);
+    m.cl_dpr_max = dpr_max;
     M->soa = (const double *)(d + osoa);
     M->npad = npad;
     MolRec &r = M->rec;
@@ -1621,8 +1656,24 @@ int tune_shapes(int units, RunFn run, int &L_out, int &W_out)
     std::vector<Cand> cands;
     const int Ls[] = {4, 8, 16, 32, 64};
     const int Ws[] = {256, 512, 1024};
-    for (int L : Ls) {
-        if (L > 4 && units < 8 * L) continue;  // too few atoms (clusters) per lane
+    // screen: each lane count once on the smallest set; those more than 3x
+    // slower than the best are dropped (on large molecules of the clustered
+    // types the slow shapes would otherwise cost minutes of tuning)
+    std::vector<int> keep;
+    {
+        std::vector<std::pair<int, double>> sc;
+        double best = 0;
+        for (int L : Ls) {
+            if (L > 4 && units < 8 * L) continue;  // too few atoms (clusters) per lane
+            double ms = 0;
+            if (int rc = run(2, L, 512, ms)) return rc;
+            sc.push_back({L, ms});
+            best = sc.size() == 1 ? ms : std::min(best, ms);
+        }
+        for (const auto &x : sc)
+            if (x.second <= 3.0 * best) keep.push_back(x.first);
+    }
+    for (int L : keep) {
         for (int W : Ws) {
             double ms = 0;
             if (int rc = run(0, L, W, ms)) return rc;
@@ -1673,9 +1724,10 @@ int autotune(const cosims_mol *M, int prec, int64_t n_hint, int &L_out, int &wg_
     Stream st;
     HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
     // stage 1 holds several rounds even at 4 lanes per trajectory (16 trajectories per wavefront)
-    const int64_t nset[2] = {std::min<int64_t>(n2, std::max<int64_t>(16384, (int64_t)M->dev.cus * 256)), n2};
+    const int64_t nset[3] = {std::min<int64_t>(n2, std::max<int64_t>(16384, (int64_t)M->dev.cus * 256)), n2,
+                             std::min<int64_t>(n2, (int64_t)M->dev.cus * 16)};
     TrajBufs B;
-    if (int rc = B.alloc(nset[1])) return rc;
+    if (int rc = B.alloc(std::max(nset[1], nset[0]))) return rc;
     int cur = -1;
     int32_t hits = 0;
     auto run = [&](int set, int L, int W, double &ms) -> int {
@@ -2005,10 +2057,10 @@ int tune_batch(const cosims_batch_job *jobs, int32_t njobs, int prec, ShapeTable
         if (by_class[c].empty()) continue;
         int units = 0;  // largest molecule of the class: the lane counts worth trying
         for (const auto &j : by_class[c]) units = std::max(units, group_units(j.mol->prep));
-        // tuning subsets: at most 64 jobs of the class, 128 / 512 trajectories each
+        // tuning subsets: at most 64 jobs of the class, 128 / 512 trajectories each (32 to screen)
         std::vector<cosims_batch_job> sub(by_class[c].begin(),
                                           by_class[c].begin() + std::min<size_t>(64, by_class[c].size()));
-        const int64_t caps[2] = {128, 512};
+        const int64_t caps[3] = {128, 512, 32};
         BatchPlan set;
         auto run = [&](int k, int L, int W, double &ms) -> int {
             ShapeTable fixed;

@@ -46,6 +46,16 @@ struct MolDev {
     const AtomF32 *c32;
     const ClusterF64 *h64;
     const ClusterF32 *h32;
+    // FP32 cluster SoA of the mixed-precision clustered kernel (ncl each):
+    // cg0 (x, y, z, (R_m + rho)^2), cg1 (charge, dipole; the dipole zero when
+    // the centre was replaced or the order is 0), cg2 ((R_d + rho)^2 and the
+    // band branch's LJ constants A, B), cse (first, end) of the members in
+    // cluster order; cxq / cab: the members' (x, y, z, q) and LJ constants.
+    const float4 *cg0, *cg1, *cg2;
+    const int2 *cse;
+    const float4 *cxq;
+    const float2 *cab;
+    float cl_dpr_max;     // largest (R_d + rho)^2 of cg2 (FP32)
 };
 
 template <typename T>

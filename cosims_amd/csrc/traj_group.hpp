@@ -40,6 +40,7 @@ __device__ __forceinline__ const GLB T *glb(const T *p)
 // cannot be copied out of an explicit address space).
 typedef float f4n __attribute__((ext_vector_type(4)));
 typedef float f2n __attribute__((ext_vector_type(2)));
+typedef int i2n __attribute__((ext_vector_type(2)));
 
 // Per-trajectory arrays of one persistent launch.
 struct TrajIO {
@@ -385,6 +386,245 @@ __device__ __forceinline__ void wave_clustered(const MolDev &m, int ncl, int lan
     ox = a; oy = b; oz = c;
 }
 
+// Mixed-precision clustered force types 1, 2, 3, 5 with one group per
+// wavefront (multipole order 0 or 1), the FP32 counterpart of
+// wave_clustered.  Same branches and terms as cl_one / cl_epilogue
+// (src/forceAccel.cpp:244-442, 470-630, 655-746, 819-851), regrouped:
+//   * lane c classifies cluster c of each 64-cluster slice from the SoA
+//     headers (one float4 load for the test, one more for a far cluster) and
+//     adds a far cluster's monopole + dipole terms at once;
+//   * near clusters (exact charges, with or without LJ) and LJ clusters
+//     (per-atom, or the band branch's fixed parameters) go to two 128-entry
+//     rings in LDS by ballot, and each ring is emptied 64 entries at a time,
+//     lane by lane, so the per-atom loops run with all lanes busy;
+//   * the field gradient is kept as M_ab = sum w d_a d_b plus a trace term:
+//     a far cluster's diagonal r^2 (cR5 + c1 r^-7) - d_a^2 (3 cR5 + 5 c1 r^-7)
+//     and off-diagonal -d_a d_b (3 cR5 + 5 c1 r^-7) (c1 = 3 p.d, cR5 = q r^-5)
+//     are w = 3 cR5 + 5 c1 r^-7 and trace cR3 + c1 r^-5; an atom's are
+//     w = 3 q r^-5 and trace q r^-3; the gradient is T = M - trace I;
+//   * LJ as r^-8 (A r^-6 - B) with the per-atom constants of grp_force32;
+//     one v_rsq_f32 per far cluster and per atom, v_rcp_f32 for LJ only.
+// The epilogue (dipole_scalar x (T and the antisymmetric part) . E) runs
+// in FP64 after the FP32 wavefront sums.
+constexpr int kRing = 128;  // ring entries per list; two lists per wavefront
+
+#pragma clang fp contract(fast)
+template <bool LJ>
+__device__ __forceinline__ void cl32_atom(f4n a, f2n ab, float px, float py, float pz, float (&s)[16])
+{
+    const float dx = px - a.x, dy = py - a.y, dz = pz - a.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float ir2 = ir * ir;
+    if (LJ) {
+        const float ir4 = ir2 * ir2;
+        const float lj = (ir4 * ir4) * (ab.x * (ir4 * ir2) - ab.y);
+        s[0] += lj * dx; s[1] += lj * dy; s[2] += lj * dz;
+    }
+    const float qr3 = a.w * ir2 * ir;
+    s[3] += qr3 * dx; s[4] += qr3 * dy; s[5] += qr3 * dz;
+    s[12] += qr3;
+    const float w = 3.0f * (qr3 * ir2);
+    const float u = w * dx, v = w * dy;
+    s[6] += u * dx; s[9] += u * dy; s[10] += u * dz;
+    s[7] += v * dy; s[11] += v * dz;
+    s[8] += (w * dz) * dz;
+}
+
+__device__ __forceinline__ void cl32_lj(f4n a, f2n ab, float px, float py, float pz, float (&s)[16])
+{
+    const float dx = px - a.x, dy = py - a.y, dz = pz - a.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float ir2 = __builtin_amdgcn_rcpf(r2);
+    const float ir6 = ir2 * ir2 * ir2;
+    const float lj = (ir6 * ir2) * (ab.x * ir6 - ab.y);
+    s[0] += lj * dx; s[1] += lj * dy; s[2] += lj * dz;
+}
+
+// far-field monopole + dipole of one cluster (src/forceAccel.cpp:258-316)
+__device__ __forceinline__ void cl32_far(f4n g, float dx, float dy, float dz, float r2, float (&s)[16])
+{
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float ir2 = ir * ir;
+    const float r3 = ir2 * ir, r5 = r3 * ir2, r7 = r5 * ir2;
+    const float c1 = 3.0f * (g.y * dx + g.z * dy + g.w * dz);
+    const float cR3 = g.x * r3, cR5 = g.x * r5;
+    const float al = c1 * r5 + cR3;
+    s[3] += al * dx - r3 * g.y; s[4] += al * dy - r3 * g.z; s[5] += al * dz - r3 * g.w;
+    s[12] += al;
+    const float w = 3.0f * cR5 + 5.0f * (c1 * r7);
+    const float u = w * dx, v = w * dy;
+    s[6] += u * dx; s[9] += u * dy; s[10] += u * dz;
+    s[7] += v * dy; s[11] += v * dz;
+    s[8] += (w * dz) * dz;
+    const float s5 = 3.0f * r5;
+    const float sx = s5 * dx, sy = s5 * dy, sz = s5 * dz;
+    s[13] += sx * g.z - sy * g.y;
+    s[14] += sx * g.w - sz * g.y;
+    s[15] += sy * g.w - sz * g.z;
+}
+#pragma clang fp contract(off)
+
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Members [k0, k1) of one cluster in cluster order, kUnroll at a time with
+// their loads issued together (indices clamped to the cluster, the extra
+// slots contribute nothing: charge and LJ constants zeroed).  MODE 0:
+// charges; 1: charges + LJ; 2: LJ with the members' constants; 3: LJ with
+// the band branch's fixed constants.
+constexpr int kUnroll = 2;
+
+template <int MODE>
+__device__ __forceinline__ void cl32_members(const MolDev &m, int k0, int k1, f2n fixed, float px, float py,
+                                             float pz, float (&s)[16])
+{
+    const GLB f4n *X = (const GLB f4n *)glb(m.cxq);
+    const GLB f2n *AB = (const GLB f2n *)glb(m.cab);
+    for (int k = k0; k < k1; k += kUnroll) {
+        f4n a[kUnroll];
+        f2n b[kUnroll];
+#pragma unroll
+        for (int i = 0; i < kUnroll; i++) {
+            const int kk = min(k + i, k1 - 1);
+            a[i] = X[kk];
+            b[i] = (MODE == 1 || MODE == 2) ? AB[kk] : fixed;
+        }
+#pragma unroll
+        for (int i = 0; i < kUnroll; i++) {
+            const bool on = k + i < k1;
+            if (!on) {
+                a[i].w = 0.0f;
+                b[i] = f2n{0.0f, 0.0f};
+            }
+            if constexpr (MODE == 0) cl32_atom<false>(a[i], b[i], px, py, pz, s);
+            else if constexpr (MODE == 1) cl32_atom<true>(a[i], b[i], px, py, pz, s);
+            else cl32_lj(a[i], b[i], px, py, pz, s);
+        }
+    }
+}
+
+// Empties cnt (<= 64) ring entries from head: lane k takes entry k's
+// cluster and its members.  Entries are 4 c + kind.
+template <bool EXACT>
+__device__ __forceinline__ void cl32_flush(const MolDev &m, const int *ring, int head, int cnt, int lane, float px,
+                                           float py, float pz, float (&s)[16])
+{
+    if (lane >= cnt) return;
+    const int v = ring[(head + lane) & (kRing - 1)];
+    const int c = v >> 2, kind = v & 3;
+    const i2n se = ((const GLB i2n *)glb(m.cse))[c];
+    if constexpr (EXACT) {
+        if (kind == 2) cl32_members<1>(m, se.x, se.y, f2n{0.0f, 0.0f}, px, py, pz, s);
+        else cl32_members<0>(m, se.x, se.y, f2n{0.0f, 0.0f}, px, py, pz, s);
+    } else {
+        if (kind == 2) {
+            const f4n g2 = ((const GLB f4n *)glb(m.cg2))[c];
+            cl32_members<3>(m, se.x, se.y, f2n{g2.y, g2.z}, px, py, pz, s);
+        } else {
+            cl32_members<2>(m, se.x, se.y, f2n{0.0f, 0.0f}, px, py, pz, s);
+        }
+    }
+}
+
+template <int FT>
+__device__ __forceinline__ void wave_clustered32(const MolDev &m, int ncl, int lane, float px, float py, float pz,
+                                                 int *lists, double &ox, double &oy, double &oz)
+{
+    constexpr bool kFar = FT == 1 || FT == 2;
+    // s: LJ force 0-2, field 3-5, M xx yy zz xy xz yz 6-11, trace 12, antisymmetric xy xz yz 13-15
+    float s[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) s[k] = 0.0f;
+    int *ex_ring = lists, *lj_ring = lists + kRing;
+    int ex_head = 0, ex_n = 0, lj_head = 0, lj_n = 0;
+    const unsigned long long below = (1ull << lane) - 1;
+    const GLB f4n *G0 = (const GLB f4n *)glb(m.cg0);
+    const GLB f4n *G1 = (const GLB f4n *)glb(m.cg1);
+    const GLB f4n *G2 = (const GLB f4n *)glb(m.cg2);
+    const float dpr_max = m.cl_dpr_max;  // no cluster's (R_d + rho)^2 exceeds it
+    // headers of the next slice are loaded while the current one is worked on
+    f4n n0 = f4n{0.0f, 0.0f, 0.0f, 0.0f}, n1 = n0;
+    if (ncl > 0) {
+        const int c = min(lane, ncl - 1);
+        n0 = G0[c];
+        if (kFar) n1 = G1[c];
+    }
+    for (int base = 0; base < ncl; base += 64) {
+        const int c = base + lane;
+        const f4n h = n0, g = n1;
+        if (base + 64 < ncl) {
+            const int cn = min(c + 64, ncl - 1);
+            n0 = G0[cn];
+            if (kFar) n1 = G1[cn];
+        }
+        int kex = 0, klj = 0;  // exact: 1 charges, 2 charges + LJ; LJ: 1 per-atom, 2 fixed parameters
+        if (c < ncl) {
+            const float dx = px - h.x, dy = py - h.y, dz = pz - h.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            // within (R_d + rho)^2 of this cluster (its LJ cut-off)
+            const bool disp = !(r2 > dpr_max) && !(r2 > G2[c].x);
+            if constexpr (kFar) {
+                if (r2 > h.w) {
+                    cl32_far(g, dx, dy, dz, r2, s);
+                    if (FT == 2) klj = 1;
+                    else if (disp) klj = 2;
+                } else {
+                    kex = (FT == 2 || disp) ? 2 : 1;
+                }
+            } else if constexpr (FT == 3) {
+                kex = disp ? 2 : 1;
+            } else {
+                if (disp) klj = 1;
+            }
+        }
+        const unsigned long long me = __ballot(kex != 0), ml = __ballot(klj != 0);
+        if (kex) ex_ring[(ex_head + ex_n + __popcll(me & below)) & (kRing - 1)] = 4 * c + kex;
+        if (klj) lj_ring[(lj_head + lj_n + __popcll(ml & below)) & (kRing - 1)] = 4 * c + klj;
+        ex_n += __popcll(me);
+        lj_n += __popcll(ml);
+        wave_sync_lds();
+        if (ex_n >= 64) {
+            cl32_flush<true>(m, ex_ring, ex_head, 64, lane, px, py, pz, s);
+            ex_head = (ex_head + 64) & (kRing - 1);
+            ex_n -= 64;
+        }
+        if (lj_n >= 64) {
+            cl32_flush<false>(m, lj_ring, lj_head, 64, lane, px, py, pz, s);
+            lj_head = (lj_head + 64) & (kRing - 1);
+            lj_n -= 64;
+        }
+        // the next slice's appends reuse the flushed slots: every lane is past its reads
+        wave_sync_lds();
+    }
+    if (ex_n) cl32_flush<true>(m, ex_ring, ex_head, ex_n, lane, px, py, pz, s);
+    if (lj_n) cl32_flush<false>(m, lj_ring, lj_head, lj_n, lane, px, py, pz, s);
+    wave_sync_lds();
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        if (FT == 5 && k >= 3) break;
+        if (FT == 3 && k >= 13) break;
+        s[k] = gsum<64>(s[k]);
+    }
+    double fx = s[0], fy = s[1], fz = s[2];
+    if constexpr (FT != 5) {
+        const double ds = m.dipole_scalar;
+        const double Ex = s[3], Ey = s[4], Ez = s[5], t = s[12];
+        const double Txx = (double)s[6] - t, Tyy = (double)s[7] - t, Tzz = (double)s[8] - t;
+        const double Txy = s[9], Txz = s[10], Tyz = s[11];
+        double Axy = 0, Axz = 0, Ayz = 0;
+        if constexpr (kFar) { Axy = s[13]; Axz = s[14]; Ayz = s[15]; }
+        fx -= ds * (Txx * Ex + (Txy - Axy) * Ey + (Txz - Axz) * Ez);
+        fy -= ds * ((Txy + Axy) * Ex + Tyy * Ey + (Tyz - Ayz) * Ez);
+        fz -= ds * ((Txz + Axz) * Ex + (Tyz + Ayz) * Ey + Tzz * Ez);
+    }
+    ox = fx; oy = fy; oz = fz;
+}
+
 // getPotential across the group (src/forceAccel.cpp:943-982), FP64, over
 // the SoA X (x y z q s6 e24 e4, stride npad).
 template <int L, typename PD>
@@ -479,9 +719,12 @@ struct TableSource {
                 grp_force64<FT, L>(glb(r.soa), npad, j, ex, ey, ez, r.m.dipole_scalar, fx, fy, fz);
         } else if constexpr (L == 64) {
             const int ncl = active ? r.m.ncl : 0;
-            if constexpr (PREC == 1)
-                wave_clustered<float, FT>(r.m, ncl, j, (float)ex, (float)ey, (float)ez, lists, fx, fy, fz);
-            else
+            if constexpr (PREC == 1) {
+                if (r.m.order <= 1)
+                    wave_clustered32<FT>(r.m, ncl, j, (float)ex, (float)ey, (float)ez, lists, fx, fy, fz);
+                else
+                    wave_clustered<float, FT>(r.m, ncl, j, (float)ex, (float)ey, (float)ez, lists, fx, fy, fz);
+            } else
                 wave_clustered<double, FT>(r.m, ncl, j, ex, ey, ez, lists, fx, fy, fz);
         } else {
             MolDev m = r.m;

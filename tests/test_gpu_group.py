@@ -53,8 +53,7 @@ def test_clustered_group_fp64_matches_lane_kernel(case, flags):
         np.testing.assert_allclose(grp["omega"], lane["omega"], rtol=5e-4)
 
 
-@pytest.mark.parametrize("case,flags", [CLUSTERED[0], CLUSTERED[4], CLUSTERED[5]],
-                         ids=["acau_charged-1111", "acau_neutral-0111", "ion2000-1111"])
+@pytest.mark.parametrize("case,flags", CLUSTERED, ids=[f"{c}-{''.join(map(str, f))}" for c, f in CLUSTERED])
 def test_clustered_group_mixed_close_to_fp64(case, flags):
     m = golden_engine_mol(golden(case), flags)
     kw = dict(job=4, iters=2, traj=4000, keep=True)
