@@ -176,44 +176,46 @@ __device__ __forceinline__ T gsum(T v)
 // with T = 3 S - tr(S) formed once after the loop (the reference sums
 // (3 d_a d_b - r^2 delta_ab) q r^-5 term by term, :790-797).
 #pragma clang fp contract(fast)
-template <int FT, int L, typename P4, typename P2>
-__device__ __forceinline__ void grp_force32(P4 xq, P2 ab, int npad, int j, float px, float py, float pz, double ds,
-                                            double &ox, double &oy, double &oz)
+// Partial sums of one lane: LJ force 0-2, field 3-5, S xx yy xy xz yz 6-10,
+// trace 11.  One pair (atom a = x, y, z, q; c = A, B) added in place; the
+// LDS and the global-memory loops both use it, so their sums are bitwise
+// identical.
+template <int FT>
+__device__ __forceinline__ void pair32(f4n a, f2n c, float px, float py, float pz, float (&s)[12])
 {
-    float fx = 0, fy = 0, fz = 0, ex = 0, ey = 0, ez = 0, sxx = 0, syy = 0, sxy = 0, sxz = 0, syz = 0, tr = 0;
-#pragma unroll 4
-    for (int kk = 0; kk < npad; kk += L) {
-        const int k = kk + j;
-        const f4n a = xq[k];
-        const f2n c = ab[k];
-        const float dx = px - a.x, dy = py - a.y, dz = pz - a.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        float ir2, ir;
-        if constexpr (FT == 4) {
-            ir = __builtin_amdgcn_rsqf(r2);
-            ir2 = ir * ir;
-        } else {
-            ir2 = __builtin_amdgcn_rcpf(r2);
-        }
-        const float ir4 = ir2 * ir2;
-        const float ir6 = ir4 * ir2;
-        const float lj = (ir4 * ir4) * (c.x * ir6 - c.y);
-        fx += lj * dx; fy += lj * dy; fz += lj * dz;
-        if constexpr (FT == 4) {
-            const float qr3 = a.w * ir2 * ir;
-            ex += qr3 * dx; ey += qr3 * dy; ez += qr3 * dz;
-            tr += qr3;
-            const float qr5 = qr3 * ir2;
-            const float u = qr5 * dx, v = qr5 * dy;
-            sxx += u * dx; syy += v * dy;
-            sxy += u * dy; sxz += u * dz; syz += v * dz;
-        }
-    }
-    fx = gsum<L>(fx); fy = gsum<L>(fy); fz = gsum<L>(fz);
+    const float dx = px - a.x, dy = py - a.y, dz = pz - a.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    float ir2, ir;
     if constexpr (FT == 4) {
-        ex = gsum<L>(ex); ey = gsum<L>(ey); ez = gsum<L>(ez);
-        sxx = gsum<L>(sxx); syy = gsum<L>(syy); tr = gsum<L>(tr);
-        sxy = gsum<L>(sxy); sxz = gsum<L>(sxz); syz = gsum<L>(syz);
+        ir = __builtin_amdgcn_rsqf(r2);
+        ir2 = ir * ir;
+    } else {
+        ir2 = __builtin_amdgcn_rcpf(r2);
+    }
+    const float ir4 = ir2 * ir2;
+    const float ir6 = ir4 * ir2;
+    const float lj = (ir4 * ir4) * (c.x * ir6 - c.y);
+    s[0] += lj * dx; s[1] += lj * dy; s[2] += lj * dz;
+    if constexpr (FT == 4) {
+        const float qr3 = a.w * ir2 * ir;
+        s[3] += qr3 * dx; s[4] += qr3 * dy; s[5] += qr3 * dz;
+        s[11] += qr3;
+        const float qr5 = qr3 * ir2;
+        const float u = qr5 * dx, v = qr5 * dy;
+        s[6] += u * dx; s[7] += v * dy;
+        s[8] += u * dy; s[9] += u * dz; s[10] += v * dz;
+    }
+}
+
+// Group sums of the partials and the polarisation epilogue (FP64).
+template <int FT, int L>
+__device__ __forceinline__ void force32_epilogue(float (&s)[12], double ds, double &ox, double &oy, double &oz)
+{
+    const float fx = gsum<L>(s[0]), fy = gsum<L>(s[1]), fz = gsum<L>(s[2]);
+    if constexpr (FT == 4) {
+        const float ex = gsum<L>(s[3]), ey = gsum<L>(s[4]), ez = gsum<L>(s[5]);
+        const float sxx = gsum<L>(s[6]), syy = gsum<L>(s[7]), tr = gsum<L>(s[11]);
+        const float sxy = gsum<L>(s[8]), sxz = gsum<L>(s[9]), syz = gsum<L>(s[10]);
         const double t = tr, Sxx = sxx, Syy = syy;
         const double Txx = 3.0 * Sxx - t, Tyy = 3.0 * Syy - t, Tzz = 2.0 * t - 3.0 * Sxx - 3.0 * Syy;
         const double Txy = 3.0 * sxy, Txz = 3.0 * sxz, Tyz = 3.0 * syz;
@@ -223,6 +225,59 @@ __device__ __forceinline__ void grp_force32(P4 xq, P2 ab, int npad, int j, float
     } else {
         ox = fx; oy = fy; oz = fz;
     }
+}
+
+// Atoms from LDS (latency of a few dozen cycles: loaded where used).
+template <int FT, int L, typename P4, typename P2>
+__device__ __forceinline__ void grp_force32(P4 xq, P2 ab, int npad, int j, float px, float py, float pz, double ds,
+                                            double &ox, double &oy, double &oz)
+{
+    float s[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) s[k] = 0.0f;
+#pragma unroll 4
+    for (int kk = 0; kk < npad; kk += L) pair32<FT>(xq[kk + j], ab[kk + j], px, py, pz, s);
+    force32_epilogue<FT, L>(s, ds, ox, oy, oz);
+}
+
+// Atoms from global memory (L2, hundreds of cycles): kSlots pairs are in
+// flight per lane.  Slot i holds pair kk + i L; as soon as its pair is added
+// the slot is reloaded with the pair kSlots further on, so each load has the
+// arithmetic of kSlots - 1 pairs to land and the slots never move between
+// registers.  npad is a multiple of 64 and so of L: a slot past the end is
+// skipped (the test is the same for the whole wavefront) and its loads
+// re-read the lane's last pair (in bounds, unused).  Same pairs in the same
+// order as grp_force32.
+constexpr int kSlots = 3;
+
+template <int FT, int L>
+__device__ __forceinline__ void grp_force32_glb(const GLB f4n *xq, const GLB f2n *ab, int npad, int j, float px,
+                                                float py, float pz, double ds, double &ox, double &oy, double &oz)
+{
+    float s[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) s[k] = 0.0f;
+    if (npad > 0) {
+        const int last = npad - L + j;
+        f4n a[kSlots];
+        f2n c[kSlots];
+#pragma unroll
+        for (int i = 0; i < kSlots; i++) {
+            const int k = min(i * L + j, last);
+            a[i] = xq[k];
+            c[i] = ab[k];
+        }
+        for (int kk = 0; kk < npad; kk += kSlots * L) {
+#pragma unroll
+            for (int i = 0; i < kSlots; i++) {
+                if (kk + i * L < npad) pair32<FT>(a[i], c[i], px, py, pz, s);
+                const int kn = min(kk + (i + kSlots) * L + j, last);
+                a[i] = xq[kn];
+                c[i] = ab[kn];
+            }
+        }
+    }
+    force32_epilogue<FT, L>(s, ds, ox, oy, oz);
 }
 #pragma clang fp contract(off)
 
@@ -713,8 +768,8 @@ struct TableSource {
         if constexpr (FT == 4 || FT == 6) {
             const int npad = active ? r.npad : 0;
             if constexpr (PREC == 1)
-                grp_force32<FT, L>(glb((const f4n *)r.xq), glb((const f2n *)r.se), npad, j, (float)ex, (float)ey,
-                                   (float)ez, r.m.dipole_scalar, fx, fy, fz);
+                grp_force32_glb<FT, L>(glb((const f4n *)r.xq), glb((const f2n *)r.se), npad, j, (float)ex, (float)ey,
+                                       (float)ez, r.m.dipole_scalar, fx, fy, fz);
             else
                 grp_force64<FT, L>(glb(r.soa), npad, j, ex, ey, ez, r.m.dipole_scalar, fx, fy, fz);
         } else if constexpr (L == 64) {
