@@ -91,7 +91,11 @@ def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, **server_kw
         return E.default_params(temp=reqs[k]["temp"], seed=seed + k, multipole=0, dispersion_cutoff=0, setup_gpu=1)
     # launch shape: lanes per trajectory by molecule size (the size rule), fixed for every request
     W = 512
-    log(f"serving: {n_requests} requests at {rate}/s, target {target}, lanes by size, wg {W}")
+    # first issue sized for a per-trajectory relative spread of 2.0 (the spread of the CCS terms runs
+    # 2.3-2.9 on this stream), then the whole estimate of what is still needed each slice
+    server_kw.setdefault("prior_rel_sd", 2.0)
+    server_kw.setdefault("issue_fraction", 1.0)
+    log(f"serving: {n_requests} requests at {rate}/s, target {target}, lanes by size, wg {W}, {server_kw}")
     srv = E.Server(prec=E.PREC_MIXED, lanes=0, wg_size=W, block_traj=10000, min_blocks=3, max_blocks=1000,
                    policy=policy, **server_kw)
     t0 = time.perf_counter()
@@ -124,6 +128,7 @@ def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, **server_kw
         "rel_err_max": max(r["rel_err"] for r in res), "blocks_mean": float(np.mean([r["blocks"] for r in res])),
         "slices": st["slices"], "kernel_s": st["kernel_ms"] / 1e3, "busy_s": st["busy_ms"] / 1e3,
         "lanes_per_trajectory": "by size class (4 x 2^class)", "wg_size": W,
+        "prior_rel_sd": server_kw["prior_rel_sd"], "issue_fraction": server_kw["issue_fraction"],
         "policy": "shortest remaining work first" if policy == E.SERVE_SHORTEST_FIRST else "equal share",
     }
 

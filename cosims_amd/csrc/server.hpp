@@ -125,15 +125,22 @@ void setup_worker(cosims_server *S)
     }
 }
 
-// Blocks job J still needs beyond those already issued: min_blocks first,
-// then as many as reach the target if the standard error keeps falling as
-// 1/sqrt(blocks).  0 while the issued blocks may suffice.
+// Blocks job J still needs beyond those already issued: min_blocks (or the
+// prior's estimate) first, then as many as reach the target if the standard
+// error keeps falling as 1/sqrt(blocks).  0 while the issued blocks may
+// suffice.
 int blocks_wanted(const cosims_server *S, const SJob &J)
 {
     const auto &c = S->cfg;
     int want;
     if (J.used < c.min_blocks) {
-        want = c.min_blocks - J.issued;
+        int first = c.min_blocks;
+        if (c.prior_rel_sd > 0) {
+            const double r = c.prior_rel_sd / J.target;
+            const double f = std::ceil(r * r / (double)c.block_traj - 1e-9);
+            first = (int)std::min<double>(std::max<double>(first, f), c.max_blocks);
+        }
+        want = std::min(first, c.max_blocks) - J.issued;
     } else {
         cosims_ccs tmp{};
         final_stats(J.omega.data(), J.omega_err.data(), J.sums.data(), J.used, c.block_traj, &tmp);
@@ -182,10 +189,12 @@ std::vector<SliceEntry> plan_slice(cosims_server *S, const std::vector<std::shar
     std::vector<size_t> order;
     for (size_t k = 0; k < act.size(); k++) {
         want[k] = blocks_wanted(S, *act[k]);
-        // the estimate of what is still needed is noisy until many blocks are
-        // in: issue at most half of it (rounded up) per slice, so blocks past
-        // the stopping point stay few
-        if (act[k]->used >= c.min_blocks) want[k] = (want[k] + 1) / 2;
+        // once the stopping rule applies, issue_fraction of the estimate of
+        // what is still needed (rounded up): 1 takes the estimate whole (the
+        // fewest round trips per request), smaller values keep the blocks past
+        // the stopping point fewer when the estimate is noisy
+        if (act[k]->used >= c.min_blocks && want[k] > 0)
+            want[k] = std::max(1, (int)std::ceil(want[k] * c.issue_fraction - 1e-9));
         cost[k] = block_cost(S, *act[k]);
         if (want[k] > 0) order.push_back(k);
     }
@@ -479,6 +488,9 @@ int cosims_server_create(const cosims_server_cfg *cfg, cosims_server **out)
     if (c.max_blocks <= 0) c.max_blocks = 1000;
     if (c.min_blocks > c.max_blocks) return fail(COSIMS_E_INVALID, "min_blocks > max_blocks");
     if (c.setup_threads <= 0) c.setup_threads = 8;
+    if (c.issue_fraction == 0) c.issue_fraction = 1.0;
+    if (!(c.prior_rel_sd >= 0 && c.prior_rel_sd < 1e3)) return fail(COSIMS_E_INVALID, "prior_rel_sd must be in [0, 1000)");
+    if (!(c.issue_fraction > 0 && c.issue_fraction <= 1)) return fail(COSIMS_E_INVALID, "issue_fraction must be in (0, 1]");
     Device d;
     if (int rc = device_init(d)) return rc;
     std::unique_ptr<cosims_server> S(new cosims_server());

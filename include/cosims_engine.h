@@ -315,6 +315,12 @@ typedef struct cosims_server_cfg {
     int32_t policy;           /* COSIMS_SERVE_* */
     int64_t min_slice_traj;   /* shortest-first slices are topped up to this many; 0 => 65536 */
     int32_t pipeline_depth;   /* slices in flight at once (1..8); 0 => 3 */
+    double issue_fraction;    /* once min_blocks are in: share of the estimated remaining blocks
+                                 issued per slice (0..1]; 0 => 1 (all of the estimate) */
+    double prior_rel_sd;      /* before any block is in, issue the blocks a relative spread of the
+                                 per-trajectory CCS terms this large would need for the target
+                                 ((prior_rel_sd / target)^2 trajectories, at least min_blocks);
+                                 0 => min_blocks only */
 } cosims_server_cfg;
 
 /* Slice composition: shortest remaining work first (requests ordered by

@@ -102,6 +102,29 @@ def test_server_policy_does_not_change_results(mixed_results):
         assert np.array_equal(r["omega"], ref["omega"])
 
 
+@pytest.mark.parametrize("issue", [dict(issue_fraction=0.5), dict(prior_rel_sd=2.0),
+                                   dict(issue_fraction=0.5, prior_rel_sd=3.0)])
+def test_server_issue_policy_does_not_change_results(mixed_results, issue):
+    """How many blocks a slice hands a request (half or all of the estimate,
+    a prior-sized first issue) changes only the blocks past the stopping
+    point, never the blocks taken or the result."""
+    res, _ = mixed_results
+    with _server(**issue) as srv:
+        tickets = _submit_all(srv, REQUESTS)
+        out = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    for r, ref in zip(out, res):
+        assert r["blocks"] == ref["blocks"]
+        assert np.array_equal(r["omega"], ref["omega"])
+    assert st["traj"] >= sum(r["n_traj"] for r in out)
+
+
+def test_server_rejects_bad_issue_settings():
+    for kw in (dict(issue_fraction=1.5), dict(issue_fraction=-0.1), dict(prior_rel_sd=-1.0)):
+        with pytest.raises(E.CosimsError, match="invalid argument"):
+            _server(**kw)
+
+
 def test_server_budget_serialises_admission(mixed_results):
     """A one-byte device-memory budget admits one request at a time; the
     results do not change."""
