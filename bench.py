@@ -75,7 +75,7 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a, dtype=float), q)) if len(a) else None
 
 
-def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, **server_kw):
+def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, fp64_max_atoms=0, **server_kw):
     """Poisson request stream through the CCS server; latency = completion -
     submission on the server's clock (setup, queueing and trajectories)."""
     import tempfile
@@ -113,6 +113,23 @@ def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, **server_kw
             f"CCS {r['wmean']:.1f} +- {100 * r['rel_err']:.2f}%  latency {r['t_done_ms'] - r['t_submit_ms']:.0f} ms")
     st = srv.stats()
     srv.close()
+    vs64 = None
+    if fp64_max_atoms > 0:
+        # CCS error against FP64: the same blocks of the request's stream (cosims_run over
+        # iter = blocks, the request's lanes) in FP64, for the requests of up to fp64_max_atoms atoms
+        dev = []
+        for k, r in enumerate(res):
+            if reqs[k]["n"] > fp64_max_atoms:
+                continue
+            m = E.Molecule.from_file(os.path.join(d, f"req{k}.mfj"), params=params(k))
+            f = m.run(E.PREC_FP64, job=k, iters=r["blocks"], traj=10000, lanes=r["lanes"], wg_size=W)
+            dev.append((r["wmean"] - f["wmean"]) / f["wmean"])
+            m.close()
+        dev = np.array(dev)
+        vs64 = {"requests": int(dev.size), "max_atoms": fp64_max_atoms, "mean_rel_dev": float(dev.mean()),
+                "mean_rel_dev_se": float(dev.std(ddof=1) / np.sqrt(dev.size)) if dev.size > 1 else None,
+                "max_abs_rel_dev": float(np.abs(dev).max())}
+        log(f"serving vs FP64: {dev.size} requests, mean rel dev {dev.mean():.2e}, max |dev| {np.abs(dev).max():.2e}")
     lat = [r["t_done_ms"] - r["t_submit_ms"] for r in res]
     makespan = (max(r["t_done_ms"] for r in res) - min(r["t_submit_ms"] for r in res)) / 1e3
     traj = sum(r["n_traj"] for r in res)
@@ -130,10 +147,11 @@ def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, **server_kw
         "lanes_per_trajectory": "by size class (4 x 2^class)", "wg_size": W,
         "prior_rel_sd": server_kw["prior_rel_sd"], "issue_fraction": server_kw["issue_fraction"],
         "policy": "shortest remaining work first" if policy == E.SERVE_SHORTEST_FIRST else "equal share",
+        "ccs_vs_fp64": vs64,
     }
 
 
-def batch_workload(E, synth, count, traj, steps):
+def batch_workload(E, synth, count, traj, steps, fp64_every=16):
     """256 synthetic conformers (N log-uniform 50-5,000), 50,000 trajectories
     each, all in shared launches (one per launch shape); per-size-class
     launch shapes from the batch tuner, tuned before the timing."""
@@ -170,6 +188,18 @@ def batch_workload(E, synth, count, traj, steps):
     out["workload"] = (f"{count} synthetic conformers, N log-uniform 50-5,000 (total {sum(n_atoms)} atoms), "
                        f"q round(N/250)+1 e, He 298 K, {traj} trajectories each, mixed precision")
     out["trajectories"] = count * traj
+    # CCS error against FP64 on paired initial conditions (same jobs and streams), on every
+    # fp64_every-th conformer; `res` holds the size-rule mixed results of the last timed batch
+    sub = list(range(0, count, fp64_every))
+    f64, _ = E.run_batch([jobs[k] for k in sub], prec=E.PREC_FP64, lanes=0)
+    dev = np.array([(res[k]["wmean"] - f["wmean"]) / f["wmean"] for k, f in zip(sub, f64)])
+    rse = np.array([f["werr"] / f["wmean"] for f in f64])
+    out["ccs_vs_fp64"] = {"conformers": len(sub), "mean_rel_dev": float(dev.mean()),
+                          "mean_rel_dev_se": float(dev.std(ddof=1) / np.sqrt(dev.size)) if dev.size > 1 else None,
+                          "max_abs_rel_dev": float(np.abs(dev).max()), "median_rel_se_per_job": float(np.median(rse)),
+                          "sample": f"every {fp64_every}th conformer, size-rule lanes, FP64 group kernels"}
+    log(f"batch vs FP64: {len(sub)} conformers, mean rel dev {dev.mean():.2e}, max |dev| {np.abs(dev).max():.2e} "
+        f"(per-job rel SE {np.median(rse):.2e})")
     for m in mols:
         m.close()
     return out
@@ -269,6 +299,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=40, help="requests of the serving stream (0: none)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-target", type=float, default=0.005)
+    ap.add_argument("--serve-fp64-atoms", type=int, default=1000,
+                    help="rerun served requests of up to this many atoms in FP64 over the same blocks (0: none)")
     ap.add_argument("--serve-compare", type=int, default=1,
                     help="also run the stream with equal-share slices (1) or not (0)")
     ap.add_argument("--workload", choices=("headline", "batch", "clustered"), default="headline",
@@ -381,7 +413,7 @@ def main():
         out["cpu_baseline"] = cpu_baseline(m, a.cpu_seconds, threads)
     out["served_latency_ms"] = None
     if a.serve_requests > 0:
-        sv = serving(E, synth, a.serve_requests, a.serve_rate, a.serve_target)
+        sv = serving(E, synth, a.serve_requests, a.serve_rate, a.serve_target, fp64_max_atoms=a.serve_fp64_atoms)
         out["serving"] = sv
         out["served_latency_ms"] = {"p50": sv["latency_ms"]["p50"], "p99": sv["latency_ms"]["p99"]}
         if a.serve_compare:

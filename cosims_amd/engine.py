@@ -507,7 +507,7 @@ class Server:
         _check(lib().cosims_server_wait(self._h, ticket, timeout_ms, C.byref(res), _p(om), _p(err), self.max_blocks))
         out = {k: getattr(res, k) for k, _ in Result._fields_ if k != "ccs"}
         out.update({k: getattr(res.ccs, k) for k in ("mean", "sd", "wmean", "werr", "max_ergy", "n_traj", "n_hit",
-                                                     "n_fail", "total_steps")})
+                                                     "n_fail", "total_steps", "lanes", "wg_size")})
         out.update(omega=om[: res.blocks].copy(), omega_err=err[: res.blocks].copy())
         return out
 
