@@ -1753,8 +1753,9 @@ int autotune(const cosims_mol *M, int prec, int64_t n_hint, int &L_out, int &wg_
 namespace {
 
 // ------------------------------------------------------------- batches ---
-constexpr int kMaxSegs = 64;      // launches of one batch (force type x launch shape)
-constexpr int kSegStreams = 4;    // streams the launches of a batch are spread over
+constexpr int kMaxSegs = 64;        // launch shapes of one batch (force type x lanes x workgroup size)
+constexpr int kMaxLaunches = 4096;  // kernel launches of one batch
+constexpr int kSegStreams = 4;      // streams the launches of a batch are spread over
 
 // Lanes per trajectory from the units one force evaluation sweeps (atoms,
 // or clusters): as many as leave each lane at least 48 units per step, from
@@ -1791,15 +1792,16 @@ struct BatchPlan {
     std::vector<int> order;              // job indices in work-list order
     struct Seg {
         int ft, L, W;
-        int64_t reserved, traj;          // trajectories of this launch (its hits: on the device, seg_end)
-        double work;                     // trajectories x units, for the launch order
+        int i0, i1;                      // its jobs: work-list positions order[i0 .. i1)
+        int64_t traj;                    // trajectories of its jobs (their hits: on the device, job_end)
     };
     std::vector<Seg> segs;
     int64_t n = 0;
     int cus = 0;
+    int nlaunch = 0;                     // launches of the last launch_enqueue
     TrajBufs B;
-    DBuf tmol, table, seg_ctr, seg_evals, seg_end;
-    int64_t cap = 0, table_cap = 0;      // allocated trajectories / table entries (reused)
+    DBuf tmol, table, seg_ctr, seg_evals, job_end;
+    int64_t cap = 0, table_cap = 0, job_end_cap = 0;  // allocated trajectories / table entries / bounds (reused)
     float sample_ms = 0;
     std::vector<MolRec> recs;            // host side of the table upload
     std::vector<hipStream_t> streams;    // default priority, made on first use
@@ -1901,7 +1903,10 @@ int stage_batch(BatchPlan &P, hipStream_t s)
         HIP_CHECK(P.table.alloc(sizeof(MolRec) * nj));
         P.table_cap = nj;
     }
-    if (!P.seg_end.p) HIP_CHECK(P.seg_end.alloc(sizeof(int32_t) * kMaxSegs));
+    if (P.job_end_cap < nj) {
+        HIP_CHECK(P.job_end.alloc(sizeof(int32_t) * nj));
+        P.job_end_cap = nj;
+    }
     // the host copy lives on the plan: the upload is asynchronous
     P.recs.resize(nj);
     for (int k = 0; k < nj; k++) P.recs[k] = P.jobs[k].mol->rec;
@@ -1914,8 +1919,8 @@ int stage_batch(BatchPlan &P, hipStream_t s)
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ctr, 64, 0);
     HIP_CHECK(hipEventRecord(P.es0, s));
     P.segs.clear();
-    double seg_work = 0;
     int64_t seg_traj = 0;
+    int seg_i0 = 0;
     for (size_t i = 0; i < P.order.size(); i++) {
         const int k = P.order[i];
         const BatchPlan::Job &J = P.jobs[k];
@@ -1928,18 +1933,17 @@ int stage_batch(BatchPlan &P, hipStream_t s)
         A.items = P.B.items[0].as<int32_t>(); A.n_items = ctr; A.item_base = (int32_t)J.off;
         hipLaunchKernelGGL(k_sample, dim3(nb), dim3(256), 0, s, A);
         HIP_CHECK(hipGetLastError());
-        seg_work += (double)J.n * J.units;
+        // this job's hits end where the item counter stands now; the bound
+        // stays on the device (no host wait here)
+        hipLaunchKernelGGL(k_copy_i32, dim3(1), dim3(1), 0, s, ctr, P.job_end.as<int32_t>() + i);
         seg_traj += J.n;
         const bool last = i + 1 == P.order.size();
         const BatchPlan::Job *nx = last ? nullptr : &P.jobs[P.order[i + 1]];
         if (last || nx->ft != J.ft || nx->L != J.L || nx->W != J.W) {
-            // the launch's work list ends where the item counter stands now;
-            // the bound stays on the device (no host wait here)
             if ((int)P.segs.size() == kMaxSegs) return fail(COSIMS_E_INVALID, "batch: too many distinct launch shapes");
-            hipLaunchKernelGGL(k_copy_i32, dim3(1), dim3(1), 0, s, ctr, P.seg_end.as<int32_t>() + P.segs.size());
-            P.segs.push_back({J.ft, J.L, J.W, 0, seg_traj, seg_work});
-            seg_work = 0;
+            P.segs.push_back({J.ft, J.L, J.W, seg_i0, (int)i + 1, seg_traj});
             seg_traj = 0;
+            seg_i0 = (int)i + 1;
         }
     }
     HIP_CHECK(hipGetLastError());
@@ -1947,20 +1951,58 @@ int stage_batch(BatchPlan &P, hipStream_t s)
     return 0;
 }
 
-// The launches of a staged plan, spread over kSegStreams streams so that
-// one launch's tail overlaps the next; the biggest launch goes first.
+// The launches of a staged plan.  A job of an exact force type whose
+// molecule fits one workgroup's LDS runs the LDS-staged kernel, one launch per
+// job (its molecule read from LDS instead of through L2: 1.3-1.6x the pair
+// rate of the table kernel); the other jobs of a launch shape, in consecutive
+// runs of the work list, share table-kernel launches.  Both kernels give the
+// same bits for a trajectory.  Launches are spread over kSegStreams streams,
+// the most work first, so one launch's tail overlaps the next ones.
 // launch_enqueue only enqueues (the stream s carries the join; kernels run
 // on the plan's own streams); launch_finish waits for them and adds their
 // time (first launch to last one's end) and counters to tot.
 int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches)
 {
     launches = 0;
-    const int nseg = (int)P.segs.size();
-    if (!P.seg_ctr.p) {
-        HIP_CHECK(P.seg_ctr.alloc(sizeof(int32_t) * kMaxSegs));
-        HIP_CHECK(P.seg_evals.alloc(sizeof(unsigned long long) * 4 * kMaxSegs));
+    struct Launch {
+        int seg, i0, i1;  // work-list positions order[i0 .. i1)
+        bool lds;
+        int64_t traj;
+        double work;
+    };
+    std::vector<Launch> plan;
+    for (int g = 0; g < (int)P.segs.size(); g++) {
+        const auto &sg = P.segs[g];
+        int run0 = -1;
+        int64_t run_traj = 0;
+        double run_work = 0;
+        for (int i = sg.i0; i <= sg.i1; i++) {
+            const BatchPlan::Job *J = i < sg.i1 ? &P.jobs[P.order[i]] : nullptr;
+            const bool lds = J && lds_mode(J->mol, prec) && group_kernel(prec, J->ft, J->L);
+            if ((!J || lds) && run0 >= 0) {  // close the run of table-kernel jobs
+                plan.push_back({g, run0, i, false, run_traj, run_work});
+                run0 = -1;
+                run_traj = 0;
+                run_work = 0;
+            }
+            if (!J) break;
+            if (lds) {
+                plan.push_back({g, i, i + 1, true, J->n, (double)J->n * J->units});
+            } else {
+                if (run0 < 0) run0 = i;
+                run_traj += J->n;
+                run_work += (double)J->n * J->units;
+            }
+        }
     }
-    while ((int)P.streams.size() < std::min(nseg, kSegStreams)) {
+    const int nl = (int)plan.size();
+    if (nl > kMaxLaunches) return fail(COSIMS_E_INVALID, "batch: too many launches");
+    P.nlaunch = nl;
+    if (!P.seg_ctr.p) {
+        HIP_CHECK(P.seg_ctr.alloc(sizeof(int32_t) * kMaxLaunches));
+        HIP_CHECK(P.seg_evals.alloc(sizeof(unsigned long long) * 4 * kMaxLaunches));
+    }
+    while ((int)P.streams.size() < std::min(nl, kSegStreams)) {
         hipStream_t x;
         HIP_CHECK(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
         P.streams.push_back(x);
@@ -1971,35 +2013,60 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
     }
     for (hipEvent_t j : P.joins) (void)hipEventDestroy(j);
     P.joins.clear();
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, P.seg_ctr.as<int32_t>(), kMaxSegs, 0);
-    hipLaunchKernelGGL(k_fill_i32, dim3(8), dim3(64), 0, s, P.seg_evals.as<int32_t>(), 8 * kMaxSegs, 0);
-    std::vector<int> order(nseg);
-    for (int i = 0; i < nseg; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return P.segs[a].work > P.segs[b].work; });
+    if (nl > 0) {
+        const int nc = std::max(1, (nl + 63) / 64);
+        hipLaunchKernelGGL(k_fill_i32, dim3(nc), dim3(64), 0, s, P.seg_ctr.as<int32_t>(), (int64_t)nl, 0);
+        hipLaunchKernelGGL(k_fill_i32, dim3(8 * nc), dim3(64), 0, s, P.seg_evals.as<int32_t>(), (int64_t)8 * nl, 0);
+    }
+    std::vector<int> order(nl);
+    for (int i = 0; i < nl; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return plan[a].work > plan[b].work; });
     HIP_CHECK(hipEventRecord(P.ev0, s));
-    for (int q = 0; q < nseg; q++) {
-        const int i = order[q];
-        const auto &g = P.segs[i];
-        const int cnt = (int)g.traj;  // upper bound of the launch's work list (its hits)
+    const int32_t *ends = P.job_end.as<int32_t>();
+    for (int q = 0; q < nl; q++) {
+        const int li = order[q];
+        const Launch &l = plan[li];
+        const auto &g = P.segs[l.seg];
+        const int cnt = (int)l.traj;  // upper bound of the launch's work list (its hits)
         if (cnt <= 0) continue;
-        MultiFn fn = multi_kernel(prec, g.ft, g.L);
-        if (!fn) return fail(COSIMS_E_INVALID, "no table kernel for this force type / lane count");
         hipStream_t x = P.streams[launches % P.streams.size()];
         HIP_CHECK(hipStreamWaitEvent(x, P.ev0, 0));
-        MultiArgs G{};
-        G.io = traj_io(P.B, P.B.items[0].as<int32_t>(), cnt);
-        G.io.counter = P.seg_ctr.as<int32_t>() + i;
-        G.io.evals = P.seg_evals.as<unsigned long long>() + 4 * i;
-        G.mols = P.table.as<MolRec>();
-        G.tmol = P.tmol.as<int32_t>();
-        G.first = i ? P.seg_end.as<int32_t>() + i - 1 : nullptr;
-        G.last = P.seg_end.as<int32_t>() + i;
-        const size_t lds = multi_lds_bytes(g.ft, g.L, g.W);
-        const int64_t grid = group_grid((const void *)fn, g.W, lds, bpc, P.cus, cnt, g.L);
-        hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(g.W), lds, x, G);
+        TrajIO io = traj_io(P.B, P.B.items[0].as<int32_t>(), cnt);
+        io.counter = P.seg_ctr.as<int32_t>() + li;
+        io.evals = P.seg_evals.as<unsigned long long>() + 4 * li;
+        const int32_t *first = l.i0 ? ends + l.i0 - 1 : nullptr;
+        const int32_t *last = ends + l.i1 - 1;
+        if (l.lds) {
+            const cosims_mol *M = P.jobs[P.order[l.i0]].mol;
+            GrpFn fn = group_kernel(prec, g.ft, g.L);
+            const size_t lds = group_lds_bytes(M, prec);
+            HIP_CHECK(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            GroupArgs G{};
+            G.io = io;
+            G.m = M->md;
+            G.npad = M->npad;
+            G.soa = M->soa;
+            G.dt0 = M->params.dt;
+            G.first = first;
+            G.last = last;
+            const int64_t grid = group_grid((const void *)fn, g.W, lds, bpc, P.cus, cnt, g.L);
+            hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(g.W), lds, x, G);
+        } else {
+            MultiFn fn = multi_kernel(prec, g.ft, g.L);
+            if (!fn) return fail(COSIMS_E_INVALID, "no table kernel for this force type / lane count");
+            MultiArgs G{};
+            G.io = io;
+            G.mols = P.table.as<MolRec>();
+            G.tmol = P.tmol.as<int32_t>();
+            G.first = first;
+            G.last = last;
+            const size_t lds = multi_lds_bytes(g.ft, g.L, g.W);
+            const int64_t grid = group_grid((const void *)fn, g.W, lds, bpc, P.cus, cnt, g.L);
+            hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(g.W), lds, x, G);
+        }
         HIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_chi, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, x, G.io.items, cnt, G.io.status,
-                           G.io.cchi, G.io.chi, G.first, G.last);
+        hipLaunchKernelGGL(k_chi, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, x, io.items, cnt, io.status,
+                           io.cchi, io.chi, first, last);
         HIP_CHECK(hipGetLastError());
         hipEvent_t j;
         HIP_CHECK(hipEventCreateWithFlags(&j, hipEventDisableTiming));
@@ -2014,9 +2081,11 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
 
 int launch_finish(BatchPlan &P, hipStream_t s, GroupRun &tot)
 {
-    const int nseg = (int)P.segs.size();
-    std::vector<unsigned long long> ev(4 * kMaxSegs);
-    HIP_CHECK(hipMemcpyAsync(ev.data(), P.seg_evals.p, sizeof(unsigned long long) * ev.size(), hipMemcpyDeviceToHost, s));
+    const int nseg = P.nlaunch;
+    std::vector<unsigned long long> ev(4 * (size_t)std::max(1, nseg));
+    if (nseg > 0)
+        HIP_CHECK(hipMemcpyAsync(ev.data(), P.seg_evals.p, sizeof(unsigned long long) * 4 * nseg,
+                                 hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     float ms = 0;
     HIP_CHECK(hipEventElapsedTime(&ms, P.ev0, P.ev1));
@@ -2057,10 +2126,22 @@ int tune_batch(const cosims_batch_job *jobs, int32_t njobs, int prec, ShapeTable
         if (by_class[c].empty()) continue;
         int units = 0;  // largest molecule of the class: the lane counts worth trying
         for (const auto &j : by_class[c]) units = std::max(units, group_units(j.mol->prep));
-        // tuning subsets: at most 64 jobs of the class, 128 / 512 trajectories each (32 to screen)
-        std::vector<cosims_batch_job> sub(by_class[c].begin(),
-                                          by_class[c].begin() + std::min<size_t>(64, by_class[c].size()));
-        const int64_t caps[3] = {128, 512, 32};
+        // tuning subsets: at most 8 jobs spread over the class, each with 1/8 (search), 1/2
+        // (finalists) and 1/32 (screen) of the fewest trajectories a job of the subset has.
+        // Jobs that fit LDS run one launch each, so the sets must be launch-sized like the
+        // batch's: on small sets the end-of-launch tail would decide the shape.
+        const size_t m = by_class[c].size(), take = std::min<size_t>(8, m);
+        std::vector<cosims_batch_job> sub;
+        int64_t T = INT64_MAX;
+        for (size_t i = 0; i < take; i++) {
+            const cosims_batch_job &j = by_class[c][i * m / take];
+            sub.push_back(j);
+            const int64_t it = j.iter > 0 ? j.iter : j.mol->params.iter;
+            const int64_t tr = j.traj > 0 ? j.traj : j.mol->params.traj;
+            T = std::min<int64_t>(T, std::max<int64_t>(1, it) * std::max<int64_t>(1, tr));
+        }
+        auto cap = [&](int64_t d) { return std::min<int64_t>(65536, std::max<int64_t>(256, T / d)); };
+        const int64_t caps[3] = {cap(8), cap(2), cap(32)};
         BatchPlan set;
         auto run = [&](int k, int L, int W, double &ms) -> int {
             ShapeTable fixed;

@@ -62,6 +62,9 @@ struct GroupArgs {
     int npad;                              // atoms incl. padding, multiple of 64
     const double *soa;                     // global FP64 SoA: x y z q s6 e24 e4, npad each
     double dt0;
+    // when set, the work list is io.items[*first .. *last) (first NULL: from
+    // 0), bounds written on the device before the launch runs
+    const int32_t *first, *last;
 };
 
 // One molecule of a multi-molecule launch.
@@ -987,8 +990,10 @@ __global__ void __launch_bounds__(1024) k_traj_grp(GroupArgs A)
         for (int i = threadIdx.x; i < 6 * npad; i += blockDim.x) X[i] = S[i];
     }
     __syncthreads();
+    TrajIO io = A.io;
+    resolve_range(io, A.first, A.last);
     LdsSource<PREC, FT, L> src{A, smem};
-    group_loop<L>(A.io, src);
+    group_loop<L>(io, src);
 }
 
 // Dynamic LDS of k_traj_multi: the cluster lists of each wavefront when

@@ -91,7 +91,8 @@ def test_batch_of_one_equals_run(ion128, prec):
 
 def test_batch_alone_equals_mix(ion128, ion2000):
     """Uneven molecules and two force types in one call: each job's result
-    is the same alone and in the mix."""
+    is the same alone and in the mix.  The exact-type jobs fit LDS and run
+    one LDS-kernel launch each; the clustered job runs the table kernel."""
     g = golden("acau_charged")
     ac1 = golden_engine_mol(g, [1, 1, 1, 1])   # clustered, type 1
     ac4 = golden_engine_mol(g, [1, 0, 0, 1])   # exact, type 4
@@ -100,7 +101,7 @@ def test_batch_alone_equals_mix(ion128, ion2000):
             (ion128, dict(job=5, iters=1, traj=1000))]
     shape = dict(lanes=16, wg_size=512)
     mix, st = E.run_batch(jobs, prec=E.PREC_MIXED, **shape)
-    assert st["launches"] == 2
+    assert st["launches"] == 5
     assert st["n_traj"] == sum(j[1].get("iters", j[0].params.iter) * j[1].get("traj", j[0].params.traj)
                                for j in jobs)
     for k, job in enumerate(jobs):
@@ -169,3 +170,21 @@ def test_large_molecule_table_kernel(ion8000):
     assert np.median(d) <= 1e-8, np.median(d)
     np.testing.assert_allclose(grp["omega"], lane["omega"], rtol=5e-4)
     assert abs(mix["wmean"] - lane["wmean"]) / lane["wmean"] <= 1e-3
+
+
+def test_batch_mixes_table_and_lds_launches(ion8000, ion128, ion2000):
+    """One launch shape, molecules on both sides of the LDS limit: the
+    8,000-atom job runs the table kernel, the others one LDS launch each;
+    every result equals the job alone and cosims_run with the same lanes."""
+    shape = dict(lanes=16, wg_size=256)
+    jobs = [(ion128, dict(job=41, iters=1, traj=1500)), (ion8000, dict(job=42, iters=1, traj=600)),
+            (ion2000, dict(job=43, iters=2, traj=800))]
+    for prec in (E.PREC_MIXED, E.PREC_FP64):
+        mix, st = E.run_batch(jobs, prec=prec, **shape)
+        assert st["launches"] == 3
+        for k, job in enumerate(jobs):
+            (alone,), _ = E.run_batch([job], prec=prec, **shape)
+            assert np.array_equal(mix[k]["omega"], alone["omega"]), (prec, k)
+            single = job[0].run(prec, job=job[1]["job"], iters=job[1]["iters"], traj=job[1]["traj"], **shape)
+            assert np.array_equal(mix[k]["omega"], single["omega"]), (prec, k)
+            assert mix[k]["total_steps"] == single["total_steps"]
