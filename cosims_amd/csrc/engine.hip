@@ -1395,6 +1395,26 @@ GrpFn group_kernel(int prec, int ft, int L)
 }
 
 template <int P, int FT>
+GrpFn tile_fn_l(int L)
+{
+    switch (L) {
+    case 4: return k_traj_tile<P, FT, 4>;
+    case 8: return k_traj_tile<P, FT, 8>;
+    case 16: return k_traj_tile<P, FT, 16>;
+    case 32: return k_traj_tile<P, FT, 32>;
+    case 64: return k_traj_tile<P, FT, 64>;
+    default: return nullptr;
+    }
+}
+
+GrpFn tile_kernel(int prec, int ft, int L)
+{
+    if (ft == 4) return prec == COSIMS_PREC_FP64 ? tile_fn_l<0, 4>(L) : tile_fn_l<1, 4>(L);
+    if (ft == 6) return prec == COSIMS_PREC_FP64 ? tile_fn_l<0, 6>(L) : tile_fn_l<1, 6>(L);
+    return nullptr;
+}
+
+template <int P, int FT>
 MultiFn multi_fn_l(int L)
 {
     switch (L) {
@@ -1434,11 +1454,44 @@ size_t group_lds_bytes(const cosims_mol *M, int prec)
 }
 
 // The LDS-staged kernel serves the exact force types when the molecule fits
-// one workgroup's LDS; everything else runs the table kernel.
+// one workgroup's LDS, the tiled kernel the larger ones; the clustered types
+// run the table kernel.
 bool lds_mode(const cosims_mol *M, int prec)
 {
     const int ft = M->prep.force_type;
     return (ft == 4 || ft == 6) && group_lds_bytes(M, prec) <= M->dev.lds_max;
+}
+bool tile_mode(const cosims_mol *M, int prec)
+{
+    const int ft = M->prep.force_type;
+    return (ft == 4 || ft == 6) && !lds_mode(M, prec);
+}
+
+// Atoms per LDS tile of k_traj_tile at workgroup size wg: the share of a
+// CU's LDS that leaves room for 1024 / wg workgroups, a multiple of 64.
+int tile_atoms(const cosims_mol *M, int prec, int wg)
+{
+    const size_t per_atom = prec == COSIMS_PREC_FP64 ? 6 * sizeof(double) : sizeof(float4) + sizeof(float2);
+    const size_t budget = (size_t)M->dev.lds_max * (size_t)wg / 1024;
+    const int t = (int)(budget / per_atom) / 64 * 64;
+    return std::max(64, std::min(t, M->npad));
+}
+size_t tile_lds_bytes(int prec, int tile)
+{
+    return (size_t)tile * (prec == COSIMS_PREC_FP64 ? 6 * sizeof(double) : sizeof(float4) + sizeof(float2));
+}
+
+// Launch arguments of the LDS-staged and tiled kernels for one molecule.
+GroupArgs group_args(const cosims_mol *M)
+{
+    GroupArgs G{};
+    G.m = M->md;
+    G.npad = M->npad;
+    G.soa = M->soa;
+    G.dt0 = M->params.dt;
+    G.xq = M->rec.xq;
+    G.se = M->rec.se;
+    return G;
 }
 
 // Work units one lane of a group walks per force evaluation: atoms, or
@@ -1537,19 +1590,20 @@ int run_multi(const MolRec *mols, const int32_t *tmol, int ft, int prec, int cus
 int run_group(const cosims_mol *M, int prec, int n_items, TrajBufs &B, hipStream_t s, int L, int wg, int bpc,
               GroupRun &out)
 {
-    if (!lds_mode(M, prec))
+    const bool tiled = tile_mode(M, prec);
+    if (!tiled && !lds_mode(M, prec))
         return run_multi(M->rec_dev, nullptr, M->prep.force_type, prec, M->dev.cus, B.items[0].as<int32_t>(), n_items,
                          B, s, L, wg, bpc, out);
-    GrpFn fn = group_kernel(prec, M->prep.force_type, L);
+    GrpFn fn = tiled ? tile_kernel(prec, M->prep.force_type, L) : group_kernel(prec, M->prep.force_type, L);
     if (!fn) return fail(COSIMS_E_INVALID, "no group kernel for this force type / lane count");
-    const size_t lds = group_lds_bytes(M, prec);
+    GroupArgs G = group_args(M);
+    size_t lds = group_lds_bytes(M, prec);
+    if (tiled) {
+        G.tile = tile_atoms(M, prec, wg);
+        lds = tile_lds_bytes(prec, G.tile);
+    }
     HIP_CHECK(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    GroupArgs G{};
     G.io = traj_io(B, B.items[0].as<int32_t>(), n_items);
-    G.m = M->md;
-    G.npad = M->npad;
-    G.soa = M->soa;
-    G.dt0 = M->params.dt;
     const int64_t grid = group_grid((const void *)fn, wg, lds, bpc, M->dev.cus, n_items, L);
     return timed_group_launch(fn, G, grid, wg, lds, s, B, n_items, out);
 }
@@ -1951,12 +2005,13 @@ int stage_batch(BatchPlan &P, hipStream_t s)
     return 0;
 }
 
-// The launches of a staged plan.  A job of an exact force type whose
-// molecule fits one workgroup's LDS runs the LDS-staged kernel, one launch per
-// job (its molecule read from LDS instead of through L2: 1.3-1.6x the pair
-// rate of the table kernel); the other jobs of a launch shape, in consecutive
-// runs of the work list, share table-kernel launches.  Both kernels give the
-// same bits for a trajectory.  Launches are spread over kSegStreams streams,
+// The launches of a staged plan.  A job of an exact force type runs one
+// launch of its own: the LDS-staged kernel when its molecule fits one
+// workgroup's LDS, the tiled kernel beyond (either way the atoms come from
+// LDS instead of through L2: 1.3-1.6x the pair rate of the table kernel); the
+// clustered jobs of a launch shape, in consecutive runs of the work list,
+// share table-kernel launches.  All three kernels give the same bits for a
+// trajectory.  Launches are spread over kSegStreams streams,
 // the most work first, so one launch's tail overlaps the next ones.
 // launch_enqueue only enqueues (the stream s carries the join; kernels run
 // on the plan's own streams); launch_finish waits for them and adds their
@@ -1978,7 +2033,9 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
         double run_work = 0;
         for (int i = sg.i0; i <= sg.i1; i++) {
             const BatchPlan::Job *J = i < sg.i1 ? &P.jobs[P.order[i]] : nullptr;
-            const bool lds = J && lds_mode(J->mol, prec) && group_kernel(prec, J->ft, J->L);
+            // a launch of its own: the LDS-staged kernel, or the tiled one beyond LDS
+            const bool lds = J && (lds_mode(J->mol, prec) || tile_mode(J->mol, prec)) &&
+                             group_kernel(prec, J->ft, J->L);
             if ((!J || lds) && run0 >= 0) {  // close the run of table-kernel jobs
                 plan.push_back({g, run0, i, false, run_traj, run_work});
                 run0 = -1;
@@ -2038,15 +2095,16 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
         const int32_t *last = ends + l.i1 - 1;
         if (l.lds) {
             const cosims_mol *M = P.jobs[P.order[l.i0]].mol;
-            GrpFn fn = group_kernel(prec, g.ft, g.L);
-            const size_t lds = group_lds_bytes(M, prec);
+            const bool tiled = tile_mode(M, prec);
+            GrpFn fn = tiled ? tile_kernel(prec, g.ft, g.L) : group_kernel(prec, g.ft, g.L);
+            GroupArgs G = group_args(M);
+            size_t lds = group_lds_bytes(M, prec);
+            if (tiled) {
+                G.tile = tile_atoms(M, prec, g.W);
+                lds = tile_lds_bytes(prec, G.tile);
+            }
             HIP_CHECK(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            GroupArgs G{};
             G.io = io;
-            G.m = M->md;
-            G.npad = M->npad;
-            G.soa = M->soa;
-            G.dt0 = M->params.dt;
             G.first = first;
             G.last = last;
             const int64_t grid = group_grid((const void *)fn, g.W, lds, bpc, P.cus, cnt, g.L);

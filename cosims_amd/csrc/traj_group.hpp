@@ -65,6 +65,11 @@ struct GroupArgs {
     // when set, the work list is io.items[*first .. *last) (first NULL: from
     // 0), bounds written on the device before the launch runs
     const int32_t *first, *last;
+    // k_traj_tile: the FP32 SoA in global memory and the atoms per LDS tile
+    // (a multiple of 64)
+    const float4 *xq;
+    const float2 *se;
+    int tile;
 };
 
 // One molecule of a multi-molecule launch.
@@ -292,9 +297,60 @@ __host__ __device__ inline void lj_constants(double s6, double e24, float &A, fl
     B = (float)(e24 * s6);
 }
 
-// FP64 reference-form pair terms (src/forceAccel.cpp:764-797, 865-880),
-// partial sums of this lane, then the group butterfly and the epilogue
-// (:800-802).  X: SoA x y z q s6 e24 with stride npad.
+// FP64 reference-form pair terms (src/forceAccel.cpp:764-797, 865-880) of
+// one atom (x, y, z, q, sigma^6, 24 eps/mu), added to this lane's partial
+// sums: LJ tx ty tz 0-2, field dX dY dZ 3-5, gradient sXX sYY sZZ sXY sXZ
+// sYZ 6-11.
+template <int FT>
+__device__ __forceinline__ void pair64(double ax, double ay, double az, double aq, double as6, double ae24,
+                                       double px, double py, double pz, double (&s)[12])
+{
+    double xD = px - ax, yD = py - ay, zD = pz - az;
+    double x2 = xD * xD, y2 = yD * yD, z2 = zD * zD;
+    double r2 = x2 + y2 + z2;
+    double r6 = r2 * r2 * r2;
+    double t2 = as6 / r6;
+    double t1 = 2 * t2 * t2;
+    double sc = ae24 * (t1 - t2) / r2;
+    s[0] += sc * xD; s[1] += sc * yD; s[2] += sc * zD;
+    if constexpr (FT == 4) {
+        double r = __builtin_sqrt(r2);
+        double r3 = r2 * r;
+        double r_3 = 1.0 / r3;
+        double r_5 = r_3 / r2;
+        r_3 *= aq;
+        r_5 *= aq;
+        s[3] += xD * r_3; s[4] += yD * r_3; s[5] += zD * r_3;
+        s[6] += (3.0 * x2 - r2) * r_5;
+        s[7] += (3.0 * y2 - r2) * r_5;
+        s[8] += (3.0 * z2 - r2) * r_5;
+        s[9] += 3 * xD * yD * r_5;
+        s[10] += 3 * xD * zD * r_5;
+        s[11] += 3 * yD * zD * r_5;
+    }
+}
+
+// Group sums of the FP64 partials and the epilogue (:800-802).
+template <int FT, int L>
+__device__ __forceinline__ void force64_epilogue(double (&s)[12], double ds, double &ox, double &oy, double &oz)
+{
+    double tx = gsum<L>(s[0]), ty = gsum<L>(s[1]), tz = gsum<L>(s[2]);
+    if constexpr (FT == 4) {
+        const double dX = gsum<L>(s[3]), dY = gsum<L>(s[4]), dZ = gsum<L>(s[5]);
+        const double sXX = gsum<L>(s[6]), sYY = gsum<L>(s[7]), sZZ = gsum<L>(s[8]);
+        const double sXY = gsum<L>(s[9]), sXZ = gsum<L>(s[10]), sYZ = gsum<L>(s[11]);
+        tx -= ds * (sXX * dX + sXY * dY + sXZ * dZ);
+        ty -= ds * (sXY * dX + sYY * dY + sYZ * dZ);
+        tz -= ds * (sXZ * dX + sYZ * dY + sZZ * dZ);
+    }
+    ox = tx; oy = ty; oz = tz;
+}
+
+// FP64 pair terms over atoms j, j+L, ... < npad of the SoA X (x y z q s6 e24
+// with stride npad), then the group sums and the epilogue.  The expressions
+// of pair64 / force64_epilogue written out (the translation unit does not
+// contract, so the two forms give the same bits; this one keeps the FP64
+// kernels within 128 VGPRs without spills).
 template <int FT, int L, typename PD>
 __device__ __forceinline__ void grp_force64(PD X, int npad, int j, double px, double py, double pz, double ds,
                                             double &ox, double &oy, double &oz)
@@ -715,6 +771,7 @@ __device__ __forceinline__ double grp_potential(PD X, int npad, bool charged, do
 // kernel arguments (wave-uniform).
 template <int PREC, int FT, int L>
 struct LdsSource {
+    static constexpr bool kCoop = false;
     const GroupArgs &A;
     const char *smem;
     __device__ void bind(int) {}
@@ -749,6 +806,7 @@ struct LdsSource {
 // read from the MolRec table.  Idle groups skip the sums.
 template <int PREC, int FT, int L>
 struct TableSource {
+    static constexpr bool kCoop = false;
     const MultiArgs &A;
     int *lists;  // this wavefront's cluster lists in LDS (clustered types, L = 64)
     int mi = 0;
@@ -800,6 +858,86 @@ struct TableSource {
     }
 };
 
+// Molecule source of k_traj_tile: one molecule too large for LDS, streamed
+// through LDS in tiles of A.tile atoms by the whole workgroup.  force() is
+// collective: every thread of the workgroup calls it (group_loop's kCoop
+// mode decides for the workgroup), stages each tile between two barriers,
+// and the wavefronts with a group that needs the force add the tile's
+// pairs.  Lane j of a group still sums atoms j, j+L, ... in order (tiles
+// start at multiples of 64), with the per-pair arithmetic and group sums of
+// the other kernels: the same bits as k_traj_grp and k_traj_multi.
+template <int PREC, int FT, int L>
+struct TileSource {
+    static constexpr bool kCoop = true;
+    const GroupArgs &A;
+    char *smem;
+    __device__ void bind(int) {}
+    __device__ int pairs() const { return A.m.n; }
+    __device__ double mu() const { return A.m.mu; }
+    __device__ double dt0() const { return A.dt0; }
+    __device__ bool exits(double px, double py, double pz) const
+    {
+        const double tx = px / A.m.a, ty = py / A.m.b, tz = pz / A.m.c;
+        const double tm2 = tx * tx + ty * ty + tz * tz;
+        return (tm2 > 1 && A.m.projection) || (tm2 > A.m.maxrad && !A.m.projection);
+    }
+    __device__ void force(bool active, int j, double ex, double ey, double ez, double &fx, double &fy,
+                          double &fz) const
+    {
+        const bool wave = __any(active);  // this wavefront adds pairs (idle groups at a far point)
+        const int npad = A.npad, T = A.tile;
+        if constexpr (PREC == 1) {
+            f4n *xq = (f4n *)smem;
+            f2n *ab = (f2n *)(xq + T);
+            const GLB f4n *gx = (const GLB f4n *)glb(A.xq);
+            const GLB f2n *ga = (const GLB f2n *)glb(A.se);
+            const float px = (float)ex, py = (float)ey, pz = (float)ez;
+            float s[12];
+#pragma unroll
+            for (int k = 0; k < 12; k++) s[k] = 0.0f;
+            for (int base = 0; base < npad; base += T) {
+                const int cnt = min(T, npad - base);
+                __syncthreads();  // every reader of the previous tile is done
+                for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+                    xq[i] = gx[base + i];
+                    ab[i] = ga[base + i];
+                }
+                __syncthreads();
+                if (wave) {
+#pragma unroll 4
+                    for (int kk = 0; kk < cnt; kk += L) pair32<FT>(xq[kk + j], ab[kk + j], px, py, pz, s);
+                }
+            }
+            if (wave) force32_epilogue<FT, L>(s, A.m.dipole_scalar, fx, fy, fz);
+        } else {
+            double *X = (double *)smem;  // x y z q s6 e24, T each
+            double s[12];
+#pragma unroll
+            for (int k = 0; k < 12; k++) s[k] = 0.0;
+            for (int base = 0; base < npad; base += T) {
+                const int cnt = min(T, npad - base);
+                __syncthreads();
+                for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+#pragma unroll
+                    for (int f = 0; f < 6; f++) X[f * T + i] = A.soa[f * npad + base + i];
+                }
+                __syncthreads();
+                if (wave) {
+#pragma unroll 2
+                    for (int k = j; k < cnt; k += L)
+                        pair64<FT>(X[k], X[T + k], X[2 * T + k], X[3 * T + k], X[4 * T + k], X[5 * T + k], ex, ey,
+                                   ez, s);
+                }
+            }
+            if (wave) force64_epilogue<FT, L>(s, A.m.dipole_scalar, fx, fy, fz);
+        }
+    }
+    __device__ double potential(bool, int j, double ex, double ey, double ez) const
+    {
+        return grp_potential<L>(glb(A.soa), A.npad, A.m.charged, A.m.dipole_scalar, j, ex, ey, ez);
+    }
+};
+
 enum : int { PH_IDLE = 0, PH_INIT = 1, PH_STEP = 2, PH_CHECK = 3 };
 
 // The trajectory state machine of diffEqSolver::verletVel
@@ -846,7 +984,13 @@ __device__ __forceinline__ void group_loop(const TrajIO &A, Src &S)
                 }
             }
         }
-        if (!__any(phase != PH_IDLE)) break;
+        // a collective source keeps every wavefront of the workgroup in the
+        // loop (and in its barriers) until the workgroup has no trajectory left
+        if constexpr (Src::kCoop) {
+            if (!__syncthreads_or(phase != PH_IDLE)) break;
+        } else {
+            if (!__any(phase != PH_IDLE)) break;
+        }
 
         // position of this group's force / potential evaluation
         // (src/diffEqSolver.cpp:131; recomputed after the force call instead
@@ -864,7 +1008,11 @@ __device__ __forceinline__ void group_loop(const TrajIO &A, Src &S)
         const bool needF = (phase == PH_INIT || phase == PH_STEP);
         const bool needU = (phase == PH_INIT && needE0) || phase == PH_CHECK;
         double fx = 0, fy = 0, fz = 0, u = 0;
-        if (__any(needF)) S.force(needF, j, ex, ey, ez, fx, fy, fz);
+        if constexpr (Src::kCoop) {
+            if (__syncthreads_or(needF)) S.force(needF, j, ex, ey, ez, fx, fy, fz);
+        } else {
+            if (__any(needF)) S.force(needF, j, ex, ey, ez, fx, fy, fz);
+        }
         if (__any(needU)) u = S.potential(needU, j, ex, ey, ez);
         if (j == 0) {
             n_evals += needF;
@@ -993,6 +1141,19 @@ __global__ void __launch_bounds__(1024) k_traj_grp(GroupArgs A)
     TrajIO io = A.io;
     resolve_range(io, A.first, A.last);
     LdsSource<PREC, FT, L> src{A, smem};
+    group_loop<L>(io, src);
+}
+
+// Molecules beyond one workgroup's LDS, exact force types: the molecule
+// streamed through LDS in tiles (TileSource); every wavefront stays in the
+// trajectory loop until its workgroup is done.
+template <int PREC, int FT, int L>
+__global__ void __launch_bounds__(1024) k_traj_tile(GroupArgs A)
+{
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    TrajIO io = A.io;
+    resolve_range(io, A.first, A.last);
+    TileSource<PREC, FT, L> src{A, smem};
     group_loop<L>(io, src);
 }
 

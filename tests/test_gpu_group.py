@@ -160,7 +160,24 @@ def ion8000():
     return E.Molecule.from_file(path, params=p)
 
 
+@pytest.mark.parametrize("prec", [E.PREC_MIXED, E.PREC_FP64])
+def test_tiled_kernel_tile_size_does_not_change_bits(ion8000, prec):
+    """8,000 atoms run the tiled kernel (molecule streamed through LDS in
+    tiles sized by the workgroup: 256 threads hold a quarter of a CU's LDS,
+    1024 all of it).  Tiles start at multiples of 64, so every lane sums the
+    same atoms in the same order: identical results at both sizes."""
+    kw = dict(job=8, iters=2, traj=600, lanes=16, keep=True)
+    a = ion8000.run(prec, wg_size=256, **kw)
+    b = ion8000.run(prec, wg_size=1024, **kw)
+    assert a["wg_size"] == 256 and b["wg_size"] == 1024
+    assert np.array_equal(a["status"], b["status"])
+    assert np.array_equal(a["chi"], b["chi"])
+    assert np.array_equal(a["omega"], b["omega"])
+
+
 def test_large_molecule_table_kernel(ion8000):
+    """Beyond LDS (the tiled kernel for these exact types): FP64 groups match
+    the lane kernel, mixed precision matches FP64."""
     kw = dict(job=6, iters=1, traj=1000, keep=True)
     lane = ion8000.run(E.PREC_FP64, lanes=1, **kw)
     grp = ion8000.run(E.PREC_FP64, lanes=16, wg_size=256, **kw)
