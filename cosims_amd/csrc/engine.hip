@@ -1809,6 +1809,8 @@ namespace {
 // ------------------------------------------------------------- batches ---
 constexpr int kMaxSegs = 64;        // launch shapes of one batch (force type x lanes x workgroup size)
 constexpr int kMaxLaunches = 4096;  // kernel launches of one batch
+constexpr int kMaxOwnLaunches = 2048;         // jobs with a launch of their own, per batch
+constexpr double kOwnLaunchWork = 65536.0;    // trajectories x atoms below which a job shares a table launch
 constexpr int kSegStreams = 4;      // streams the launches of a batch are spread over
 
 // Lanes per trajectory from the units one force evaluation sweeps (atoms,
@@ -2009,9 +2011,9 @@ int stage_batch(BatchPlan &P, hipStream_t s)
 // launch of its own: the LDS-staged kernel when its molecule fits one
 // workgroup's LDS, the tiled kernel beyond (either way the atoms come from
 // LDS instead of through L2: 1.3-1.6x the pair rate of the table kernel); the
-// clustered jobs of a launch shape, in consecutive runs of the work list,
-// share table-kernel launches.  All three kernels give the same bits for a
-// trajectory.  Launches are spread over kSegStreams streams,
+// clustered jobs of a launch shape, tiny jobs and jobs beyond
+// kMaxOwnLaunches, in consecutive runs of the work list, share table-kernel
+// launches.  All three kernels give the same bits for a trajectory.  Launches are spread over kSegStreams streams,
 // the most work first, so one launch's tail overlaps the next ones.
 // launch_enqueue only enqueues (the stream s carries the join; kernels run
 // on the plan's own streams); launch_finish waits for them and adds their
@@ -2026,6 +2028,7 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
         double work;
     };
     std::vector<Launch> plan;
+    int own = 0;
     for (int g = 0; g < (int)P.segs.size(); g++) {
         const auto &sg = P.segs[g];
         int run0 = -1;
@@ -2034,8 +2037,11 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
         for (int i = sg.i0; i <= sg.i1; i++) {
             const BatchPlan::Job *J = i < sg.i1 ? &P.jobs[P.order[i]] : nullptr;
             // a launch of its own: the LDS-staged kernel, or the tiled one beyond LDS
+            // (tiny jobs, and jobs past the per-batch count, share table launches)
             const bool lds = J && (lds_mode(J->mol, prec) || tile_mode(J->mol, prec)) &&
-                             group_kernel(prec, J->ft, J->L);
+                             group_kernel(prec, J->ft, J->L) && (double)J->n * J->units >= kOwnLaunchWork &&
+                             own < kMaxOwnLaunches;
+            own += lds;
             if ((!J || lds) && run0 >= 0) {  // close the run of table-kernel jobs
                 plan.push_back({g, run0, i, false, run_traj, run_work});
                 run0 = -1;

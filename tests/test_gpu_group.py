@@ -205,3 +205,18 @@ def test_batch_mixes_table_and_lds_launches(ion8000, ion128, ion2000):
             single = job[0].run(prec, job=job[1]["job"], iters=job[1]["iters"], traj=job[1]["traj"], **shape)
             assert np.array_equal(mix[k]["omega"], single["omega"]), (prec, k)
             assert mix[k]["total_steps"] == single["total_steps"]
+
+
+def test_batch_tiny_jobs_share_a_table_launch(ion128, ion2000):
+    """Jobs below 65,536 trajectories x atoms share one table-kernel launch
+    (a launch each would cost more than their work); a larger job keeps its
+    own.  Every result equals cosims_run (LDS kernel) bit for bit."""
+    shape = dict(lanes=8, wg_size=256)
+    jobs = [(ion128, dict(job=50 + k, iters=1, traj=300)) for k in range(4)]
+    jobs.append((ion2000, dict(job=60, iters=1, traj=1000)))
+    mix, st = E.run_batch(jobs, prec=E.PREC_MIXED, **shape)
+    assert st["launches"] == 2
+    for k, (m, spec) in enumerate(jobs):
+        single = m.run(E.PREC_MIXED, job=spec["job"], iters=1, traj=spec["traj"], **shape)
+        assert np.array_equal(mix[k]["omega"], single["omega"]), k
+        assert mix[k]["total_steps"] == single["total_steps"], k
