@@ -144,6 +144,10 @@ def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, fp64_max_at
         "trajectories_issued": st["traj"],
         "rel_err_max": max(r["rel_err"] for r in res), "blocks_mean": float(np.mean([r["blocks"] for r in res])),
         "slices": st["slices"], "kernel_s": st["kernel_ms"] / 1e3, "busy_s": st["busy_ms"] / 1e3,
+        # reference FLOPs of every trajectory issued (blocks past a stopping point included) over the
+        # time some slice was in flight
+        "achieved_tflops_busy": (st["pair_evals"] * FLOP_PAIR + st["force_evals"] * FLOP_EPI
+                                 + st["total_steps"] * FLOP_VERLET) / (st["busy_ms"] * 1e-3) / 1e12,
         "lanes_per_trajectory": "by size class (4 x 2^class)", "wg_size": W,
         "prior_rel_sd": server_kw["prior_rel_sd"], "issue_fraction": server_kw["issue_fraction"],
         "policy": "shortest remaining work first" if policy == E.SERVE_SHORTEST_FIRST else "equal share",

@@ -385,6 +385,7 @@ __global__ void __launch_bounds__(256) k_check(CheckArgs A)
 struct ReduceArgs {
     const int8_t *status;
     const double *chi, *ergy;
+    const double *cchi;  // when set: accepted trajectories' cos(chi) (chi taken here, as k_chi does)
     const int32_t *totsteps;
     int64_t per_iter;
     double ccs_const;
@@ -400,7 +401,11 @@ __global__ void __launch_bounds__(256) k_reduce(ReduceArgs A)
     double s = 0, s2 = 0, mx = 0, st = 0, nh = 0, nf = 0;
     for (int64_t j = threadIdx.x; j < A.per_iter; j += blockDim.x) {
         const int8_t stt = A.status[base + j];
-        const double c = (stt == ST_MISS) ? 0.0 : A.chi[base + j];
+        double c = (stt == ST_MISS) ? 0.0 : A.chi[base + j];
+        if (A.cchi && stt == ST_OK) {
+            const double t3 = A.cchi[base + j];
+            c = (t3 > 1) ? 0.0 : ((t3 < -1) ? 3.14159265358979323846 : acos(t3));
+        }
         const double x = A.ccs_const * (1 - cos(fabs(c)));
         s += x;
         s2 += x * x;
@@ -2129,9 +2134,7 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
             hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(g.W), lds, x, G);
         }
         HIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_chi, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, x, io.items, cnt, io.status,
-                           io.cchi, io.chi, first, last);
-        HIP_CHECK(hipGetLastError());
+        // no acos pass here: k_reduce takes chi from the stored cosine (ReduceArgs::cchi)
         hipEvent_t j;
         HIP_CHECK(hipEventCreateWithFlags(&j, hipEventDisableTiming));
         P.joins.push_back(j);
@@ -2503,6 +2506,7 @@ int cosims_run_batch(cosims_batch_job *jobs, int32_t njobs, const cosims_batch_c
         it_base[k] = it0;
         ReduceArgs RA{};
         RA.status = P.B.status.as<int8_t>() + J.off; RA.chi = P.B.chi.as<double>() + J.off;
+        RA.cchi = P.B.exitv2.as<double>() + J.off;
         RA.ergy = P.B.ergy.as<double>() + J.off; RA.totsteps = P.B.tot.as<int32_t>() + J.off;
         RA.per_iter = J.T; RA.ccs_const = J.mol->prep.ccs_const; RA.out = red.as<double>() + kSums * it0;
         hipLaunchKernelGGL(k_reduce, dim3(J.iters), dim3(256), 0, st.s, RA);

@@ -282,6 +282,7 @@ int issue_slice(cosims_server *S, Slice &X)
         const auto &J = P.jobs[i];
         ReduceArgs RA{};
         RA.status = P.B.status.as<int8_t>() + J.off; RA.chi = P.B.chi.as<double>() + J.off;
+        RA.cchi = P.B.exitv2.as<double>() + J.off;
         RA.ergy = P.B.ergy.as<double>() + J.off; RA.totsteps = P.B.tot.as<int32_t>() + J.off;
         RA.per_iter = J.T; RA.ccs_const = J.mol->prep.ccs_const;
         RA.out = X.e[i].job->dsums.as<double>() + kSums * X.e[i].it0;
