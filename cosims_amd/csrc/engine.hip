@@ -1813,7 +1813,7 @@ namespace {
 
 // ------------------------------------------------------------- batches ---
 constexpr int kMaxSegs = 64;        // launch shapes of one batch (force type x lanes x workgroup size)
-constexpr int kMaxLaunches = 4096;  // kernel launches of one batch
+constexpr int kMaxLaunches = 8192;  // kernel launches of one batch (own launches + the table runs between them)
 constexpr int kMaxOwnLaunches = 2048;         // jobs with a launch of their own, per batch
 constexpr double kOwnLaunchWork = 65536.0;    // trajectories x atoms below which a job shares a table launch
 constexpr int kSegStreams = 4;      // streams the launches of a batch are spread over
