@@ -284,9 +284,7 @@ __global__ void __launch_bounds__(256) k_traj(TrajArgs A)
                     state = 0;
                 }
             } else {
-                const double tx = px / m.a, ty = py / m.b, tz = pz / m.c;
-                const double tm2 = tx * tx + ty * ty + tz * tz;
-                if (((tm2 > 1 && m.projection) || (tm2 > m.maxrad && !m.projection)) && steps > 10) {
+                if (outside(m, px, py, pz) && steps > 10) {
                     // exit: scattering angle (vector3D::Angle, src/vector3D.h:134-143)
                     const double w0 = A.v0[3 * t], w1 = A.v0[3 * t + 1], w2 = A.v0[3 * t + 2];
                     const double t1 = w0 * vx + w1 * vy + w2 * vz;
@@ -937,6 +935,7 @@ int upload(const cosims_mol *M)
     m.order = p.multipole_order; m.projection = p.projection ? 1 : 0;
     m.mu = p.mu; m.dipole_scalar = p.dipole_scalar; m.a = p.a; m.b = p.b; m.c = p.c;
     m.maxrad = std::max(p.a, std::max(p.b, p.c));
+    m.ia = 1.0 / p.a; m.ib = 1.0 / p.b; m.ic = 1.0 / p.c;
     m.a64 = (const AtomF64 *)(d + o64); m.a32 = (const AtomF32 *)(d + o32);
     m.c64 = (const AtomF64 *)(d + oc64); m.c32 = (const AtomF32 *)(d + oc32);
     m.h64 = (const ClusterF64 *)(d + oh64); m.h32 = (const ClusterF32 *)(d + oh32);

@@ -40,6 +40,7 @@ using ClusterF32 = ClusterT<float>;
 struct MolDev {
     int n, ncl, force_type, charged, order, projection;
     double mu, dipole_scalar, a, b, c, maxrad;
+    double ia, ib, ic;    // 1/a, 1/b, 1/c (IEEE), for outside()
     const AtomF64 *a64;   // atom order
     const AtomF32 *a32;
     const AtomF64 *c64;   // cluster member order
@@ -57,6 +58,29 @@ struct MolDev {
     const float2 *cab;
     float cl_dpr_max;     // largest (R_d + rho)^2 of cg2 (FP32)
 };
+
+// Outside the boundary ellipsoid (src/diffEqSolver.cpp:99, 174): the
+// reference's |p / (a, b, c)|^2 > 1 with projection, > max(a, b, c) without
+// (its test as written).  The products with the reciprocals carry a few ulp
+// of error against the quotients, so they decide wherever they lie more than
+// 1e-12 (relative) from the threshold; closer, the quotients decide.  The
+// answer is always the divisions' answer (outside_div), without three FP64
+// divisions a step.  The FP64 group kernels keep outside_div: at the edge of
+// 128 VGPRs, the extra live values of the fast path would spill.
+__device__ __forceinline__ bool outside_div(const MolDev &m, double px, double py, double pz)
+{
+    const double tx = px / m.a, ty = py / m.b, tz = pz / m.c;
+    const double tm2 = tx * tx + ty * ty + tz * tz;
+    return (tm2 > 1 && m.projection) || (tm2 > m.maxrad && !m.projection);
+}
+__device__ __forceinline__ bool outside(const MolDev &m, double px, double py, double pz)
+{
+    const double thr = m.projection ? 1.0 : m.maxrad;
+    const double ux = px * m.ia, uy = py * m.ib, uz = pz * m.ic;
+    const double q = ux * ux + uy * uy + uz * uz;
+    if (fabs(q - thr) > 1e-12 * thr) return q > thr;
+    return outside_div(m, px, py, pz);
+}
 
 template <typename T>
 __device__ __forceinline__ const CAS T *cas(const T *p)
