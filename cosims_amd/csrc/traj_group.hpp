@@ -215,27 +215,28 @@ __device__ __forceinline__ void pair32(f4n a, f2n c, float px, float py, float p
     }
 }
 
-// Group sums of the partials and the polarisation epilogue.  The force is
-// F - ds T E with T = 3 S - tr(S) I the field gradient and E the field, both
-// sums over the atoms.  T E is linear in T, so after the group sum of E each
-// lane applies it to its own partial T and only the three force components
-// are summed next: 6 group sums a step instead of 12.
+// Group sums of the partials and the polarisation epilogue (FP64).  (Summing
+// the field first and letting each lane apply it to its own gradient
+// partials needs 6 group sums instead of 12, but its two dependent chains
+// cost the lock-step tiled kernel 3-9 % and gained nothing measurable in the
+// LDS kernel; all exact-type kernels share this one, so they stay bitwise
+// identical.)
 template <int FT, int L>
 __device__ __forceinline__ void force32_epilogue(float (&s)[12], double ds, double &ox, double &oy, double &oz)
 {
+    const float fx = gsum<L>(s[0]), fy = gsum<L>(s[1]), fz = gsum<L>(s[2]);
     if constexpr (FT == 4) {
         const float ex = gsum<L>(s[3]), ey = gsum<L>(s[4]), ez = gsum<L>(s[5]);
-        const float sxx = s[6], syy = s[7], sxy = s[8], sxz = s[9], syz = s[10], tr = s[11];
-        const float szz = tr - sxx - syy;  // the trace of S is the sum of q r^-3
-        const float tx = 3.0f * (sxx * ex + sxy * ey + sxz * ez) - tr * ex;
-        const float ty = 3.0f * (sxy * ex + syy * ey + syz * ez) - tr * ey;
-        const float tz = 3.0f * (sxz * ex + syz * ey + szz * ez) - tr * ez;
-        const float d = (float)ds;
-        ox = gsum<L>(s[0] - d * tx);
-        oy = gsum<L>(s[1] - d * ty);
-        oz = gsum<L>(s[2] - d * tz);
+        const float sxx = gsum<L>(s[6]), syy = gsum<L>(s[7]), tr = gsum<L>(s[11]);
+        const float sxy = gsum<L>(s[8]), sxz = gsum<L>(s[9]), syz = gsum<L>(s[10]);
+        const double t = tr, Sxx = sxx, Syy = syy;
+        const double Txx = 3.0 * Sxx - t, Tyy = 3.0 * Syy - t, Tzz = 2.0 * t - 3.0 * Sxx - 3.0 * Syy;
+        const double Txy = 3.0 * sxy, Txz = 3.0 * sxz, Tyz = 3.0 * syz;
+        ox = (double)fx - ds * (Txx * ex + Txy * ey + Txz * ez);
+        oy = (double)fy - ds * (Txy * ex + Tyy * ey + Tyz * ez);
+        oz = (double)fz - ds * (Txz * ex + Tyz * ey + Tzz * ez);
     } else {
-        ox = gsum<L>(s[0]); oy = gsum<L>(s[1]); oz = gsum<L>(s[2]);
+        ox = fx; oy = fy; oz = fz;
     }
 }
 
