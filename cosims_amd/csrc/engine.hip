@@ -2027,7 +2027,7 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
     launches = 0;
     struct Launch {
         int seg, i0, i1;  // work-list positions order[i0 .. i1)
-        bool lds;
+        bool own;         // one job on the LDS-staged or tiled kernel (else a table-kernel run)
         int64_t traj;
         double work;
     };
@@ -2042,18 +2042,18 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
             const BatchPlan::Job *J = i < sg.i1 ? &P.jobs[P.order[i]] : nullptr;
             // a launch of its own: the LDS-staged kernel, or the tiled one beyond LDS
             // (tiny jobs, and jobs past the per-batch count, share table launches)
-            const bool lds = J && (lds_mode(J->mol, prec) || tile_mode(J->mol, prec)) &&
-                             group_kernel(prec, J->ft, J->L) && (double)J->n * J->units >= kOwnLaunchWork &&
-                             own < kMaxOwnLaunches;
-            own += lds;
-            if ((!J || lds) && run0 >= 0) {  // close the run of table-kernel jobs
+            const bool mine = J && (lds_mode(J->mol, prec) || tile_mode(J->mol, prec)) &&
+                              group_kernel(prec, J->ft, J->L) && (double)J->n * J->units >= kOwnLaunchWork &&
+                              own < kMaxOwnLaunches;
+            own += mine;
+            if ((!J || mine) && run0 >= 0) {  // close the run of table-kernel jobs
                 plan.push_back({g, run0, i, false, run_traj, run_work});
                 run0 = -1;
                 run_traj = 0;
                 run_work = 0;
             }
             if (!J) break;
-            if (lds) {
+            if (mine) {
                 plan.push_back({g, i, i + 1, true, J->n, (double)J->n * J->units});
             } else {
                 if (run0 < 0) run0 = i;
@@ -2103,7 +2103,7 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
         io.evals = P.seg_evals.as<unsigned long long>() + 4 * li;
         const int32_t *first = l.i0 ? ends + l.i0 - 1 : nullptr;
         const int32_t *last = ends + l.i1 - 1;
-        if (l.lds) {
+        if (l.own) {
             const cosims_mol *M = P.jobs[P.order[l.i0]].mol;
             const bool tiled = tile_mode(M, prec);
             GrpFn fn = tiled ? tile_kernel(prec, g.ft, g.L) : group_kernel(prec, g.ft, g.L);
