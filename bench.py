@@ -30,7 +30,7 @@ PEAK_FP32 = 157.3          # TFLOP/s, MI355X FP32 vector (packed) peak
 PEAK_FP32_UNPACKED = 78.6  # TFLOP/s, FP32 vector without packed math (= FP64 vector)
 # HBM bytes per launch of the timed mixed kernel (rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0,
 # scale factors from a 256 MiB device copy), from profiles/traj_grp_mixed_ion2000.md
-TRAFFIC_BYTES = (114758 * 2 + 129265) * 1024
+TRAFFIC_BYTES = (114577 * 2 + 133501) * 1024
 
 
 def log(*a):
