@@ -1,4 +1,6 @@
-"""Group kernels beyond the LDS-staged exact types, and batched runs (MI355X).
+"""Group kernels beyond the LDS-staged exact types (clustered types through
+the table kernel, molecules beyond LDS through the tiled kernel), and
+batched runs (MI355X).
 
 The lane-per-trajectory FP64 kernel (lanes=1) sums in the reference's atom /
 cluster order and is pinned bit-for-bit to the reference in
@@ -151,7 +153,7 @@ def test_batch_tuner_and_stats(ion128, ion2000):
 
 @pytest.fixture(scope="module")
 def ion8000():
-    """8,000 atoms: beyond one workgroup's LDS in FP32 and FP64 (table kernel)."""
+    """8,000 atoms: beyond one workgroup's LDS in FP32 and FP64 (tiled kernel)."""
     import tempfile
     ion = synth.synthetic_ion(8000, 18.0 * 4 ** (1 / 3), 0.3, 33, 3001)
     path = os.path.join(tempfile.mkdtemp(prefix="cosims_t_"), "ion8000.mfj")
@@ -175,7 +177,7 @@ def test_tiled_kernel_tile_size_does_not_change_bits(ion8000, prec):
     assert np.array_equal(a["omega"], b["omega"])
 
 
-def test_large_molecule_table_kernel(ion8000):
+def test_large_molecule_beyond_lds(ion8000):
     """Beyond LDS (the tiled kernel for these exact types): FP64 groups match
     the lane kernel, mixed precision matches FP64."""
     kw = dict(job=6, iters=1, traj=1000, keep=True)
