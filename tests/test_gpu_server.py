@@ -178,3 +178,23 @@ def test_server_rejects_bad_requests():
             srv.submit(bad, _params(1))
         with pytest.raises(E.CosimsError, match="unknown ticket"):
             srv.wait(12345)
+
+
+def test_server_large_molecule_equals_run():
+    """A request beyond one workgroup's LDS (8,000 atoms: the tiled kernel)
+    gives cosims_run's result over the same blocks."""
+    import tempfile
+
+    from cosims_amd import synth
+    ion = synth.synthetic_ion(8000, 18.0 * 4 ** (1 / 3), 0.3, 33, 3001)
+    path = os.path.join(tempfile.mkdtemp(prefix="cosims_s_"), "ion8000.mfj")
+    synth.write_mfj(path, "ion8000", ion)
+    mol = E.read_molfile(path)
+    p = dict(multipole=0, dispersion_cutoff=0)
+    with _server(max_blocks=6) as srv:
+        r = srv.wait(srv.submit(mol, _params(300, **p), job=30, target=0.05))
+    assert r["status"] == 0 and 3 <= r["blocks"] <= 6
+    m = E.Molecule.from_file(path, params=_params(300, **p))
+    run = m.run(E.PREC_MIXED, job=30, iters=r["blocks"], traj=BLOCK, **SHAPE)
+    assert np.array_equal(r["omega"], run["omega"])
+    assert r["wmean"] == run["wmean"]
