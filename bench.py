@@ -95,6 +95,10 @@ def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, fp64_max_at
     # 2.3-2.9 on this stream), then the whole estimate of what is still needed each slice
     server_kw.setdefault("prior_rel_sd", 2.0)
     server_kw.setdefault("issue_fraction", 1.0)
+    # workgroups of 1024 for the two largest size classes (768 atoms and up: fewer, larger LDS
+    # tiles and one LDS staging shared by more groups); 950-960 k traj/s against 800-890 k at 512
+    # on the same box
+    server_kw.setdefault("class_wg", [0, 0, 0, 1024, 1024])
     log(f"serving: {n_requests} requests at {rate}/s, target {target}, lanes by size, wg {W}, {server_kw}")
     srv = E.Server(prec=E.PREC_MIXED, lanes=0, wg_size=W, block_traj=10000, min_blocks=3, max_blocks=1000,
                    policy=policy, **server_kw)
@@ -149,6 +153,7 @@ def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, fp64_max_at
         "achieved_tflops_busy": (st["pair_evals"] * FLOP_PAIR + st["force_evals"] * FLOP_EPI
                                  + st["total_steps"] * FLOP_VERLET) / (st["busy_ms"] * 1e-3) / 1e12,
         "lanes_per_trajectory": "by size class (4 x 2^class)", "wg_size": W,
+        "class_wg": server_kw["class_wg"],
         "prior_rel_sd": server_kw["prior_rel_sd"], "issue_fraction": server_kw["issue_fraction"],
         "policy": "shortest remaining work first" if policy == E.SERVE_SHORTEST_FIRST else "equal share",
         "ccs_vs_fp64": vs64,

@@ -268,8 +268,9 @@ int issue_slice(cosims_server *S, Slice &X)
         b.ft = J.mol->prep.force_type;
         b.units = group_units(J.mol->prep);
         b.it0 = (uint32_t)e.it0;
-        b.L = c.lanes > 0 ? c.lanes : lanes_rule(b.units);
-        b.W = std::max(c.wg_size, b.L);
+        const int k = size_class(b.units);
+        b.L = c.lanes > 0 ? c.lanes : (c.class_lanes[k] > 0 ? c.class_lanes[k] : lanes_rule(b.units));
+        b.W = std::max(c.lanes <= 0 && c.class_wg[k] > 0 ? c.class_wg[k] : c.wg_size, b.L);
         P.n += b.n;
         P.jobs.push_back(b);
         J.issued += e.nb;
@@ -476,6 +477,13 @@ int cosims_server_create(const cosims_server_cfg *cfg, cosims_server **out)
         return fail(COSIMS_E_INVALID, "server lanes must be 0 (size rule), 4, 8, 16, 32 or 64");
     if (c.wg_size % 64 || c.wg_size > 1024)
         return fail(COSIMS_E_INVALID, "server wg_size must be a multiple of 64, at most 1024");
+    for (int k = 0; k < COSIMS_SIZE_CLASSES; k++) {
+        const int L = c.class_lanes[k], W = c.class_wg[k];
+        if (L != 0 && L != 4 && L != 8 && L != 16 && L != 32 && L != 64)
+            return fail(COSIMS_E_INVALID, "server class_lanes must be 0, 4, 8, 16, 32 or 64");
+        if (W < 0 || W % 64 || W > 1024)
+            return fail(COSIMS_E_INVALID, "server class_wg must be 0 or a multiple of 64, at most 1024");
+    }
     if (c.min_slice_traj <= 0) c.min_slice_traj = 65536;
     if (c.pipeline_depth <= 0) c.pipeline_depth = 3;
     if (c.pipeline_depth > kMaxDepth) return fail(COSIMS_E_INVALID, "pipeline_depth at most 8");

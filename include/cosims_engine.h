@@ -321,6 +321,9 @@ typedef struct cosims_server_cfg {
                                  per-trajectory CCS terms this large would need for the target
                                  ((prior_rel_sd / target)^2 trajectories, at least min_blocks);
                                  0 => min_blocks only */
+    int32_t class_lanes[COSIMS_SIZE_CLASSES]; /* with lanes 0: lanes of each size class (e.g. from
+                                                 cosims_tune_batch); 0 => the size rule */
+    int32_t class_wg[COSIMS_SIZE_CLASSES];    /* workgroup size of each class; 0 => wg_size */
 } cosims_server_cfg;
 
 /* Slice composition: shortest remaining work first (requests ordered by

@@ -171,6 +171,32 @@ def test_server_size_rule_equals_run():
         assert np.array_equal(r["omega"], run["omega"]), (f, L)
 
 
+def test_server_class_shapes_equal_run():
+    """Per-size-class lanes and workgroup sizes: each request runs with its
+    class's shape and gives cosims_run's result with that shape."""
+    CL = [8, 16, 0, 0, 32]       # class 2, 3: the size rule
+    CW = [256, 0, 0, 512, 1024]  # class 1, 2: wg_size
+    with _server(lanes=0, wg_size=256, class_lanes=CL, class_wg=CW) as srv:
+        tickets = _submit_all(srv, REQUESTS)
+        res = [srv.wait(t) for t in tickets]
+    for (f, p, j, t), r in zip(REQUESTS, res):
+        m = E.Molecule.from_file(os.path.join(GOLDEN, f), params=_params(100 + j, **p))
+        inf = m.info()
+        k = E.size_class(inf["n"] if inf["force_type"] in (4, 6) else inf["ncl"])
+        L = CL[k] or (4 << k)
+        W = max(CW[k] or 256, L)
+        assert (r["lanes"], r["wg_size"]) == (L, W), (f, k)
+        run = m.run(E.PREC_MIXED, job=j, iters=r["blocks"], traj=BLOCK, lanes=L, wg_size=W)
+        assert np.array_equal(r["omega"], run["omega"]), (f, L, W)
+
+
+def test_server_rejects_bad_class_shapes():
+    for kw in (dict(class_lanes=[0, 0, 12, 0, 0]), dict(class_wg=[0, 100, 0, 0, 0]),
+               dict(class_wg=[2048, 0, 0, 0, 0])):
+        with pytest.raises(E.CosimsError, match="invalid argument"):
+            _server(lanes=0, **kw)
+
+
 def test_server_rejects_bad_requests():
     with _server() as srv:
         bad = dict(xyz=np.zeros((0, 3)), mass=[], sigma=[], epsilon=[], charged=0, sigma_max=3.5)
