@@ -180,12 +180,20 @@ def batch_workload(E, synth, count, traj, steps, fp64_every=16):
     log(f"batch: tuned lanes {CL} wg {CW}")
     n_atoms = [m.info()["n"] for m in mols]
     out = {}
-    for label, kw in (("tuned", dict(lanes=-1, class_lanes=CL, class_wg=CW)), ("size_rule", dict(lanes=0))):
+    # one persistent launch per launch shape for all conformers (molecules through L2), its class
+    # shapes tuned for that mode in an untimed call
+    _, st_sh = E.run_batch(jobs, E.PREC_MIXED, lanes=-1, shared=True)
+    SL, SW = st_sh["class_lanes"], st_sh["class_wg"]
+    log(f"batch: shared-launch tuned lanes {SL} wg {SW}")
+    for label, kw in (("tuned", dict(lanes=-1, class_lanes=CL, class_wg=CW)), ("size_rule", dict(lanes=0)),
+                      ("shared_launch_tuned", dict(lanes=-1, class_lanes=SL, class_wg=SW, shared=True))):
         times = []
         for k in range(steps):
             E.device_sync()
             t1 = time.perf_counter()
             res, st = E.run_batch(jobs, prec=E.PREC_MIXED, **kw)
+            if label == "size_rule":
+                res_rule = res
             E.device_sync()
             times.append(time.perf_counter() - t1)
             log(f"batch {label} {k}: {times[-1]:.2f} s, kernel {st['kernel_ms']:.0f} ms, {st['launches']} launches")
@@ -196,12 +204,14 @@ def batch_workload(E, synth, count, traj, steps, fp64_every=16):
                       "class_lanes": st["class_lanes"], "class_wg": st["class_wg"]}
     out["workload"] = (f"{count} synthetic conformers, N log-uniform 50-5,000 (total {sum(n_atoms)} atoms), "
                        f"q round(N/250)+1 e, He 298 K, {traj} trajectories each, mixed precision")
+    out["modes"] = ("tuned / size_rule: a launch of its own per conformer (molecule staged in LDS); "
+                    "shared_launch_tuned: all conformers of a launch shape in one persistent launch")
     out["trajectories"] = count * traj
     # CCS error against FP64 on paired initial conditions (same jobs and streams), on every
-    # fp64_every-th conformer; `res` holds the size-rule mixed results of the last timed batch
+    # fp64_every-th conformer, against the size-rule mixed results
     sub = list(range(0, count, fp64_every))
     f64, _ = E.run_batch([jobs[k] for k in sub], prec=E.PREC_FP64, lanes=0)
-    dev = np.array([(res[k]["wmean"] - f["wmean"]) / f["wmean"] for k, f in zip(sub, f64)])
+    dev = np.array([(res_rule[k]["wmean"] - f["wmean"]) / f["wmean"] for k, f in zip(sub, f64)])
     rse = np.array([f["werr"] / f["wmean"] for f in f64])
     out["ccs_vs_fp64"] = {"conformers": len(sub), "mean_rel_dev": float(dev.mean()),
                           "mean_rel_dev_se": float(dev.std(ddof=1) / np.sqrt(dev.size)) if dev.size > 1 else None,

@@ -1859,6 +1859,7 @@ struct BatchPlan {
     int64_t n = 0;
     int cus = 0;
     int nlaunch = 0;                     // launches of the last launch_enqueue
+    bool shared = false;                 // every job of a launch shape in one table-kernel launch
     TrajBufs B;
     DBuf tmol, table, seg_ctr, seg_evals, job_end;
     int64_t cap = 0, table_cap = 0, job_end_cap = 0;  // allocated trajectories / table entries / bounds (reused)
@@ -2042,7 +2043,7 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
             const BatchPlan::Job *J = i < sg.i1 ? &P.jobs[P.order[i]] : nullptr;
             // a launch of its own: the LDS-staged kernel, or the tiled one beyond LDS
             // (tiny jobs, and jobs past the per-batch count, share table launches)
-            const bool mine = J && (lds_mode(J->mol, prec) || tile_mode(J->mol, prec)) &&
+            const bool mine = J && !P.shared && (lds_mode(J->mol, prec) || tile_mode(J->mol, prec)) &&
                               group_kernel(prec, J->ft, J->L) && (double)J->n * J->units >= kOwnLaunchWork &&
                               own < kMaxOwnLaunches;
             own += mine;
@@ -2177,7 +2178,7 @@ int launch_batch(BatchPlan &P, int prec, int bpc, hipStream_t s, GroupRun &tot, 
 
 // Size-class autotuner of a batch: for every size class present, the
 // lanes x workgroup search of tune_shapes on that class's jobs alone.
-int tune_batch(const cosims_batch_job *jobs, int32_t njobs, int prec, ShapeTable &out)
+int tune_batch(const cosims_batch_job *jobs, int32_t njobs, int prec, ShapeTable &out, bool shared = false)
 {
     if (!jobs || njobs <= 0) return fail(COSIMS_E_INVALID, "cosims_tune_batch: no jobs");
     Stream st;
@@ -2214,6 +2215,7 @@ int tune_batch(const cosims_batch_job *jobs, int32_t njobs, int prec, ShapeTable
             fixed.lanes = L;
             fixed.wg = W;
             if (int rc = plan_batch(sub.data(), (int32_t)sub.size(), caps[k], fixed, set)) return rc;
+            set.shared = shared;
             for (auto &J : set.jobs) J.job = 0xfffffff0u - (uint32_t)k;  // a stream of its own
             if (int rc = stage_batch(set, st.s)) return rc;
             GroupRun R;
@@ -2483,10 +2485,11 @@ int cosims_run_batch(cosims_batch_job *jobs, int32_t njobs, const cosims_batch_c
         bool given = false;
         for (int c = 0; c < COSIMS_SIZE_CLASSES; c++) given |= cfg->class_lanes[c] != 0;
         if (!given)
-            if (int rc = tune_batch(jobs, njobs, prec, shape)) return rc;
+            if (int rc = tune_batch(jobs, njobs, prec, shape, cfg->shared != 0)) return rc;
     }
     BatchPlan P;
     if (int rc = plan_batch(jobs, njobs, 0, shape, P)) return rc;
+    P.shared = cfg->shared != 0;
     Stream st;
     HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
     if (int rc = stage_batch(P, st.s)) return rc;

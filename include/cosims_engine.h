@@ -231,14 +231,18 @@ typedef struct cosims_ccs {
 int cosims_run(const cosims_mol *m, const cosims_params *p, const cosims_run_cfg *cfg, double *omega,
                double *omega_err, cosims_ccs *out);
 
-/* Many CCS runs sharing persistent launches.  The reference runs one
- * molecule per process (src/main.cpp:119-140, one monteTraj::runTraj each);
- * here molecules of uneven size share one work list per launch shape (force
- * type x lanes per trajectory x workgroup size), larger molecules first,
- * each group of lanes reading the atoms of its own trajectory's molecule;
- * the launches of a batch run concurrently on separate streams.  Each job's
- * results are those cosims_run gives for the same molecule, seed, job id
- * and launch shape.
+/* Many CCS runs in one call.  The reference runs one molecule per process
+ * (src/main.cpp:119-140, one monteTraj::runTraj each); here all jobs are
+ * sampled into one work list, ordered by launch shape (force type x lanes
+ * per trajectory x workgroup size), larger molecules first.  With
+ * cfg.shared, the molecules of uneven size of a launch shape share one
+ * persistent launch, each group of lanes reading the atoms of its own
+ * trajectory's molecule through L2; by default a job of an exact type with
+ * enough work runs a launch of its own with its molecule in LDS (faster),
+ * and only the rest share launches.  The launches of a batch run
+ * concurrently on separate streams.  Each job's results are those
+ * cosims_run gives for the same molecule, seed, job id and launch shape, in
+ * either mode.
  *
  * Lanes per trajectory follow the molecule's size: size class k holds the
  * molecules whose force loop sweeps n units (atoms, or clusters for the
@@ -265,6 +269,10 @@ typedef struct cosims_batch_cfg {
     int32_t blocks_per_cu;    /* 0 => occupancy limit */
     int32_t class_lanes[COSIMS_SIZE_CLASSES];
     int32_t class_wg[COSIMS_SIZE_CLASSES];
+    int32_t shared;           /* 0: a job of an exact type whose work is large enough runs a launch of
+                                 its own (its molecule staged in LDS, or tiled through LDS); 1: every
+                                 job of a launch shape shares one persistent launch, each group
+                                 reading its trajectory's molecule through L2 */
 } cosims_batch_cfg;
 
 typedef struct cosims_batch_stats {

@@ -222,3 +222,20 @@ def test_batch_tiny_jobs_share_a_table_launch(ion128, ion2000):
         single = m.run(E.PREC_MIXED, job=spec["job"], iters=1, traj=spec["traj"], **shape)
         assert np.array_equal(mix[k]["omega"], single["omega"]), k
         assert mix[k]["total_steps"] == single["total_steps"], k
+
+
+@pytest.mark.parametrize("prec", [E.PREC_MIXED, E.PREC_FP64])
+def test_batch_shared_launch_equals_own_launches(ion128, ion2000, ion8000, prec):
+    """shared=True: every job of a launch shape in one persistent launch
+    (molecules of 128, 2,000 and 8,000 atoms read through L2); the results
+    are those of the default own launches (LDS-staged, tiled) bit for bit."""
+    shape = dict(lanes=16, wg_size=256)
+    jobs = [(ion128, dict(job=71, iters=1, traj=1500)), (ion2000, dict(job=72, iters=1, traj=1000)),
+            (ion8000, dict(job=73, iters=1, traj=400))]
+    shared, st = E.run_batch(jobs, prec=prec, shared=True, **shape)
+    assert st["launches"] == 1
+    own, st2 = E.run_batch(jobs, prec=prec, **shape)
+    assert st2["launches"] == 3
+    for k in range(len(jobs)):
+        assert np.array_equal(shared[k]["omega"], own[k]["omega"]), (prec, k)
+        assert shared[k]["total_steps"] == own[k]["total_steps"], (prec, k)
