@@ -10,15 +10,17 @@
 // getPotential across the group, and a failed attempt restarts at once with
 // dt/(k+1): one launch covers every attempt of every trajectory.
 //
-// Two kernels share the trajectory state machine (group_loop):
+// Three kernels share the trajectory state machine (group_loop):
 //   k_traj_grp    one molecule, staged once per workgroup in LDS (SoA: FP32
 //                 x,y,z,q | LJ constants A, B as float4 + float2, or the FP64
 //                 x,y,z,q,sigma^6,24 eps/mu), exact force types 4 and 6;
+//   k_traj_tile   one molecule too large for LDS, exact types, streamed
+//                 through LDS in tiles by the whole workgroup, which
+//                 integrates in lock-step;
 //   k_traj_multi  any number of molecules of one force type, read from HBM
 //                 through L2 (a table of MolRec, one molecule per trajectory):
-//                 molecules of uneven size in one launch, molecules too large
-//                 for LDS, and the clustered force types 1, 2, 3, 5 (lanes
-//                 split the clusters).
+//                 molecules of uneven size in one launch, and the clustered
+//                 force types 1, 2, 3, 5 (lanes split the clusters).
 // Atom arrays are padded to a multiple of 64 with inert atoms (charge and LJ
 // parameters zero, far away) so every lane of a group runs the same trip
 // count.
