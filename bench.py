@@ -268,6 +268,19 @@ def clustered_workload(E, synth, traj, oracle_sample):
         out[label]["rel_dev_vs_fp64"] = (r["wmean"] - f["wmean"]) / f["wmean"]
         out[label]["paired_rel_dev_vs_fp64"] = float(dd.mean() / f["omega"].mean())
         out[label]["paired_rel_se"] = float(dd.std(ddof=1) / np.sqrt(dd.size) / f["omega"].mean())
+    # the clustered field in FP64 on the same initial conditions: the error of mixed precision on
+    # its own force model (the deviation from the exact sum above is the model's)
+    m1, r1 = runs["clustered"]
+    t1 = time.perf_counter()
+    f1 = m1.run(E.PREC_FP64, job=0, keep=True, lanes=64, wg_size=512)
+    dt = time.perf_counter() - t1
+    d1 = r1["omega"] - f1["omega"]
+    log(f"c4 clustered FP64: {dt:.2f} s, CCS {f1['wmean']:.2f} +- {f1['werr']:.2f}")
+    out["clustered_fp64"] = {"trajectories_per_s": traj / dt, "ccs": f1["wmean"], "ccs_err": f1["werr"],
+                             "lanes": f1["lanes"]}
+    out["clustered"]["paired_rel_dev_vs_clustered_fp64"] = float(d1.mean() / f1["omega"].mean())
+    out["clustered"]["paired_rel_se_vs_clustered_fp64"] = float(d1.std(ddof=1) / np.sqrt(d1.size)
+                                                                / f1["omega"].mean())
     if oracle_sample > 0:
         import oracle
         A = m4.atoms()
