@@ -58,6 +58,22 @@ def golden_engine_mol(g, flags):
                                     sc["dispersion_radius"], sc["multipole_radius"])
 
 
+def paired_ccs_dev(chi_a, st_a, chi_b, st_b):
+    """Relative CCS difference of run a against run b on the same initial
+    conditions, and its standard error.  Every sampled trajectory (status
+    != 0: accepted, or failed at chi = pi/2) adds the same constant times
+    1 - cos(chi) to the CCS sum (src/monteTraj.cpp:471-477), so the paired
+    difference is the sum of the per-trajectory differences of 1 - cos(chi)
+    over the sum of run b's, and its standard error follows from their
+    spread.  Mixed precision and FP64 agree to ~1e-9 rad on almost every
+    trajectory; the difference is carried by the few chaotic ones (about one
+    in a thousand) whose end states part, so it shrinks as 1/sqrt(n)."""
+    wa = np.where(np.asarray(st_a) != 0, 1.0 - np.cos(chi_a), 0.0)
+    wb = np.where(np.asarray(st_b) != 0, 1.0 - np.cos(chi_b), 0.0)
+    d = wa - wb
+    return d.sum() / wb.sum(), d.std(ddof=1) * np.sqrt(d.size) / wb.sum()
+
+
 def golden_engine_setup(g, threads=0, setup_gpu=0):
     """Engine molecule set up from the case's raw input files, as the
     reference's main.cpp reads them."""

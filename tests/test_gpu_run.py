@@ -7,7 +7,8 @@ conditions; the same ones are fed to the oracle.
 
 Tolerances: FP64 GPU per-trajectory median |dchi| <= 1e-8 rad on non-failed
 trajectories and CCS within 0.05 %; mixed precision CCS within 0.1 % of the
-FP64 oracle on the same (paired) initial conditions.
+FP64 oracle on the same (paired) initial conditions, or within 4 standard
+errors of the paired difference where that is larger.
 """
 import os
 
@@ -15,6 +16,7 @@ import numpy as np
 import pytest
 
 import oracle
+from conftest import paired_ccs_dev
 from cosims_amd import engine as E
 from cosims_amd import synth
 
@@ -98,7 +100,8 @@ def test_mixed_run_matches_fp64_oracle(ion128, oracle_run):
     assert np.array_equal(r["status"] != 0, o["hit"] == 1)
     fin = oracle.reduce_final(o["omega"], o["err"])
     dev = abs(r["wmean"] - fin["wmean"]) / fin["wmean"]
-    assert dev <= 1e-3, dev
+    _, se = paired_ccs_dev(r["chi"], r["status"], o["chi"], o["hit"])
+    assert dev <= max(1e-3, 4 * se), (dev, se)
 
 
 @pytest.mark.parametrize("lanes", [1, 8])
