@@ -1862,15 +1862,27 @@ int tune_shapes(int units, RunFn run, int &L_out, int &W_out)
         if (!seen && fin.size() < 3) fin.push_back(c);
     }
     cands.swap(fin);
-    for (Cand &c : cands) {
-        double a = 0, b = 0;
-        if (int rc = run(1, c.L, c.W, a)) return rc;
-        if (int rc = run(1, c.L, c.W, b)) return rc;
-        c.ms = std::min(a, b);
-    }
+    // each finalist three times on sets of the run's size (set 3: another
+    // Philox stream), the mean counts: with few lanes a trajectory the
+    // longest trajectories run about as long as the launch, and whether the
+    // wavefronts sharing their SIMDs leave early decides the launch's end,
+    // which differs from run to run on the same trajectories (2,000 atoms,
+    // 16 lanes: 530 or 640 ms, 32 lanes: 545-560 ms, equal on average).
+    // Within 4 % of the best, the most lanes win (the steadier drain).
+    for (Cand &c : cands)
+        if (int rc = run(1, c.L, c.W, c.ms)) return rc;
+    for (int set : {3, 1})
+        for (Cand &c : cands) {
+            double b = 0;
+            if (int rc = run(set, c.L, c.W, b)) return rc;
+            c.ms += b;
+        }
     std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
-    L_out = cands[0].L;
-    W_out = cands[0].W;
+    Cand pick = cands[0];
+    for (const Cand &c : cands)
+        if (c.ms <= 1.04 * cands[0].ms && c.L > pick.L) pick = c;
+    L_out = pick.L;
+    W_out = pick.W;
     return 0;
 }
 
@@ -1897,8 +1909,8 @@ int autotune(const cosims_mol *M, int prec, int64_t n_hint, int &L_out, int &wg_
     Stream st;
     HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
     // stage 1 holds several rounds even at 4 lanes per trajectory (16 trajectories per wavefront)
-    const int64_t nset[3] = {std::min<int64_t>(n2, std::max<int64_t>(16384, (int64_t)M->dev.cus * 256)), n2,
-                             std::min<int64_t>(n2, (int64_t)M->dev.cus * 16)};
+    const int64_t nset[4] = {std::min<int64_t>(n2, std::max<int64_t>(16384, (int64_t)M->dev.cus * 256)), n2,
+                             std::min<int64_t>(n2, (int64_t)M->dev.cus * 16), n2};
     TrajBufs B;
     if (int rc = B.alloc(std::max(nset[1], nset[0]))) return rc;
     int cur = -1;
@@ -2343,7 +2355,7 @@ int tune_batch(const cosims_batch_job *jobs, int32_t njobs, int prec, ShapeTable
             T = std::min<int64_t>(T, std::max<int64_t>(1, it) * std::max<int64_t>(1, tr));
         }
         auto cap = [&](int64_t d) { return std::min<int64_t>(65536, std::max<int64_t>(256, T / d)); };
-        const int64_t caps[3] = {cap(8), cap(2), cap(32)};
+        const int64_t caps[4] = {cap(8), cap(2), cap(32), cap(2)};
         BatchPlan set;
         auto run = [&](int k, int L, int W, double &ms) -> int {
             ShapeTable fixed;
