@@ -29,8 +29,9 @@ FLOP_PAIR, FLOP_EPI, FLOP_VERLET = 58, 21, 50   # algorithmic FLOPs, type-4 pair
 PEAK_FP32 = 157.3          # TFLOP/s, MI355X FP32 vector (packed) peak
 PEAK_FP32_UNPACKED = 78.6  # TFLOP/s, FP32 vector without packed math (= FP64 vector)
 # HBM bytes per launch of the timed mixed kernel (rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0,
-# scale factors from a 256 MiB device copy), from profiles/traj_grp_mixed_ion2000.md
-TRAFFIC_BYTES = (114577 * 2 + 133501) * 1024
+# scale factors from a 256 MiB device copy; 32 lanes, workgroup 1024), from
+# profiles/traj_grp_mixed_ion2000.md
+TRAFFIC_BYTES = (147674 * 2 + 186291) * 1024
 
 
 def log(*a):
@@ -327,6 +328,9 @@ def main():
     ap.add_argument("--traj", type=int, default=10000, help="trajectories per CCS integral")
     ap.add_argument("--iters", type=int, default=100, help="CCS integrals per step")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--lanes", type=int, default=0,
+                    help="lanes per trajectory of the mixed run (0: the autotuner's; fixes the shape for profiling)")
+    ap.add_argument("--wg", type=int, default=0, help="workgroup size of the mixed run (0: the autotuner's)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--serve-requests", type=int, default=40, help="requests of the serving stream (0: none)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
@@ -375,15 +379,16 @@ def main():
         f"axes=({inf['a']:.2f},{inf['b']:.2f},{inf['c']:.2f})")
     n_traj = a.traj * a.iters
 
+    shape = dict(lanes=a.lanes, wg_size=a.wg)
     for w in range(a.warmup):
-        r = m.run(E.PREC_MIXED, job=0)
+        r = m.run(E.PREC_MIXED, job=0, **shape)
         log(f"warmup {w}: {r['run_ms']:.1f} ms")
     E.device_sync()
     times, runs = [], []
     t_all = time.perf_counter()
     for k in range(a.steps):
         t1 = time.perf_counter()
-        r = m.run(E.PREC_MIXED, job=0)
+        r = m.run(E.PREC_MIXED, job=0, **shape)
         times.append(time.perf_counter() - t1)
         runs.append(r)
         log(f"step {k}: {times[-1] * 1e3:.1f} ms, traj kernel {r['traj_kernel_ms']} ms, "
