@@ -1381,12 +1381,14 @@ __global__ void __launch_bounds__(256) k_order_scatter(const int32_t *items, int
 
 // Enqueues the reorder of items[*first .. *last) (bounds on the device, or
 // [0, n) when last is NULL) into out at the same positions; n bounds the
-// range.  hist: 2 kOrderBins int32 of scratch.
+// range.  hist: 2 kOrderBins int32 of scratch, cleared here unless the
+// caller has (clear false).
 int order_items(const int32_t *items, int32_t *out, const int32_t *first, const int32_t *last, int64_t n,
-                const uint8_t *bin, int32_t *hist, hipStream_t s)
+                const uint8_t *bin, int32_t *hist, hipStream_t s, bool clear = true)
 {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL(k_fill_i32, dim3(2 * kOrderBins / 64), dim3(64), 0, s, hist, (int64_t)2 * kOrderBins, 0);
+    if (clear)
+        hipLaunchKernelGGL(k_fill_i32, dim3(2 * kOrderBins / 64), dim3(64), 0, s, hist, (int64_t)2 * kOrderBins, 0);
     const unsigned nh = (unsigned)std::min<int64_t>(1024, (n + 4095) / 4096);
     hipLaunchKernelGGL(k_order_hist, dim3(nh), dim3(256), 0, s, items, first, last, n, bin, hist);
     const unsigned nc = (unsigned)((n + kOrderChunk - 1) / kOrderChunk);
@@ -2228,8 +2230,24 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
     std::vector<int> order(nl);
     for (int i = 0; i < nl; i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return plan[a].work > plan[b].work; });
-    HIP_CHECK(hipEventRecord(P.ev0, s));
     const int32_t *ends = P.job_end.as<int32_t>();
+    // every launch's range of the work list, longest expected trajectories
+    // first, into items[1], before the launches start: the small sort
+    // kernels would otherwise wait for CU slots behind running launches
+    if (kOrderWork && nl > 0) {
+        const int64_t ns = (int64_t)2 * kOrderBins * nl;
+        hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s, P.order_scr.as<int32_t>(),
+                           ns, 0);
+        for (int li = 0; li < nl; li++) {
+            const Launch &l = plan[li];
+            if (int rc = order_items(P.B.items[0].as<int32_t>(), P.B.items[1].as<int32_t>(),
+                                     l.i0 ? ends + l.i0 - 1 : nullptr, ends + l.i1 - 1, l.traj,
+                                     P.B.bin.as<uint8_t>(), P.order_scr.as<int32_t>() + 2 * kOrderBins * li, s,
+                                     false))
+                return rc;
+        }
+    }
+    HIP_CHECK(hipEventRecord(P.ev0, s));
     for (int q = 0; q < nl; q++) {
         const int li = order[q];
         const Launch &l = plan[li];
@@ -2240,15 +2258,7 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
         HIP_CHECK(hipStreamWaitEvent(x, P.ev0, 0));
         const int32_t *first = l.i0 ? ends + l.i0 - 1 : nullptr;
         const int32_t *last = ends + l.i1 - 1;
-        // the launch's range of the work list, longest expected trajectories first
-        const int32_t *work = P.B.items[0].as<int32_t>();
-        if (kOrderWork) {
-            if (int rc = order_items(work, P.B.items[1].as<int32_t>(), first, last, cnt, P.B.bin.as<uint8_t>(),
-                                     P.order_scr.as<int32_t>() + 2 * kOrderBins * li, x))
-                return rc;
-            work = P.B.items[1].as<int32_t>();
-        }
-        TrajIO io = traj_io(P.B, work, cnt);
+        TrajIO io = traj_io(P.B, (kOrderWork ? P.B.items[1] : P.B.items[0]).as<int32_t>(), cnt);
         io.counter = P.seg_ctr.as<int32_t>() + li;
         io.evals = P.seg_evals.as<unsigned long long>() + 4 * li;
         if (l.own) {
