@@ -1866,11 +1866,11 @@ int tune_shapes(int units, RunFn run, int &L_out, int &W_out)
     cands.swap(fin);
     // each finalist three times on sets of the run's size (set 3: another
     // Philox stream), the mean counts: with few lanes a trajectory the
-    // longest trajectories run about as long as the launch, and whether the
-    // wavefronts sharing their SIMDs leave early decides the launch's end,
-    // which differs from run to run on the same trajectories (2,000 atoms,
-    // 16 lanes: 530 or 640 ms, 32 lanes: 545-560 ms, equal on average).
-    // Within 4 % of the best, the most lanes win (the steadier drain).
+    // longest trajectories run about as long as the launch, so a launch's
+    // end can differ from run to run on the same trajectories (2,000 atoms
+    // at 16 lanes without the raised issue priority of long trajectories:
+    // 520 or 630 ms; with it 514-533 ms).  Within 2 % of the best, the most
+    // lanes win (the steadier drain).
     for (Cand &c : cands)
         if (int rc = run(1, c.L, c.W, c.ms)) return rc;
     for (int set : {3, 1})
@@ -1882,7 +1882,7 @@ int tune_shapes(int units, RunFn run, int &L_out, int &W_out)
     std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
     Cand pick = cands[0];
     for (const Cand &c : cands)
-        if (c.ms <= 1.04 * cands[0].ms && c.L > pick.L) pick = c;
+        if (c.ms <= 1.02 * cands[0].ms && c.L > pick.L) pick = c;
     L_out = pick.L;
     W_out = pick.W;
     return 0;
