@@ -29,9 +29,9 @@ FLOP_PAIR, FLOP_EPI, FLOP_VERLET = 58, 21, 50   # algorithmic FLOPs, type-4 pair
 PEAK_FP32 = 157.3          # TFLOP/s, MI355X FP32 vector (packed) peak
 PEAK_FP32_UNPACKED = 78.6  # TFLOP/s, FP32 vector without packed math (= FP64 vector)
 # HBM bytes per launch of the timed mixed kernel (rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0,
-# scale factors from a 256 MiB device copy; 32 lanes, workgroup 1024), from
+# scale factors from a 256 MiB device copy; 16 lanes, workgroup 256), from
 # profiles/traj_grp_mixed_ion2000.md
-TRAFFIC_BYTES = (147674 * 2 + 186291) * 1024
+TRAFFIC_BYTES = (147044 * 2 + 185893) * 1024
 
 
 def log(*a):
