@@ -339,6 +339,8 @@ def main():
                     help="rerun served requests of up to this many atoms in FP64 over the same blocks (0: none)")
     ap.add_argument("--serve-class-wg", type=str, default="0,0,0,1024,1024",
                     help="workgroup size per size class of the serving stream (0: the server's default)")
+    ap.add_argument("--serve-class-lanes", type=str, default="0,0,0,0,0",
+                    help="lanes per trajectory per size class of the serving stream (0: the size rule)")
     ap.add_argument("--serve-compare", type=int, default=1,
                     help="also run the stream with equal-share slices (1) or not (0)")
     ap.add_argument("--workload", choices=("headline", "batch", "clustered"), default="headline",
@@ -453,13 +455,14 @@ def main():
     out["served_latency_ms"] = None
     if a.serve_requests > 0:
         cwg = [int(x) for x in a.serve_class_wg.split(",")]
+        cl = [int(x) for x in a.serve_class_lanes.split(",")]
         sv = serving(E, synth, a.serve_requests, a.serve_rate, a.serve_target, fp64_max_atoms=a.serve_fp64_atoms,
-                     class_wg=cwg)
+                     class_wg=cwg, class_lanes=cl)
         out["serving"] = sv
         out["served_latency_ms"] = {"p50": sv["latency_ms"]["p50"], "p99": sv["latency_ms"]["p99"]}
         if a.serve_compare:
             eq = serving(E, synth, a.serve_requests, a.serve_rate, a.serve_target, policy=E.SERVE_EQUAL_SHARE,
-                         class_wg=cwg)
+                         class_wg=cwg, class_lanes=cl)
             out["serving_equal_share"] = {k: eq[k] for k in ("latency_ms", "served_rate", "trajectories_per_s",
                                                              "makespan_s", "slices", "trajectories_issued")}
     print(json.dumps(out), flush=True)
