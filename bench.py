@@ -100,6 +100,11 @@ def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, fp64_max_at
     # tiles and one LDS staging shared by more groups); 950-960 k traj/s against 800-890 k at 512
     # on the same box
     server_kw.setdefault("class_wg", [0, 0, 0, 1024, 1024])
+    # twice the size rule's lanes below 1,536 atoms: a served slice is a few hundred thousand
+    # trajectories, so its drain weighs more than in one large run (p50 2.37-2.39 s against
+    # 2.57-2.58 s, p99 and throughput no worse, same box; half the rule's lanes: 840 k against
+    # 971 k traj/s)
+    server_kw.setdefault("class_lanes", [8, 16, 32, 64, 64])
     log(f"serving: {n_requests} requests at {rate}/s, target {target}, lanes by size, wg {W}, {server_kw}")
     srv = E.Server(prec=E.PREC_MIXED, lanes=0, wg_size=W, block_traj=10000, min_blocks=3, max_blocks=1000,
                    policy=policy, **server_kw)
@@ -153,7 +158,7 @@ def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, fp64_max_at
         # time some slice was in flight
         "achieved_tflops_busy": (st["pair_evals"] * FLOP_PAIR + st["force_evals"] * FLOP_EPI
                                  + st["total_steps"] * FLOP_VERLET) / (st["busy_ms"] * 1e-3) / 1e12,
-        "lanes_per_trajectory": "by size class (4 x 2^class)", "wg_size": W,
+        "lanes_per_trajectory": "by size class", "class_lanes": server_kw.get("class_lanes"), "wg_size": W,
         "class_wg": server_kw["class_wg"],
         "prior_rel_sd": server_kw["prior_rel_sd"], "issue_fraction": server_kw["issue_fraction"],
         "policy": "shortest remaining work first" if policy == E.SERVE_SHORTEST_FIRST else "equal share",
@@ -339,8 +344,9 @@ def main():
                     help="rerun served requests of up to this many atoms in FP64 over the same blocks (0: none)")
     ap.add_argument("--serve-class-wg", type=str, default="0,0,0,1024,1024",
                     help="workgroup size per size class of the serving stream (0: the server's default)")
-    ap.add_argument("--serve-class-lanes", type=str, default="0,0,0,0,0",
-                    help="lanes per trajectory per size class of the serving stream (0: the size rule)")
+    ap.add_argument("--serve-class-lanes", type=str, default="8,16,32,64,64",
+                    help="lanes per trajectory per size class of the serving stream (0: the size rule, "
+                         "4 x 2^class)")
     ap.add_argument("--serve-compare", type=int, default=1,
                     help="also run the stream with equal-share slices (1) or not (0)")
     ap.add_argument("--workload", choices=("headline", "batch", "clustered"), default="headline",
