@@ -359,6 +359,10 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 3 device-to-device copies of 256 MiB (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); trajectories shard "
+              "trivially across GPUs but no multi-GPU path is built", file=sys.stderr)
+        sys.exit(2)
     if a.copy_probe:
         copy_probe()
         return
@@ -371,10 +375,6 @@ def main():
             res = clustered_workload(E, synth, a.c4_traj, a.oracle_sample)
         print(json.dumps({"workload": a.workload, **res}), flush=True)
         return
-    if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); trajectories shard "
-              "trivially across GPUs but no multi-GPU path is built", file=sys.stderr)
-        sys.exit(2)
 
     from cosims_amd import engine as E
     from cosims_amd import synth

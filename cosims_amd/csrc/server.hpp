@@ -41,7 +41,9 @@ struct SJob {
     std::vector<double> omega, omega_err, sums;  // per block
     bool finished = false;
     DBuf dsums;                         // per-block sums on the device (max_blocks x kSums)
-    size_t bytes = 0;
+    size_t bytes = 0;                   // charged against the budget from admission ...
+    int inflight = 0;                   // ... until it has retired and no slice in flight holds it
+    bool released = false;
     double t_submit = 0, t_ready = 0, t_admit = 0, t_done = 0, setup_ms = 0;
     cosims_ccs ccs{};
     ~SJob()
@@ -63,8 +65,10 @@ struct cosims_server {
     int setups_running = 0;
     std::deque<std::shared_ptr<SJob>> queued, ready;
     std::vector<std::shared_ptr<SJob>> active;
+    std::vector<std::shared_ptr<SJob>> retiring;  // retired, still held by a slice in flight
     std::map<int64_t, std::shared_ptr<SJob>> jobs;
     size_t bytes_in_use = 0, bytes_peak = 0, budget = 0;
+    size_t slice_reserve = 0;         // trajectory buffers of pipeline_depth slices, outside the budget
     double units_per_ms = 1.0e9;      // force-loop work per kernel ms (learned from the slices)
     cosims_server_stats st{};
     std::vector<std::thread> workers;
@@ -115,8 +119,9 @@ void setup_worker(cosims_server *S)
                 S->done_cv.notify_all();
             } else {
                 J->mol = m;
-                J->bytes = m->dbytes + sizeof(double) * kSums * S->cfg.max_blocks +
-                           (size_t)S->cfg.block_traj * kTrajBytes * 2;
+                // the molecule and its per-block sums; the slices' trajectory
+                // buffers are reserved once, outside the budget
+                J->bytes = m->dbytes + sizeof(double) * kSums * S->cfg.max_blocks;
                 J->state = 2;
                 S->ready.push_back(J);
             }
@@ -274,6 +279,7 @@ int issue_slice(cosims_server *S, Slice &X)
         P.n += b.n;
         P.jobs.push_back(b);
         J.issued += e.nb;
+        J.inflight++;
     }
     if (P.n > INT32_MAX / 4) return fail(COSIMS_E_INVALID, "server: slice too large");
     if (int rc = order_plan(P)) return rc;
@@ -348,6 +354,23 @@ int collect_slice(cosims_server *S, Slice &X)
     S->st.force_evals += R.evals;
     S->st.total_steps += R.steps;
     return 0;
+}
+
+// Gives a retired request's device bytes back to the budget once no slice in
+// flight holds it any more (its molecule and sums stay alive until then).
+// Caller holds S->mu.
+void release_retired(cosims_server *S)
+{
+    std::vector<std::shared_ptr<SJob>> keep;
+    for (auto &J : S->retiring) {
+        if (J->inflight > 0) {
+            keep.push_back(J);
+        } else if (!J->released) {
+            S->bytes_in_use -= J->bytes;
+            J->released = true;
+        }
+    }
+    S->retiring.swap(keep);
 }
 
 // The scheduler keeps up to pipeline_depth slices in flight: whenever a place is
@@ -431,7 +454,11 @@ void scheduler(cosims_server *S)
                 const double now = S->now_ms();
                 S->st.busy_ms += now - std::max(X.t_issue, last_end);
                 last_end = now;
+                for (auto &e : X.e) e.job->inflight--;
             }
+            if (rc)  // every stream is drained: no slice holds a request any more
+                for (auto &f : fly)
+                    for (auto &e : f.e) e.job->inflight = 0;
             std::vector<std::shared_ptr<SJob>> keep;
             for (auto &J : S->active) {
                 if (rc || J->finished) {
@@ -439,7 +466,7 @@ void scheduler(cosims_server *S)
                     J->err = rc;
                     J->errmsg = msg;
                     J->t_done = S->now_ms();
-                    S->bytes_in_use -= J->bytes;
+                    S->retiring.push_back(J);
                     if (rc) S->st.failed++;
                     else S->st.done++;
                 } else {
@@ -447,6 +474,7 @@ void scheduler(cosims_server *S)
                 }
             }
             S->active.swap(keep);
+            release_retired(S);
         }
         S->done_cv.notify_all();
         if (rc) {
@@ -512,6 +540,15 @@ int cosims_server_create(const cosims_server_cfg *cfg, cosims_server **out)
         HIP_CHECK(hipMemGetInfo(&fr, &tot));
         S->budget = fr / 5 * 4;
     }
+    // the trajectory buffers of pipeline_depth slices (a slice holds at most
+    // slice_traj + min_slice_traj trajectories, and one block more) come out
+    // of the budget before any request is admitted
+    S->slice_reserve = (size_t)c.pipeline_depth * (size_t)(c.slice_traj + c.min_slice_traj + c.block_traj) *
+                       (size_t)kTrajBytes;
+    if (S->slice_reserve >= S->budget)
+        return fail(COSIMS_E_NOMEM, "server: hbm_budget does not cover the slices' trajectory buffers (" +
+                                        std::to_string(S->slice_reserve) + " bytes)");
+    S->budget -= S->slice_reserve;
     cosims_server *raw = S.get();
     for (int k = 0; k < c.setup_threads; k++) S->workers.emplace_back(setup_worker, raw);
     S->sched = std::thread(scheduler, raw);

@@ -122,3 +122,59 @@ def run_cli_example(entry, work, extra=(), timeout=600):
     args += ["-s", str(entry["seed"]), *extra]
     r = subprocess.run(args, cwd=work, capture_output=True, text=True, timeout=timeout)
     return r.returncode, r.stdout + r.stderr, os.path.join(work, os.path.basename(entry["log"]))
+
+
+def digest(a, dtype):
+    """SHA-256 of an array's little-endian bytes: how the large ion's golden
+    record (tests/golden/make_golden.py) stores prepared atoms and clusters."""
+    import hashlib
+    b = np.ascontiguousarray(np.asarray(a, dtype=dtype).reshape(-1)).astype(np.dtype(dtype).newbyteorder("<"))
+    return hashlib.sha256(b.tobytes()).hexdigest()
+
+
+_LARGE = {}
+
+
+def large_ion_file():
+    """The 20,000-atom synthetic ion (radius 40 A, +25 e, q ~ N(0, 0.4 e))
+    written by cosims_amd.synth, checked against the SHA-256 its golden
+    record holds before anything compares against it."""
+    import hashlib
+    import tempfile
+    if "path" not in _LARGE:
+        from cosims_amd import synth
+        path = os.path.join(tempfile.mkdtemp(prefix="cosims_c4_"), "ion20000.mfj")
+        synth.write_mfj(path, "ion20000", synth.config_ion("ion20000"))
+        want = golden("ion20000")["mol_sha256"]
+        assert hashlib.sha256(open(path, "rb").read()).hexdigest() == want, "synthetic input changed"
+        _LARGE["path"] = path
+    return _LARGE["path"]
+
+
+def large_ion_params(g, multipole, cutoff, **kw):
+    """Engine run parameters of the large ion's golden input, with the
+    force-type switches set."""
+    from cosims_amd import engine as E
+    sc = g["scalars"]
+    return E.default_params(temp=sc["temp"], dt=sc["dt"], multipole=multipole, dispersion_cutoff=cutoff,
+                            multipole_radius=sc["multipole_radius"], dispersion_radius=sc["dispersion_radius"],
+                            max_cluster_size=int(sc["max_cluster_size"]), multipole_order=int(sc["multipole_order"]),
+                            seed=20261203, **kw)
+
+
+def check_large_ion_setup(g, m, clusters=True):
+    """Engine setup of the large ion against the reference's record."""
+    sc = g["scalars"]
+    inf = m.info()
+    assert inf["n"] == 20000
+    assert (inf["a"], inf["b"], inf["c"], inf["bmax"], inf["mu"]) == (sc["a"], sc["b"], sc["c"], sc["bmax"], sc["mu"])
+    A = m.atoms()
+    for k in ("xyz", "q", "sig6", "eps24m", "eps4"):
+        assert digest(A[k], np.float64) == g["atoms_sha256"][k], k
+    if clusters:
+        assert inf["ncl"] == int(sc["ncl"])
+        cl = m.clusters()
+        assert digest(cl["start"], np.int32) == g["pcm_sha256"]["start"]
+        assert digest(cl["atoms"], np.int32) == g["pcm_sha256"]["atoms"]
+        assert digest(cl["center"], np.float64) == g["pcm_sha256"]["center"]
+        assert digest(cl["maxdist"], np.float64) == g["pcm_sha256"]["maxdist"]

@@ -15,6 +15,12 @@ are what the reference's own classes return for them:
     integrals with their standard errors, the final statistics, and the
     lines logfilter.log_lines keeps.
 
+The 20,000-atom ion (ion20000) is stored compactly: its molecule file is
+regenerated by cosims_amd.synth (its SHA-256 is recorded so a test can
+check the input first), and the prepared atoms and PCM clusters are
+recorded as SHA-256 digests of their little-endian float64 / int32 bytes
+plus a few sums, instead of every value.
+
 Floats are written with repr() (round-trip exact).
 """
 from __future__ import annotations
@@ -57,6 +63,12 @@ CASES = [
          inp="temp 298\ntraj 10\niter 100\nmultipole false\ndispersion_cutoff false\n"
              "multipole_radius 20\ndispersion_radius 23.43015325287593\n", atoms=None,
          flags=[(1, 0, 0, 1), (1, 1, 1, 1)], nverlet=300),
+    # the large ion: exact (type 4) and clustered (type 1, the reference's
+    # defaults for a charged molecule) arms, 64 sampled lines each
+    dict(name="ion20000", mol="ion20000.mfj", synthetic=True, compact=True,
+         inp="temp 298\ntraj 500\niter 1\nmultipole true\ndispersion_cutoff true\n"
+             "multipole_radius 20\ndispersion_radius 23.43015325287593\n", atoms=None,
+         flags=[(1, 0, 0, 1), (1, 1, 1, 1)], nverlet=[64, 64]),
 ]
 
 SEED = 20261203
@@ -102,6 +114,18 @@ def ff(a):
     return [float(x) for x in np.asarray(a, dtype=np.float64).reshape(-1)]
 
 
+def digest(a, dtype):
+    """SHA-256 of an array's little-endian bytes (the compact record)."""
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(np.asarray(a, dtype=dtype).reshape(-1)).astype(
+        np.dtype(dtype).newbyteorder("<")).tobytes()).hexdigest()
+
+
+def file_digest(path):
+    import hashlib
+    return hashlib.sha256(open(path, "rb").read()).hexdigest()
+
+
 def run_case(L, case, work):
     molpath = os.path.join(work, case["mol"])
     argv = ["cosims", "-m", molpath, "-s", "123456789", "-n", "8"]
@@ -124,10 +148,20 @@ def run_case(L, case, work):
     cen = np.empty(3 * n + 3); md = np.empty(n + 1)
     ncl = L.ref_clusters(start, atoms, cen, md)
     out = dict(case=case["name"], mol=case["mol"], input=case["inp"], atoms_dat=case["atoms"],
-               scalars=sc,
-               atoms=dict(xyz=ff(xyz), q=ff(q), sig6=ff(sig6), eps24m=ff(e24), eps4=ff(e4), mass=ff(mass)),
-               pcm=dict(start=[int(x) for x in start[: ncl + 1]], atoms=[int(x) for x in atoms],
-                        center=ff(cen[: 3 * ncl]), maxdist=ff(md[:ncl])))
+               scalars=sc)
+    if case.get("compact"):
+        out["mol_sha256"] = file_digest(molpath)
+        out["atoms_sha256"] = dict(xyz=digest(xyz, np.float64), q=digest(q, np.float64),
+                                   sig6=digest(sig6, np.float64), eps24m=digest(e24, np.float64),
+                                   eps4=digest(e4, np.float64))
+        out["atoms_fsum"] = dict(xyz=[math.fsum(xyz[k::3]) for k in range(3)], q=math.fsum(q),
+                                 sig6=math.fsum(sig6), eps24m=math.fsum(e24), eps4=math.fsum(e4))
+        out["pcm_sha256"] = dict(start=digest(start[: ncl + 1], np.int32), atoms=digest(atoms, np.int32),
+                                 center=digest(cen[: 3 * ncl], np.float64), maxdist=digest(md[:ncl], np.float64))
+    else:
+        out["atoms"] = dict(xyz=ff(xyz), q=ff(q), sig6=ff(sig6), eps24m=ff(e24), eps4=ff(e4), mass=ff(mass))
+        out["pcm"] = dict(start=[int(x) for x in start[: ncl + 1]], atoms=[int(x) for x in atoms],
+                          center=ff(cen[: 3 * ncl]), maxdist=ff(md[:ncl]))
     rng = np.random.default_rng(SEED + n)
     X = xyz.reshape(-1, 3)
     pts = probe_points(X, sc["bmax"], 48, rng)
@@ -147,7 +181,10 @@ def run_case(L, case, work):
         forces.append(entry)
     verlets = []
     for k, fl in enumerate(case["flags"][:2]):
-        nv = case["nverlet"] if k == 0 else max(100, case["nverlet"] // 4)
+        if isinstance(case["nverlet"], list):
+            nv = case["nverlet"][k]
+        else:
+            nv = case["nverlet"] if k == 0 else max(100, case["nverlet"] // 4)
         L.ref_set_flags(*fl)
         ic = oracle.sample_ic(cfg, np.arange(nv) + 1000 * k)
         hit = ic["hit"].astype(bool)
@@ -239,6 +276,9 @@ def main(names=None):
             for f in os.listdir(HERE):
                 if f.endswith((".mfj", ".pdb", ".dat")):
                     os.symlink(os.path.join(HERE, f), os.path.join(work, f))
+            if case.get("synthetic"):  # written here, not kept next to this script
+                key = case["mol"][:-4]
+                synth.write_mfj(os.path.join(work, case["mol"]), key, synth.config_ion(key))
             L = load_ref()
             devnull = os.open(os.devnull, os.O_WRONLY)
             saved = os.dup(1)

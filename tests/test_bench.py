@@ -10,9 +10,14 @@ import bench
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def test_more_than_one_gpu_is_refused():
-    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
-                       text=True, timeout=120)
+import pytest
+
+
+@pytest.mark.parametrize("extra", [[], ["--workload", "batch"], ["--workload", "clustered"], ["--copy-probe"]])
+def test_more_than_one_gpu_is_refused(extra):
+    """Every workload refuses --gpus other than 1 before touching a GPU."""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2", *extra],
+                       capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU only" in r.stderr
     assert r.stdout == ""

@@ -1,0 +1,224 @@
+"""The three larger workloads on the GPU (MI355X), each against the
+reference's golden record, the oracle or the engine's own single runs.
+
+* 20,000-atom ion (radius 40 A, +25 e): the reference's setup digests, its
+  FP64 forces and potentials for the exact (type 4) and clustered (type 1)
+  force fields bit for bit, its trajectories through the lane kernel, the
+  tiled kernel (exact type, molecule streamed through LDS) and the 64-lane
+  clustered table kernel (median |dchi| <= 1e-8 rad), and mixed precision
+  against FP64 on paired initial conditions (CCS within 0.1 %).
+* 256 conformers, N log-uniform in 50-5,000: one call, default (own
+  launches) and shared modes; every conformer's per-integral CCS equals
+  cosims_run of that conformer alone bit for bit; every 16th conformer at
+  50,000 trajectories in mixed precision within 0.1 % of FP64 (paired).
+* A replayed Poisson stream of 12 requests (N 109-18,935 atoms, He 222-382
+  K, 20 req/s) through the request server to 0.5 % relative standard
+  error: every result equals cosims_run over the same blocks bit for bit.
+"""
+import concurrent.futures as cf
+import os
+import tempfile
+import time
+
+import numpy as np
+import pytest
+
+from cosims_amd import engine as E
+from cosims_amd import synth
+
+from conftest import check_large_ion_setup, golden, large_ion_file, large_ion_params, paired_ccs_dev
+
+pytestmark = pytest.mark.gpu
+
+
+# ---------------------------------------------------------------- 20,000 atoms
+@pytest.fixture(scope="module")
+def large():
+    """Engine molecules of the large ion set up on the GPU (boundary
+    marches), clustered (type 1) and exact (type 4)."""
+    g = golden("ion20000")
+    path = large_ion_file()
+    m1 = E.Molecule.from_file(path, params=large_ion_params(g, 1, 1, setup_gpu=1))
+    m4 = E.Molecule.from_file(path, params=large_ion_params(g, 0, 0, setup_gpu=1))
+    assert (m1.info()["force_type"], m4.info()["force_type"]) == (1, 4)
+    return g, {1: m1, 4: m4}
+
+
+def test_large_ion_gpu_setup_matches_reference(large):
+    g, m = large
+    check_large_ion_setup(g, m[1], clusters=True)
+    check_large_ion_setup(g, m[4], clusters=False)
+
+
+def test_large_ion_accel_fp64_bitexact(large):
+    g, m = large
+    for f in g["forces"]:
+        mol = m[f["force_type"]]
+        pts = np.array(f["points"]).reshape(-1, 3)
+        acc = mol.accel(pts, E.PREC_FP64)
+        ref = np.array(f["acc"]).reshape(-1, 3)
+        assert np.array_equal(acc, ref), (f["force_type"], np.abs(acc - ref).max())
+        assert np.array_equal(mol.potential(pts), f["pot"]), f["force_type"]
+
+
+def test_large_ion_accel_mixed_close(large):
+    g, m = large
+    for f in g["forces"]:
+        pts = np.array(f["points"]).reshape(-1, 3)
+        acc = m[f["force_type"]].accel(pts, E.PREC_MIXED)
+        ref = np.array(f["acc"]).reshape(-1, 3)
+        scale = np.abs(ref).max(axis=1, keepdims=True)
+        assert np.all(np.abs(acc - ref) <= 2e-4 * scale + 1e-12), (f["force_type"], np.abs(acc - ref).max())
+
+
+def test_large_ion_lane_kernel_matches_reference(large):
+    g, m = large
+    for v in g["verlet"]:
+        r = m[v["force_type"]].verlet(v["p0"], v["v0"], E.PREC_FP64)
+        ok = np.array(v["ok"])
+        assert np.mean(r["ok"] == ok) >= 0.95, v["force_type"]
+        both = (r["ok"] == 1) & (ok == 1)
+        d = np.abs(r["chi"][both] - np.array(v["chi"])[both])
+        assert np.median(d) <= 1e-8, (v["force_type"], np.median(d))
+
+
+# the golden trajectories are the hits among Philox (seed 20261203, job 7,
+# iteration 0) trajectories 0-63 (type 4) and 1000-1063 (type 1)
+GOLDEN_FIRST = {4: 0, 1: 1000}
+
+
+@pytest.mark.parametrize("ft,lanes,wg", [(4, 16, 256), (4, 64, 1024), (1, 64, 256)],
+                         ids=["tiled-16", "tiled-64", "clustered-64"])
+def test_large_ion_group_kernels_fp64_match_reference(large, ft, lanes, wg):
+    """FP64 group kernels on the golden initial conditions: the tiled kernel
+    for the exact type (20,000 atoms exceed one workgroup's LDS) and the
+    64-lane table kernel for the clustered type."""
+    g, m = large
+    v = next(x for x in g["verlet"] if x["force_type"] == ft)
+    first = GOLDEN_FIRST[ft]
+    r = m[ft].run(E.PREC_FP64, job=7, iters=1, traj=first + 64, lanes=lanes, wg_size=wg, keep=True)
+    sl = slice(first, first + 64)
+    hit = r["status"][sl] != 0
+    assert hit.sum() == len(v["chi"])
+    st = r["status"][sl][hit]
+    ok = np.array(v["ok"])
+    assert np.mean((st == 1) == (ok == 1)) >= 0.95
+    both = (st == 1) & (ok == 1)
+    d = np.abs(r["chi"][sl][hit][both] - np.array(v["chi"])[both])
+    assert np.median(d) <= 1e-8, (ft, lanes, np.median(d))
+
+
+@pytest.mark.parametrize("ft", [4, 1], ids=["exact", "clustered"])
+def test_large_ion_mixed_vs_fp64_paired(large, ft):
+    """Mixed precision against FP64 on the same 10,000 initial conditions,
+    same launch shape (tiled kernel / 64-lane clustered kernel)."""
+    _, m = large
+    shape = dict(lanes=64, wg_size=512) if ft == 1 else dict(lanes=16, wg_size=512)
+    kw = dict(job=11, iters=2, traj=5000, keep=True, **shape)
+    f64 = m[ft].run(E.PREC_FP64, **kw)
+    mix = m[ft].run(E.PREC_MIXED, **kw)
+    assert np.array_equal(mix["status"] != 0, f64["status"] != 0)
+    both = (mix["status"] == 1) & (f64["status"] == 1)
+    d = np.abs(mix["chi"][both] - f64["chi"][both])
+    assert np.median(d) <= 1e-6, np.median(d)
+    dev, se = paired_ccs_dev(mix["chi"], mix["status"], f64["chi"], f64["status"])
+    print(f"20,000 atoms type {ft}: paired CCS dev {dev:.2e} +- {se:.2e}")
+    assert abs(dev) <= 1e-3, (dev, se)
+
+
+# ---------------------------------------------------------------- 256 conformers
+def _make_mols(specs, threads=8):
+    """Engine molecules from (name, ion, params) in parallel (host setup
+    workers; the boundary marches run on the GPU)."""
+    d = tempfile.mkdtemp(prefix="cosims_cfg_")
+
+    def one(spec):
+        name, ion, p = spec
+        path = os.path.join(d, name + ".mfj")
+        synth.write_mfj(path, name, ion)
+        return E.Molecule.from_file(path, params=p)
+
+    with cf.ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(one, specs))
+
+
+@pytest.fixture(scope="module")
+def conformers():
+    confs = synth.uneven_conformers(count=256)
+    specs = [(f"c{k}", ion, E.default_params(temp=298.0, seed=7000 + k, multipole=0, dispersion_cutoff=0,
+                                             setup_gpu=1, traj=500, iter=1))
+             for k, (n, ion) in enumerate(confs)]
+    mols = _make_mols(specs)
+    ns = [m.info()["n"] for m in mols]
+    assert min(ns) < 60 and max(ns) > 4500
+    return mols
+
+
+def test_conformer_batch_equals_single_runs(conformers):
+    """256 uneven conformers, 500 trajectories each, size-rule lanes: one
+    call in the default mode (own launches for the larger jobs, shared
+    launches for the tiny ones) and in the shared mode; every conformer's
+    per-integral CCS equals cosims_run of it alone with its launch shape."""
+    jobs = [(m, dict(job=k, iters=1, traj=500)) for k, m in enumerate(conformers)]
+    own, st = E.run_batch(jobs, prec=E.PREC_MIXED, lanes=0)
+    shared, st2 = E.run_batch(jobs, prec=E.PREC_MIXED, lanes=0, shared=True)
+    assert st["n_traj"] == st2["n_traj"] == 256 * 500
+    assert st2["launches"] <= E.SIZE_CLASSES
+    for k, (m, spec) in enumerate(jobs):
+        single = m.run(E.PREC_MIXED, job=k, iters=1, traj=500, lanes=own[k]["lanes"], wg_size=own[k]["wg_size"])
+        assert np.array_equal(own[k]["omega"], single["omega"]), k
+        assert own[k]["total_steps"] == single["total_steps"], k
+        assert np.array_equal(shared[k]["omega"], own[k]["omega"]), k
+
+
+def test_conformer_mixed_vs_fp64_every_16th(conformers):
+    """Every 16th conformer at the workload's 50,000 trajectories: mixed
+    precision within 0.1 % of FP64 on the same initial conditions (same
+    streams, size-rule lanes)."""
+    jobs = [(conformers[k], dict(job=k, iters=1, traj=50000)) for k in range(0, 256, 16)]
+    mix, _ = E.run_batch(jobs, prec=E.PREC_MIXED, lanes=0)
+    f64, _ = E.run_batch(jobs, prec=E.PREC_FP64, lanes=0)
+    dev = np.array([(a["wmean"] - b["wmean"]) / b["wmean"] for a, b in zip(mix, f64)])
+    print("conformer CCS deviations (mixed vs FP64):", " ".join(f"{x:.1e}" for x in dev))
+    assert np.all(np.abs(dev) <= 1e-3), dev
+
+
+# ---------------------------------------------------------------- request stream
+STREAM_SEED = 5027  # 12 requests: 109-18,935 atoms, 222-382 K
+
+
+def test_poisson_stream_results_equal_single_runs():
+    reqs = synth.serving_stream(count=12, rate=20.0, seed=STREAM_SEED)
+    ns = [r["n"] for r in reqs]
+    assert min(ns) < 200 and max(ns) > 15000
+    d = tempfile.mkdtemp(prefix="cosims_stream_")
+    files = []
+    for k, r in enumerate(reqs):
+        path = os.path.join(d, f"req{k}.mfj")
+        synth.write_mfj(path, f"req{k}", r["ion"])
+        files.append(path)
+
+    def params(k):
+        return E.default_params(temp=reqs[k]["temp"], seed=STREAM_SEED + k, multipole=0, dispersion_cutoff=0,
+                                setup_gpu=1)
+    CL, CW = [8, 16, 32, 64, 64], [0, 0, 0, 1024, 1024]
+    with E.Server(prec=E.PREC_MIXED, lanes=0, wg_size=512, block_traj=10000, min_blocks=3, max_blocks=200,
+                  prior_rel_sd=2.0, class_lanes=CL, class_wg=CW) as srv:
+        t0 = time.perf_counter()
+        tickets = []
+        for k, r in enumerate(reqs):  # arrivals replayed in real time
+            delay = t0 + r["t"] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(E.read_molfile(files[k]), params(k), job=k, target=0.005))
+        res = [srv.wait(t) for t in tickets]
+    admits = sorted(r["t_admit_ms"] for r in res)
+    dones = sorted(r["t_done_ms"] for r in res)
+    assert admits[-1] > admits[0] and admits[-1] < dones[-1]  # later requests joined running work
+    for k, r in enumerate(res):
+        assert r["status"] == 0 and r["rel_err"] <= 0.005, (k, r["rel_err"])
+        m = E.Molecule.from_file(files[k], params=params(k))
+        run = m.run(E.PREC_MIXED, job=k, iters=r["blocks"], traj=10000, lanes=r["lanes"], wg_size=r["wg_size"])
+        assert np.array_equal(r["omega"], run["omega"]), (k, ns[k])
+        assert r["wmean"] == run["wmean"], k
+        m.close()

@@ -11,7 +11,8 @@ Tolerances: FP64 group vs FP64 lane kernel, per trajectory, median |dchi|
 <= 1e-8 rad on trajectories both accept, success flags equal on >= 99 %,
 per-integral CCS within 5e-4; mixed precision vs FP64 on paired initial
 conditions: median |dchi| <= 1e-6 rad, >= 99 % of trajectories within 1e-3
-rad, and the paired CCS difference within max(1e-3, 4 standard errors).  Batched runs: every job's per-integral CCS is
+rad, and the paired CCS difference within 1e-3 (its standard error is
+printed beside it).  Batched runs: every job's per-integral CCS is
 bit-identical to the same job run alone (same launch shape), and to
 cosims_run of the same molecule.
 """
@@ -69,7 +70,8 @@ def test_clustered_group_mixed_close_to_fp64(case, flags):
         assert np.median(d) <= 1e-6, (lanes, np.median(d))
         assert np.mean(d <= 1e-3) >= 0.99, lanes
         dev, se = paired_ccs_dev(mix["chi"], mix["status"], f64["chi"], f64["status"])
-        assert abs(dev) <= max(1e-3, 4 * se), (lanes, dev, se)
+        print(f"{case} {flags} lanes {lanes}: paired CCS dev {dev:.2e} +- {se:.2e}")
+        assert abs(dev) <= 1e-3, (lanes, dev, se)
 
 
 @pytest.fixture(scope="module")

@@ -100,8 +100,11 @@ def test_mixed_run_matches_fp64_oracle(ion128, oracle_run):
     assert np.array_equal(r["status"] != 0, o["hit"] == 1)
     fin = oracle.reduce_final(o["omega"], o["err"])
     dev = abs(r["wmean"] - fin["wmean"]) / fin["wmean"]
-    _, se = paired_ccs_dev(r["chi"], r["status"], o["chi"], o["hit"])
-    assert dev <= max(1e-3, 4 * se), (dev, se)
+    pdev, se = paired_ccs_dev(r["chi"], r["status"], o["chi"], o["hit"])
+    # the 0.1 % criterion on its own (the paired standard error is reported)
+    print(f"mixed vs FP64 oracle: dev {dev:.2e}, paired {pdev:.2e} +- {se:.2e}")
+    assert dev <= 1e-3, (dev, se)
+    assert abs(pdev) <= 1e-3, (pdev, se)
 
 
 @pytest.mark.parametrize("lanes", [1, 8])

@@ -125,15 +125,36 @@ def test_server_rejects_bad_issue_settings():
             _server(**kw)
 
 
+def _slice_reserve():
+    """Device bytes the server sets aside for its slices' trajectory buffers
+    (taken out of hbm_budget before any request is admitted)."""
+    big = 10 << 30
+    with _server(hbm_budget=big) as srv:
+        return big - srv.stats()["budget"]
+
+
+def test_server_budget_must_cover_slice_buffers():
+    reserve = _slice_reserve()
+    assert reserve >= 3 * (32768 + 65536 + BLOCK) * 100  # three slices in flight
+    with pytest.raises(E.CosimsError, match="out of memory"):
+        _server(hbm_budget=reserve)
+
+
 def test_server_budget_serialises_admission(mixed_results):
-    """A one-byte device-memory budget admits one request at a time; the
+    """A budget one byte beyond the slice buffers admits one request at a
+    time (its bytes come back only once no slice in flight holds it); the
     results do not change."""
     res, _ = mixed_results
-    with _server(hbm_budget=1) as srv:
+    with _server(hbm_budget=_slice_reserve() + 1) as srv:
         tickets = _submit_all(srv, REQUESTS)
         out = [srv.wait(t) for t in tickets]
+        # slices issued past the last stopping point still hold that request
+        t_end = time.time() + 60
+        while srv.stats()["bytes_in_use"] and time.time() < t_end:
+            time.sleep(0.01)
         st = srv.stats()
     assert st["done"] == len(REQUESTS)
+    assert st["bytes_in_use"] == 0 and st["bytes_peak"] > 0
     spans = sorted((r["t_admit_ms"], r["t_done_ms"]) for r in out)
     for (a0, d0), (a1, d1) in zip(spans, spans[1:]):
         assert a1 >= d0  # never two requests active at once
