@@ -110,11 +110,14 @@ def test_large_ion_group_kernels_fp64_match_reference(large, ft, lanes, wg):
 
 @pytest.mark.parametrize("ft", [4, 1], ids=["exact", "clustered"])
 def test_large_ion_mixed_vs_fp64_paired(large, ft):
-    """Mixed precision against FP64 on the same 10,000 initial conditions,
-    same launch shape (tiled kernel / 64-lane clustered kernel)."""
+    """Mixed precision against FP64 on the same 200,000 initial conditions,
+    same launch shape (tiled kernel / 64-lane clustered kernel).  On this
+    ion about one trajectory in a few hundred is chaotic enough for the two
+    precisions to part, so the paired difference has a standard error of
+    ~1.5e-3 at 10,000 trajectories and ~3.5e-4 at 200,000."""
     _, m = large
     shape = dict(lanes=64, wg_size=512) if ft == 1 else dict(lanes=16, wg_size=512)
-    kw = dict(job=11, iters=2, traj=5000, keep=True, **shape)
+    kw = dict(job=11, iters=20, traj=10000, keep=True, **shape)
     f64 = m[ft].run(E.PREC_FP64, **kw)
     mix = m[ft].run(E.PREC_MIXED, **kw)
     assert np.array_equal(mix["status"] != 0, f64["status"] != 0)
