@@ -1130,13 +1130,13 @@ __device__ __forceinline__ void group_loop(const TrajIO &A, Src &S)
     }
 }
 
-template <int PREC, int FT, int L>
-__global__ void __launch_bounds__(1024) k_traj_grp(GroupArgs A)
+// The molecule of A into LDS for LdsSource: FP32 x, y, z, q | A, B, or the
+// FP64 x, y, z, q, sigma^6, 24 eps/mu (npad each), by the whole workgroup.
+template <int PREC>
+__device__ __forceinline__ void stage_molecule(const GroupArgs &A, char *smem)
 {
-    extern __shared__ __attribute__((aligned(16))) char smem[];
     const int npad = A.npad;
     const double *S = A.soa;
-    // stage the molecule (once per workgroup; the kernel is persistent)
     if constexpr (PREC == 1) {
         float4 *xq = (float4 *)smem;
         float2 *se = (float2 *)(xq + npad);
@@ -1150,11 +1150,48 @@ __global__ void __launch_bounds__(1024) k_traj_grp(GroupArgs A)
         double *X = (double *)smem;
         for (int i = threadIdx.x; i < 6 * npad; i += blockDim.x) X[i] = S[i];
     }
+}
+
+template <int PREC, int FT, int L>
+__global__ void __launch_bounds__(1024) k_traj_grp(GroupArgs A)
+{
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    // stage the molecule (once per workgroup; the kernel is persistent)
+    stage_molecule<PREC>(A, smem);
     __syncthreads();
     TrajIO io = A.io;
     resolve_range(io, A.first, A.last);
     LdsSource<PREC, FT, L> src{A, smem};
     group_loop<L>(io, src);
+}
+
+// Rate probe of the force sum alone: the molecule staged in LDS as for
+// k_traj_grp, then every group of L lanes evaluates the force `evals` times
+// with the trajectory kernels' own force call (LdsSource::force: the same
+// per-pair arithmetic, group sums and epilogue), at points 20 % beyond the
+// boundary ellipsoid's a axis (the exact types' pair loop has no
+// data-dependent branch, so the point does not change the work).  No Verlet
+// state machine, no potential checks, and every group does the same work, so
+// the launch has no drain: its pair rate is the ceiling of the trajectory
+// kernel's instruction mix.  The sum of the forces goes to *sink only if it
+// equals a value it never takes (so the work is not optimised away).
+template <int PREC, int FT, int L>
+__global__ void __launch_bounds__(1024) k_force_rate(GroupArgs A, int evals, double *sink)
+{
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    stage_molecule<PREC>(A, smem);
+    __syncthreads();
+    LdsSource<PREC, FT, L> src{A, smem};
+    const int j = threadIdx.x & (L - 1);
+    const int grp = (blockIdx.x * blockDim.x + threadIdx.x) / L;
+    const double x0 = 1.2 * A.m.a, y0 = 0.05 * A.m.b * (double)(grp % 7 - 3), z0 = 0.05 * A.m.c * (double)(grp % 5 - 2);
+    double acc = 0;
+    for (int e = 0; e < evals; e++) {
+        double fx, fy, fz;
+        src.force(true, j, x0 + 1e-3 * e, y0, z0, fx, fy, fz);
+        acc += fx + fy + fz;
+    }
+    if (acc == -1.0e300) *sink = acc;
 }
 
 // Molecules beyond one workgroup's LDS, exact force types: the molecule

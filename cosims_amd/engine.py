@@ -26,7 +26,7 @@ EXPORTS = [
     "cosims_mol_atoms", "cosims_mol_clusters", "cosims_accel", "cosims_potential", "cosims_sample",
     "cosims_sample_reference",
     "cosims_verlet", "cosims_run", "cosims_run_batch", "cosims_tune_batch", "cosims_strerror",
-    "cosims_last_error", "cosims_device_info", "cosims_device_sync", "cosims_device_fma_rate",
+    "cosims_last_error", "cosims_device_info", "cosims_device_sync", "cosims_device_fma_rate", "cosims_force_rate",
     "cosims_server_create", "cosims_server_submit", "cosims_server_wait", "cosims_server_stats_get",
     "cosims_server_destroy",
 ]
@@ -191,6 +191,8 @@ def lib():
         L.cosims_last_error.restype = C.c_char_p
         L.cosims_device_info.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_int32)]
         L.cosims_device_fma_rate.argtypes = [C.POINTER(C.c_double)]
+        L.cosims_force_rate.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _LIB = L
     return _LIB
 
@@ -360,6 +362,13 @@ class Molecule:
         _check(lib().cosims_mol_clusters(self._h, *[_p(a) for a in (st, at, ce, md, pos, q, dip, quad, rep)]))
         return dict(start=st, atoms=at, center=ce, maxdist=md, pos=pos, q=q, dip=dip, quad=quad,
                     replace=rep)
+
+    def force_rate(self, prec=PREC_MIXED, lanes=16, wg_size=256):
+        """Measured rate of the force sum alone (the trajectory kernels'
+        ceiling): dict(tflops=reference TFLOP/s, pairs_per_s, kernel_ms)."""
+        t, p, ms = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().cosims_force_rate(self._h, prec, lanes, wg_size, C.byref(t), C.byref(p), C.byref(ms)))
+        return dict(tflops=t.value, pairs_per_s=p.value, kernel_ms=ms.value)
 
     def accel(self, pos, prec=PREC_FP64):
         pos = _f64(pos).reshape(-1, 3)

@@ -379,9 +379,22 @@ const char *cosims_last_error(void);
 /* Device name and compute-unit count of the current device. */
 int cosims_device_info(char *name, size_t cap, int32_t *cus);
 /* Measured FP32 FMA rate of the current device for code built as this
- * library is (no packed FP32), in TFLOP/s: the vector ceiling of the mixed
- * force sum. */
+ * library is (no packed FP32; all operands in VGPRs, which gfx950 issues at
+ * 2 cycles a wavefront), in TFLOP/s. */
 int cosims_device_fma_rate(double *tflops);
+/* Measured rate of the force sum alone on this molecule (exact force
+ * types 4 and 6, molecule within one workgroup's LDS): the trajectory
+ * kernels' force call (per-pair arithmetic, group sums over `lanes` lanes,
+ * epilogue) with the molecule staged in LDS, a full-occupancy grid of
+ * wg_size-thread workgroups, every group evaluating the same number of
+ * forces -- no Verlet bookkeeping, no energy checks, no launch drain.  The
+ * ceiling of the trajectory kernels' instruction mix, in reference TFLOP/s
+ * (58 FLOP a pair + 21 an evaluation for type 4, src/forceAccel.cpp:764-802;
+ * 22 a pair for type 6, :865-880), atom pairs per second and the launch's
+ * time (best of three).  No counterpart in the reference (a benchmark
+ * query); pairs_per_s and kernel_ms may be NULL. */
+int cosims_force_rate(const cosims_mol *m, int32_t prec, int32_t lanes, int32_t wg_size, double *tflops,
+                      double *pairs_per_s, double *kernel_ms);
 /* Waits for all work on the current device (benchmark brackets). */
 int cosims_device_sync(void);
 

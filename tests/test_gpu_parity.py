@@ -91,7 +91,24 @@ def test_device_sqrt_div_correctly_rounded():
 
 
 def test_device_fma_rate_is_plausible():
-    """The measured non-packed FP32 FMA ceiling lies between a quarter of and
-    the published packed peak (157.3 TFLOP/s on MI355X)."""
+    """The measured non-packed FP32 FMA rate with VGPR operands: gfx950
+    issues such an instruction in 2 cycles a wavefront, so the rate lies
+    above the 4-cycle figure (78.6 TFLOP/s at 2.4 GHz, less a throttled
+    clock) and at most the published 157.3 TFLOP/s."""
     t = E.device_fma_rate()
-    assert 157.3 / 4 < t <= 157.3, t
+    assert 100.0 < t <= 157.3, t
+
+
+def test_force_rate_probe():
+    """The force-sum ceiling probe on the 2,000-atom ion: mixed precision
+    between the 4-cycle FP32 figure's half and the published peak, FP64
+    below the FP64 vector peak; molecules it cannot stage are refused."""
+    from conftest import golden_engine_mol
+    m = golden_engine_mol(golden("ion2000"), (1, 0, 0, 1))
+    mix = m.force_rate(E.PREC_MIXED, 16, 512)
+    assert 40.0 < mix["tflops"] <= 157.3, mix
+    assert mix["pairs_per_s"] * 58 <= mix["tflops"] * 1e12 * 1.001
+    f64 = m.force_rate(E.PREC_FP64, 32, 512)
+    assert 2.0 < f64["tflops"] <= 78.6, f64
+    with pytest.raises(E.CosimsError, match="fits LDS only"):
+        golden_engine_mol(golden("ion2000"), (1, 1, 1, 1)).force_rate(E.PREC_MIXED, 16, 512)
