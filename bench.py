@@ -53,6 +53,22 @@ def kernel_flops(n_atoms, res, r=0):
     return res["round_evals"][r] * (n_atoms * FLOP_PAIR + FLOP_EPI) + res["round_steps"][r] * FLOP_VERLET
 
 
+# the FP64 oracle against the compiled reference on the same CPU and workload (128-atom ion,
+# 3 x 20,000 trajectories): oracle speed / reference speed at 1 and 8 threads, from
+# profiles/cpu_reference_vs_oracle_ion128.md (oracle/time_ref.py)
+ORACLE_OVER_REFERENCE = {"threads_1": 1.05, "threads_8": 1.21}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(m, budget_s, threads):
     """The FP64 oracle on the same workload's initial conditions, bounded."""
     import oracle
@@ -70,7 +86,10 @@ def cpu_baseline(m, budget_s, threads):
         done += int(s["hit"].size)  # sampled trajectories (misses cost nothing)
         batches += 1
     return dict(value=done / t_used, unit="trajectories/s", cores=threads, kind="port",
-                sample=f"{done} trajectories of the 2,000-atom ion (FP64 oracle, OpenMP, {t_used:.1f} s)")
+                sample=f"{done} trajectories of the 2,000-atom ion (FP64 oracle, OpenMP, {t_used:.1f} s)",
+                cpu=cpu_model(), nproc=os.cpu_count(),
+                port_speed_over_reference=ORACLE_OVER_REFERENCE,
+                port_speed_source="profiles/cpu_reference_vs_oracle_ion128.md")
 
 
 def pct(a, q):
@@ -359,6 +378,9 @@ def main():
     ap.add_argument("--oracle-sample", type=int, default=300)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 3 device-to-device copies of 256 MiB (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--ceiling-probe", action="store_true",
+                    help="only one mixed run of the headline workload at --lanes/--wg and the force-loop ceiling "
+                         "probe at the same shape (for rocprofv3 counter passes over both kernels)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); trajectories shard "
@@ -366,6 +388,16 @@ def main():
         sys.exit(2)
     if a.copy_probe:
         copy_probe()
+        return
+    if a.ceiling_probe:
+        from cosims_amd import engine as E
+        from cosims_amd import synth
+        m = make_molecule(E, synth, a.traj, a.iters, seed=20261203)
+        r = m.run(E.PREC_MIXED, job=0, lanes=a.lanes, wg_size=a.wg)
+        c = m.force_rate(E.PREC_MIXED, r["lanes"], r["wg_size"])
+        log(f"trajectory launch {r['traj_kernel_ms'][0]:.1f} ms ({kernel_flops(m.info()['n'], r) / r['traj_kernel_ms'][0] / 1e9:.1f} "
+            f"TFLOP/s), lanes {r['lanes']} wg {r['wg_size']}; force-loop ceiling {c['tflops']:.1f} TFLOP/s "
+            f"({c['kernel_ms']:.1f} ms a launch)")
         return
     if a.workload != "headline":
         from cosims_amd import engine as E
@@ -425,13 +457,13 @@ def main():
     flops = np.array([kernel_flops(inf["n"], r) for r in runs])
     achieved = float(np.mean(flops / (kms * 1e-3)) / 1e12)
     # the measured ceiling of the kernel's instruction mix: its force call alone (molecule in LDS,
-    # full occupancy, no Verlet bookkeeping, no drain) at the same launch shape, and the best
-    # workgroup size at the same lanes
-    ceil = m.force_rate(E.PREC_MIXED, mixed["lanes"], mixed["wg_size"])
-    ceil_best = max((m.force_rate(E.PREC_MIXED, mixed["lanes"], w)["tflops"], w) for w in (256, 512, 1024)
-                    if w >= mixed["lanes"])
-    log(f"force-loop ceiling: {ceil['tflops']:.1f} TFLOP/s at the run's shape, {ceil_best[0]:.1f} at wg "
-        f"{ceil_best[1]}; FMA probe {fma:.1f} TFLOP/s")
+    # full occupancy, no Verlet bookkeeping, no drain) at the run's lanes, the best of the three
+    # workgroup sizes (at the run's own workgroup size the probe varies by ~10 % from box to box)
+    ceil_w = {w: m.force_rate(E.PREC_MIXED, mixed["lanes"], w)["tflops"] for w in (256, 512, 1024)
+              if w >= mixed["lanes"]}
+    ceil_wg = max(ceil_w, key=ceil_w.get)
+    log(f"force-loop ceiling: {ceil_w[ceil_wg]:.1f} TFLOP/s (wg {ceil_wg}), by wg {ceil_w}; "
+        f"FMA probe {fma:.1f} TFLOP/s")
 
     out = {
         "metric": METRIC, "value": value, "unit": "trajectories/s", "n_gpus": 1, "steps": a.steps,
@@ -443,10 +475,11 @@ def main():
                    "precision": "mixed"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32,
-                     "ceiling_measured": ceil["tflops"], "frac_of_ceiling": achieved / ceil["tflops"],
-                     "ceiling_best_wg": {"tflops": ceil_best[0], "wg_size": ceil_best[1]},
-                     "ceiling_source": "cosims_force_rate: the kernel's force call alone, molecule in LDS, "
-                                       "full occupancy, no Verlet bookkeeping, no drain",
+                     "ceiling_measured": ceil_w[ceil_wg], "frac_of_ceiling": achieved / ceil_w[ceil_wg],
+                     "ceiling_by_wg": {str(w): t for w, t in ceil_w.items()},
+                     "ceiling_source": "cosims_force_rate at the run's lanes, best workgroup size: the kernel's "
+                                       "force call alone, molecule in LDS, full occupancy, no Verlet bookkeeping, "
+                                       "no drain",
                      "fma_probe_tflops": fma,
                      "traffic": TRAFFIC_BYTES, "traffic_source": "profiles/traj_grp_mixed_ion2000.md",
                      "kernel": f"k_traj_grp<mixed,type4,{mixed['lanes']}>",

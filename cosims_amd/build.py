@@ -2,7 +2,8 @@
 and the command-line program that drives it.
 
 Compiles in-tree: objects under cosims_amd/_build/, the library at
-cosims_amd/libcosims_engine.so, the program at cosims_amd/cosims.  Run
+cosims_amd/libcosims_engine.so, the program at cosims_amd/cosims, the VALU
+issue-rate probes at cosims_amd/issue_probe.  Run
 ``python -m cosims_amd.build``.
 """
 from __future__ import annotations
@@ -18,6 +19,7 @@ SRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcosims_engine.so")
 CLI = os.path.join(HERE, "cosims")
+PROBE = os.path.join(HERE, "issue_probe")
 ARCH = os.environ.get("COSIMS_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -62,6 +64,10 @@ def build(verbose: bool = False) -> str:
     tmp = CLI + ".tmp"
     _run([HIPCC, "-o", tmp, o, "-L" + HERE, "-lcosims_engine", "-Wl,-rpath,$ORIGIN"], verbose)
     os.replace(tmp, CLI)
+    # the VALU issue-rate probes behind the roofline figures (a measurement program)
+    tmp = PROBE + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, *inc, "-o", tmp, os.path.join(SRC, "issue_probe.hip")], verbose)
+    os.replace(tmp, PROBE)
     return LIB
 
 

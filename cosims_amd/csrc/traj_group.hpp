@@ -304,29 +304,43 @@ __host__ __device__ inline void lj_constants(double s6, double e24, float &A, fl
     B = (float)(e24 * s6);
 }
 
-// FP64 reference-form pair terms (src/forceAccel.cpp:764-797, 865-880) of
-// one atom (x, y, z, q, sigma^6, 24 eps/mu), added to this lane's partial
-// sums: LJ tx ty tz 0-2, field dX dY dZ 3-5, gradient sXX sYY sZZ sXY sXZ
-// sYZ 6-11.
+// FP64 pair terms of the group kernels (src/forceAccel.cpp:764-797,
+// 865-880) of one atom (x, y, z, q, sigma^6, 24 eps/mu), added to this
+// lane's partial sums: LJ tx ty tz 0-2, field dX dY dZ 3-5, gradient sXX sYY
+// sZZ sXY sXZ sYZ 6-11.  The reference's four IEEE divisions and square root
+// a pair (sigma^6 / r^6, / r^2, 1 / r^3, / r^2) become products with
+// ir = 1 / sqrt(r^2) and ir2 = ir^2, ir from v_rsq_f64 refined by two
+// Newton steps (within a few ulp of the quotients).  These kernels sum in
+// lane-split order and are not bit-identical to the reference anyway; the
+// lane kernel keeps the IEEE operations and the reference's order.
+__device__ __forceinline__ double rsqrt64(double r2)
+{
+    double y = __builtin_amdgcn_rsq(r2);
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        // y += y (1 - r2 y^2) / 2
+        const double e = __builtin_fma(-(r2 * y), y, 1.0);
+        y = __builtin_fma(0.5 * y, e, y);
+    }
+    return y;
+}
+
 template <int FT>
 __device__ __forceinline__ void pair64(double ax, double ay, double az, double aq, double as6, double ae24,
                                        double px, double py, double pz, double (&s)[12])
 {
-    double xD = px - ax, yD = py - ay, zD = pz - az;
-    double x2 = xD * xD, y2 = yD * yD, z2 = zD * zD;
-    double r2 = x2 + y2 + z2;
-    double r6 = r2 * r2 * r2;
-    double t2 = as6 / r6;
-    double t1 = 2 * t2 * t2;
-    double sc = ae24 * (t1 - t2) / r2;
+    const double xD = px - ax, yD = py - ay, zD = pz - az;
+    const double x2 = xD * xD, y2 = yD * yD, z2 = zD * zD;
+    const double r2 = x2 + y2 + z2;
+    const double ir = rsqrt64(r2);
+    const double ir2 = ir * ir;
+    const double t2 = as6 * (ir2 * ir2 * ir2);
+    const double t1 = 2 * t2 * t2;
+    const double sc = ae24 * (t1 - t2) * ir2;
     s[0] += sc * xD; s[1] += sc * yD; s[2] += sc * zD;
     if constexpr (FT == 4) {
-        double r = __builtin_sqrt(r2);
-        double r3 = r2 * r;
-        double r_3 = 1.0 / r3;
-        double r_5 = r_3 / r2;
-        r_3 *= aq;
-        r_5 *= aq;
+        const double r_3 = aq * (ir * ir2);
+        const double r_5 = r_3 * ir2;
         s[3] += xD * r_3; s[4] += yD * r_3; s[5] += zD * r_3;
         s[6] += (3.0 * x2 - r2) * r_5;
         s[7] += (3.0 * y2 - r2) * r_5;
@@ -354,52 +368,19 @@ __device__ __forceinline__ void force64_epilogue(double (&s)[12], double ds, dou
 }
 
 // FP64 pair terms over atoms j, j+L, ... < npad of the SoA X (x y z q s6 e24
-// with stride npad), then the group sums and the epilogue.  The expressions
-// of pair64 / force64_epilogue written out (the translation unit does not
-// contract, so the two forms give the same bits; this one keeps the FP64
-// kernels within 128 VGPRs without spills).
+// with stride npad), then the group sums and the epilogue: pair64 and
+// force64_epilogue, the same operations as the tiled kernel's, so the LDS,
+// table and tiled kernels give the same bits.
 template <int FT, int L, typename PD>
 __device__ __forceinline__ void grp_force64(PD X, int npad, int j, double px, double py, double pz, double ds,
                                             double &ox, double &oy, double &oz)
 {
     const PD Y = X + npad, Z = X + 2 * npad, Q = X + 3 * npad, S6 = X + 4 * npad, E24 = X + 5 * npad;
-    double sXX = 0, sYY = 0, sZZ = 0, sXY = 0, sXZ = 0, sYZ = 0, dX = 0, dY = 0, dZ = 0, tx = 0, ty = 0, tz = 0;
-#pragma unroll 2
-    for (int k = j; k < npad; k += L) {
-        double xD = px - X[k], yD = py - Y[k], zD = pz - Z[k];
-        double x2 = xD * xD, y2 = yD * yD, z2 = zD * zD;
-        double r2 = x2 + y2 + z2;
-        double r6 = r2 * r2 * r2;
-        double t2 = S6[k] / r6;
-        double t1 = 2 * t2 * t2;
-        double sc = E24[k] * (t1 - t2) / r2;
-        tx += sc * xD; ty += sc * yD; tz += sc * zD;
-        if constexpr (FT == 4) {
-            double r = __builtin_sqrt(r2);
-            double r3 = r2 * r;
-            double r_3 = 1.0 / r3;
-            double r_5 = r_3 / r2;
-            r_3 *= Q[k];
-            r_5 *= Q[k];
-            dX += xD * r_3; dY += yD * r_3; dZ += zD * r_3;
-            sXX += (3.0 * x2 - r2) * r_5;
-            sYY += (3.0 * y2 - r2) * r_5;
-            sZZ += (3.0 * z2 - r2) * r_5;
-            sXY += 3 * xD * yD * r_5;
-            sXZ += 3 * xD * zD * r_5;
-            sYZ += 3 * yD * zD * r_5;
-        }
-    }
-    tx = gsum<L>(tx); ty = gsum<L>(ty); tz = gsum<L>(tz);
-    if constexpr (FT == 4) {
-        dX = gsum<L>(dX); dY = gsum<L>(dY); dZ = gsum<L>(dZ);
-        sXX = gsum<L>(sXX); sYY = gsum<L>(sYY); sZZ = gsum<L>(sZZ);
-        sXY = gsum<L>(sXY); sXZ = gsum<L>(sXZ); sYZ = gsum<L>(sYZ);
-        tx -= ds * (sXX * dX + sXY * dY + sXZ * dZ);
-        ty -= ds * (sXY * dX + sYY * dY + sYZ * dZ);
-        tz -= ds * (sXZ * dX + sYZ * dY + sZZ * dZ);
-    }
-    ox = tx; oy = ty; oz = tz;
+    double s[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) s[k] = 0.0;
+    for (int k = j; k < npad; k += L) pair64<FT>(X[k], Y[k], Z[k], Q[k], S6[k], E24[k], px, py, pz, s);
+    force64_epilogue<FT, L>(s, ds, ox, oy, oz);
 }
 
 // Clustered force types with the clusters split over the group's lanes:
@@ -923,7 +904,6 @@ struct TileSource {
                 }
                 __syncthreads();
                 if (wave) {
-#pragma unroll 2
                     for (int k = j; k < cnt; k += L)
                         pair64<FT>(X[k], X[T + k], X[2 * T + k], X[3 * T + k], X[4 * T + k], X[5 * T + k], ex, ey,
                                    ez, s);

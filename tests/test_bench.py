@@ -41,3 +41,11 @@ def test_traffic_constant_matches_its_profile():
     assert fetch and write
     kib = 2 * int(fetch.group(1).replace(",", "")) + int(write.group(1).replace(",", ""))
     assert kib * 1024 == bench.TRAFFIC_BYTES
+
+
+def test_cpu_ratio_constant_matches_its_profile():
+    with open(os.path.join(ROOT, "profiles", "cpu_reference_vs_oracle_ion128.md")) as f:
+        text = f.read()
+    rows = dict(re.findall(r"^\| (\d+) \|.*\| ([0-9.]+) \|$", text, re.M))
+    assert float(rows["1"]) == bench.ORACLE_OVER_REFERENCE["threads_1"]
+    assert float(rows["8"]) == bench.ORACLE_OVER_REFERENCE["threads_8"]
