@@ -188,8 +188,10 @@ def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, fp64_max_at
 
 def batch_workload(E, synth, count, traj, steps, fp64_every=16):
     """256 synthetic conformers (N log-uniform 50-5,000), 50,000 trajectories
-    each, all in shared launches (one per launch shape); per-size-class
-    launch shapes from the batch tuner, tuned before the timing."""
+    each in one call: the default mode (one persistent launch per launch
+    shape, each workgroup staging the molecule of the conformer it works on
+    in LDS) and own launches (a launch per conformer); per-size-class launch
+    shapes from the batch tuner for each mode, tuned before the timing."""
     import tempfile
     d = tempfile.mkdtemp(prefix="cosims_batch_")
     mols = []
@@ -208,11 +210,11 @@ def batch_workload(E, synth, count, traj, steps, fp64_every=16):
     out = {}
     # one persistent launch per launch shape for all conformers (molecules through L2), its class
     # shapes tuned for that mode in an untimed call
-    _, st_sh = E.run_batch(jobs, E.PREC_MIXED, lanes=-1, shared=True)
-    SL, SW = st_sh["class_lanes"], st_sh["class_wg"]
-    log(f"batch: shared-launch tuned lanes {SL} wg {SW}")
+    _, st_own = E.run_batch(jobs, E.PREC_MIXED, lanes=-1, own_launches=True)
+    OL, OW = st_own["class_lanes"], st_own["class_wg"]
+    log(f"batch: own-launch tuned lanes {OL} wg {OW}")
     for label, kw in (("tuned", dict(lanes=-1, class_lanes=CL, class_wg=CW)), ("size_rule", dict(lanes=0)),
-                      ("shared_launch_tuned", dict(lanes=-1, class_lanes=SL, class_wg=SW, shared=True))):
+                      ("own_launches_tuned", dict(lanes=-1, class_lanes=OL, class_wg=OW, own_launches=True))):
         times = []
         for k in range(steps):
             E.device_sync()
@@ -230,8 +232,9 @@ def batch_workload(E, synth, count, traj, steps, fp64_every=16):
                       "class_lanes": st["class_lanes"], "class_wg": st["class_wg"]}
     out["workload"] = (f"{count} synthetic conformers, N log-uniform 50-5,000 (total {sum(n_atoms)} atoms), "
                        f"q round(N/250)+1 e, He 298 K, {traj} trajectories each, mixed precision")
-    out["modes"] = ("tuned / size_rule: a launch of its own per conformer (molecule staged in LDS); "
-                    "shared_launch_tuned: all conformers of a launch shape in one persistent launch")
+    out["modes"] = ("tuned / size_rule (default mode): all conformers of a launch shape in one persistent launch, "
+                    "each workgroup staging the molecule of the conformer it works on in LDS; "
+                    "own_launches_tuned: a launch of its own per conformer")
     out["trajectories"] = count * traj
     # CCS error against FP64 on paired initial conditions (same jobs and streams), on every
     # fp64_every-th conformer, against the size-rule mixed results
@@ -371,7 +374,7 @@ def main():
                     help="also run the stream with equal-share slices (1) or not (0)")
     ap.add_argument("--workload", choices=("headline", "batch", "clustered"), default="headline",
                     help="headline: the 2,000-atom ion + serving stream (the JSON contract line); batch: 256 uneven "
-                         "conformers in shared launches; clustered: the 20,000-atom ion, clustered vs exact")
+                         "conformers in one call; clustered: the 20,000-atom ion, clustered vs exact")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-traj", type=int, default=50000)
     ap.add_argument("--c4-traj", type=int, default=500000)

@@ -1321,22 +1321,32 @@ __global__ void k_copy_i32(const int32_t *src, int32_t *dst)
 // Counting sort of a work list by order bucket, longest first:
 // items[b .. e) -> out[b .. e), b = *first (0 when first is NULL), e = *last
 // (n when last is NULL); bin is indexed by item.  hist: kOrderBins counts,
-// then kOrderBins cursors, zero on entry.
+// then kOrderBins cursors, zero on entry.  With ends set, workgroup row y
+// sorts work-list position p = p0 + y on its own: b = ends[p - 1] (0 for
+// p = 0), e = ends[p], its own 2 kOrderBins of hist.
 constexpr int kOrderChunk = 2048;  // items a workgroup of the scatter pass places
 
-__device__ __forceinline__ void order_range(const int32_t *first, const int32_t *last, int64_t n, int64_t &b,
-                                            int64_t &e)
+__device__ __forceinline__ void order_range(const int32_t *first, const int32_t *last, int64_t n, const int32_t *ends,
+                                            int p0, int64_t &b, int64_t &e, int32_t *&hist)
 {
+    if (ends) {
+        const int p = p0 + (int)blockIdx.y;
+        b = p ? ends[p - 1] : 0;
+        e = ends[p];
+        hist += (size_t)2 * kOrderBins * blockIdx.y;
+        return;
+    }
     b = first ? *first : 0;
     e = last ? *last : n;
 }
 
 __global__ void __launch_bounds__(256) k_order_hist(const int32_t *items, const int32_t *first, const int32_t *last,
-                                                    int64_t n, const uint8_t *bin, int32_t *hist)
+                                                    int64_t n, const uint8_t *bin, int32_t *hist,
+                                                    const int32_t *ends, int p0)
 {
     __shared__ int32_t h[kOrderBins];
     int64_t b, e;
-    order_range(first, last, n, b, e);
+    order_range(first, last, n, ends, p0, b, e, hist);
     for (int i = threadIdx.x; i < kOrderBins; i += blockDim.x) h[i] = 0;
     __syncthreads();
     for (int64_t i = b + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += (int64_t)gridDim.x * blockDim.x)
@@ -1348,11 +1358,11 @@ __global__ void __launch_bounds__(256) k_order_hist(const int32_t *items, const 
 
 __global__ void __launch_bounds__(256) k_order_scatter(const int32_t *items, int32_t *out, const int32_t *first,
                                                        const int32_t *last, int64_t n, const uint8_t *bin,
-                                                       int32_t *hist)
+                                                       int32_t *hist, const int32_t *ends, int p0)
 {
     __shared__ int32_t start[kOrderBins], cnt[kOrderBins], base[kOrderBins];
     int64_t b, e;
-    order_range(first, last, n, b, e);
+    order_range(first, last, n, ends, p0, b, e, hist);
     const int64_t c0 = b + (int64_t)blockIdx.x * kOrderChunk;
     if (c0 >= e) return;  // uniform over the workgroup
     const int64_t c1 = min(c0 + (int64_t)kOrderChunk, e);
@@ -1395,9 +1405,28 @@ int order_items(const int32_t *items, int32_t *out, const int32_t *first, const 
     if (clear)
         hipLaunchKernelGGL(k_fill_i32, dim3(2 * kOrderBins / 64), dim3(64), 0, s, hist, (int64_t)2 * kOrderBins, 0);
     const unsigned nh = (unsigned)std::min<int64_t>(1024, (n + 4095) / 4096);
-    hipLaunchKernelGGL(k_order_hist, dim3(nh), dim3(256), 0, s, items, first, last, n, bin, hist);
+    hipLaunchKernelGGL(k_order_hist, dim3(nh), dim3(256), 0, s, items, first, last, n, bin, hist,
+                       (const int32_t *)nullptr, 0);
     const unsigned nc = (unsigned)((n + kOrderChunk - 1) / kOrderChunk);
-    hipLaunchKernelGGL(k_order_scatter, dim3(nc), dim3(256), 0, s, items, out, first, last, n, bin, hist);
+    hipLaunchKernelGGL(k_order_scatter, dim3(nc), dim3(256), 0, s, items, out, first, last, n, bin, hist,
+                       (const int32_t *)nullptr, 0);
+    HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// The same for work-list positions p0 .. p0 + npos - 1 at once, each range
+// [ends[p - 1], ends[p]) sorted on its own (at most max_n items each); hist:
+// 2 kOrderBins int32 per position, zero on entry.
+int order_positions(const int32_t *items, int32_t *out, const int32_t *ends, int p0, int npos, int64_t max_n,
+                    const uint8_t *bin, int32_t *hist, hipStream_t s)
+{
+    if (npos <= 0 || max_n <= 0) return 0;
+    const unsigned nh = (unsigned)std::min<int64_t>(64, (max_n + 4095) / 4096);
+    hipLaunchKernelGGL(k_order_hist, dim3(nh, (unsigned)npos), dim3(256), 0, s, items, (const int32_t *)nullptr,
+                       (const int32_t *)nullptr, (int64_t)0, bin, hist, ends, p0);
+    const unsigned nc = (unsigned)((max_n + kOrderChunk - 1) / kOrderChunk);
+    hipLaunchKernelGGL(k_order_scatter, dim3(nc, (unsigned)npos), dim3(256), 0, s, items, out,
+                       (const int32_t *)nullptr, (const int32_t *)nullptr, (int64_t)0, bin, hist, ends, p0);
     HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -1514,6 +1543,28 @@ GrpFn group_kernel(int prec, int ft, int L)
 {
     if (ft == 4) return prec == COSIMS_PREC_FP64 ? grp_fn_l<0, 4>(L) : grp_fn_l<1, 4>(L);
     if (ft == 6) return prec == COSIMS_PREC_FP64 ? grp_fn_l<0, 6>(L) : grp_fn_l<1, 6>(L);
+    return nullptr;
+}
+
+using LdsMultiFn = void (*)(LdsMultiArgs);
+
+template <int P, int FT>
+LdsMultiFn ldsm_fn_l(int L)
+{
+    switch (L) {
+    case 4: return k_traj_lds_multi<P, FT, 4>;
+    case 8: return k_traj_lds_multi<P, FT, 8>;
+    case 16: return k_traj_lds_multi<P, FT, 16>;
+    case 32: return k_traj_lds_multi<P, FT, 32>;
+    case 64: return k_traj_lds_multi<P, FT, 64>;
+    default: return nullptr;
+    }
+}
+
+LdsMultiFn lds_multi_kernel(int prec, int ft, int L)
+{
+    if (ft == 4) return prec == COSIMS_PREC_FP64 ? ldsm_fn_l<0, 4>(L) : ldsm_fn_l<1, 4>(L);
+    if (ft == 6) return prec == COSIMS_PREC_FP64 ? ldsm_fn_l<0, 6>(L) : ldsm_fn_l<1, 6>(L);
     return nullptr;
 }
 
@@ -2021,9 +2072,15 @@ struct BatchPlan {
     int64_t n = 0;
     int cus = 0;
     int nlaunch = 0;                     // launches of the last launch_enqueue
-    bool shared = false;                 // every job of a launch shape in one table-kernel launch
+    bool shared = false;                 // every job of a launch shape in one persistent launch
     TrajBufs B;
     DBuf tmol, table, seg_ctr, seg_evals, job_end, order_scr;
+    // shared launches with the molecule in LDS: job of each work-list position, per-position
+    // refill counters and sort scratch, the first position of each workgroup (host copy kept
+    // for the asynchronous upload)
+    DBuf pos_job, pos_ctr, pos_scr, wg_pos;
+    int64_t pos_cap = 0, wg_cap = 0;
+    std::vector<int32_t> wg_pos_host;
     int64_t cap = 0, table_cap = 0, job_end_cap = 0;  // allocated trajectories / table entries / bounds (reused)
     int order_cap = 0;                   // launches order_scr holds work-list sort scratch for
     float sample_ms = 0;
@@ -2135,6 +2192,14 @@ int stage_batch(BatchPlan &P, hipStream_t s)
     P.recs.resize(nj);
     for (int k = 0; k < nj; k++) P.recs[k] = P.jobs[k].mol->rec;
     HIP_CHECK(hipMemcpyAsync(P.table.p, P.recs.data(), sizeof(MolRec) * nj, hipMemcpyHostToDevice, s));
+    if (P.pos_cap < nj) {
+        HIP_CHECK(P.pos_job.alloc(sizeof(int32_t) * nj));
+        HIP_CHECK(P.pos_ctr.alloc(sizeof(int32_t) * nj));
+        HIP_CHECK(P.pos_scr.alloc(sizeof(int32_t) * 2 * kOrderBins * nj));
+        P.pos_cap = nj;
+    }
+    // job of each work-list position (P.order lives on the plan: the upload is asynchronous)
+    HIP_CHECK(hipMemcpyAsync(P.pos_job.p, P.order.data(), sizeof(int32_t) * nj, hipMemcpyHostToDevice, s));
     int32_t *ctr = P.B.counters.as<int32_t>();
     if (!P.es0) {
         HIP_CHECK(hipEventCreate(&P.es0));
@@ -2176,54 +2241,77 @@ int stage_batch(BatchPlan &P, hipStream_t s)
     return 0;
 }
 
-// The launches of a staged plan.  A job of an exact force type runs one
-// launch of its own: the LDS-staged kernel when its molecule fits one
-// workgroup's LDS, the tiled kernel beyond (either way the atoms come from
-// LDS instead of through L2: 1.3-1.6x the pair rate of the table kernel); the
-// clustered jobs of a launch shape, tiny jobs and jobs beyond
-// kMaxOwnLaunches, in consecutive runs of the work list, share table-kernel
-// launches.  All three kernels give the same bits for a trajectory.  Launches are spread over kSegStreams streams,
-// the most work first, so one launch's tail overlaps the next ones.
-// launch_enqueue only enqueues (the stream s carries the join; kernels run
-// on the plan's own streams); launch_finish waits for them and adds their
-// time (first launch to last one's end) and counters to tot.
+// The launches of a staged plan.
+//   Own launches (the default, P.shared false): a job of an exact force type
+//   runs one launch of its own, the LDS-staged kernel when its molecule fits
+//   one workgroup's LDS, the tiled kernel beyond; the clustered jobs of a
+//   launch shape, tiny jobs and jobs beyond kMaxOwnLaunches, in consecutive
+//   runs of the work list, share table-kernel launches.
+//   Shared launches (P.shared): the jobs of a launch shape whose molecule
+//   fits LDS share one persistent k_traj_lds_multi launch (each workgroup
+//   stages the molecule of the job it works on, and moves to the job with the
+//   most trajectories left when its job's range is spent); larger exact-type
+//   jobs keep tiled launches of their own; clustered jobs share table-kernel
+//   launches.
+// All kernels give the same bits for a trajectory.  Launches are spread over
+// kSegStreams streams, the most work first, so one launch's tail overlaps the
+// next ones.  launch_enqueue only enqueues (the stream s carries the join;
+// kernels run on the plan's own streams); launch_finish waits for them and
+// adds their time (first launch to last one's end) and counters to tot.
 int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches)
 {
     launches = 0;
+    enum Kind { OWN, TABLE, LDS_MULTI };
     struct Launch {
         int seg, i0, i1;  // work-list positions order[i0 .. i1)
-        bool own;         // one job on the LDS-staged or tiled kernel (else a table-kernel run)
+        Kind kind;
         int64_t traj;
         double work;
+        int64_t max_n;    // most trajectories of one of its jobs
     };
     std::vector<Launch> plan;
     int own = 0;
     for (int g = 0; g < (int)P.segs.size(); g++) {
         const auto &sg = P.segs[g];
         int run0 = -1;
-        int64_t run_traj = 0;
+        Kind run_kind = TABLE;
+        int64_t run_traj = 0, run_max = 0;
         double run_work = 0;
         for (int i = sg.i0; i <= sg.i1; i++) {
             const BatchPlan::Job *J = i < sg.i1 ? &P.jobs[P.order[i]] : nullptr;
-            // a launch of its own: the LDS-staged kernel, or the tiled one beyond LDS
-            // (tiny jobs, and jobs past the per-batch count, share table launches)
-            const bool mine = J && !P.shared && (lds_mode(J->mol, prec) || tile_mode(J->mol, prec)) &&
-                              group_kernel(prec, J->ft, J->L) && (double)J->n * J->units >= kOwnLaunchWork &&
-                              own < kMaxOwnLaunches;
-            own += mine;
-            if ((!J || mine) && run0 >= 0) {  // close the run of table-kernel jobs
-                plan.push_back({g, run0, i, false, run_traj, run_work});
+            Kind k = TABLE;
+            bool mine = false;
+            if (J) {
+                const bool exact = group_kernel(prec, J->ft, J->L) != nullptr;
+                if (P.shared) {
+                    if (exact && lds_mode(J->mol, prec)) k = LDS_MULTI;
+                    else if (exact && tile_mode(J->mol, prec)) mine = true;
+                } else {
+                    // a launch of its own: the LDS-staged kernel, or the tiled one beyond LDS
+                    // (tiny jobs, and jobs past the per-batch count, share table launches)
+                    mine = (lds_mode(J->mol, prec) || tile_mode(J->mol, prec)) && exact &&
+                           (double)J->n * J->units >= kOwnLaunchWork && own < kMaxOwnLaunches;
+                }
+                own += mine;
+            }
+            if (run0 >= 0 && (!J || mine || k != run_kind)) {  // close the run of shared jobs
+                plan.push_back({g, run0, i, run_kind, run_traj, run_work, run_max});
                 run0 = -1;
                 run_traj = 0;
                 run_work = 0;
+                run_max = 0;
             }
             if (!J) break;
             if (mine) {
-                plan.push_back({g, i, i + 1, true, J->n, (double)J->n * J->units});
+                plan.push_back({g, i, i + 1, OWN, J->n, (double)J->n * J->units, J->n});
             } else {
-                if (run0 < 0) run0 = i;
+                if (run0 < 0) {
+                    run0 = i;
+                    run_kind = k;
+                }
                 run_traj += J->n;
                 run_work += (double)J->n * J->units;
+                run_max = std::max(run_max, J->n);
             }
         }
     }
@@ -2258,21 +2346,88 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
     for (int i = 0; i < nl; i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return plan[a].work > plan[b].work; });
     const int32_t *ends = P.job_end.as<int32_t>();
+    const int nj = (int)P.jobs.size();
+    bool any_multi = false;
+    for (const Launch &l : plan) any_multi |= l.kind == LDS_MULTI;
+    if (any_multi) {
+        const int64_t nb = (int64_t)nj + (int64_t)2 * kOrderBins * nj;
+        hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0, s, P.pos_ctr.as<int32_t>(),
+                           (int64_t)nj, 0);
+        hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nb - nj + 255) / 256)), dim3(256), 0, s,
+                           P.pos_scr.as<int32_t>(), nb - nj, 0);
+    }
     // every launch's range of the work list, longest expected trajectories
     // first, into items[1], before the launches start: the small sort
-    // kernels would otherwise wait for CU slots behind running launches
+    // kernels would otherwise wait for CU slots behind running launches.
+    // Shared LDS launches sort each job's range on its own (a workgroup
+    // drains one job at a time).
     if (kOrderWork && nl > 0) {
         const int64_t ns = (int64_t)2 * kOrderBins * nl;
         hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s, P.order_scr.as<int32_t>(),
                            ns, 0);
         for (int li = 0; li < nl; li++) {
             const Launch &l = plan[li];
-            if (int rc = order_items(P.B.items[0].as<int32_t>(), P.B.items[1].as<int32_t>(),
-                                     l.i0 ? ends + l.i0 - 1 : nullptr, ends + l.i1 - 1, l.traj,
-                                     P.B.bin.as<uint8_t>(), P.order_scr.as<int32_t>() + 2 * kOrderBins * li, s,
-                                     false))
-                return rc;
+            int rc = 0;
+            if (l.kind == LDS_MULTI)
+                rc = order_positions(P.B.items[0].as<int32_t>(), P.B.items[1].as<int32_t>(), ends, l.i0, l.i1 - l.i0,
+                                     l.max_n, P.B.bin.as<uint8_t>(), P.pos_scr.as<int32_t>() + 2 * kOrderBins * l.i0,
+                                     s);
+            else
+                rc = order_items(P.B.items[0].as<int32_t>(), P.B.items[1].as<int32_t>(),
+                                 l.i0 ? ends + l.i0 - 1 : nullptr, ends + l.i1 - 1, l.traj, P.B.bin.as<uint8_t>(),
+                                 P.order_scr.as<int32_t>() + 2 * kOrderBins * li, s, false);
+            if (rc) return rc;
         }
+    }
+    // shared LDS launches: grids and each workgroup's first job, in proportion to the jobs'
+    // work (trajectories x atoms), so that they end about together and few workgroups move
+    struct MultiShape {
+        LdsMultiFn fn = nullptr;
+        size_t lds = 0;
+        int64_t grid = 0, wg0 = 0;
+    };
+    std::vector<MultiShape> ms(nl);
+    int64_t wg_total = 0;
+    for (int li = 0; li < nl; li++) {
+        const Launch &l = plan[li];
+        if (l.kind != LDS_MULTI) continue;
+        const auto &g = P.segs[l.seg];
+        MultiShape &m = ms[li];
+        m.fn = lds_multi_kernel(prec, g.ft, g.L);
+        if (!m.fn) return fail(COSIMS_E_INVALID, "no shared LDS kernel for this force type / lane count");
+        int npad = 0;
+        for (int i = l.i0; i < l.i1; i++) npad = std::max(npad, P.jobs[P.order[i]].mol->npad);
+        m.lds = kCtlBytes + (size_t)npad * (prec == COSIMS_PREC_FP64 ? 6 * sizeof(double)
+                                                                      : sizeof(float4) + sizeof(float2));
+        HIP_CHECK(hipFuncSetAttribute((const void *)m.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)m.lds));
+        m.grid = group_grid((const void *)m.fn, g.W, m.lds, bpc, P.cus, l.traj, g.L);
+        m.wg0 = wg_total;
+        wg_total += m.grid;
+    }
+    if (wg_total > 0) {
+        if (P.wg_cap < wg_total) {
+            HIP_CHECK(P.wg_pos.alloc(sizeof(int32_t) * wg_total));
+            P.wg_cap = wg_total;
+        }
+        P.wg_pos_host.assign((size_t)wg_total, 0);
+        for (int li = 0; li < nl; li++) {
+            const Launch &l = plan[li];
+            if (l.kind != LDS_MULTI) continue;
+            const MultiShape &m = ms[li];
+            int p = l.i0;
+            double cum = 0, cum_p = (double)P.jobs[P.order[p]].n * P.jobs[P.order[p]].units;
+            for (int64_t w = 0; w < m.grid; w++) {
+                const double target = ((double)w + 0.5) / (double)m.grid * l.work;
+                while (p + 1 < l.i1 && cum_p <= target) {
+                    cum = cum_p;
+                    p++;
+                    cum_p = cum + (double)P.jobs[P.order[p]].n * P.jobs[P.order[p]].units;
+                }
+                P.wg_pos_host[(size_t)(m.wg0 + w)] = p;
+            }
+        }
+        HIP_CHECK(hipMemcpyAsync(P.wg_pos.p, P.wg_pos_host.data(), sizeof(int32_t) * wg_total, hipMemcpyHostToDevice,
+                                 s));
     }
     HIP_CHECK(hipEventRecord(P.ev0, s));
     for (int q = 0; q < nl; q++) {
@@ -2288,7 +2443,7 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
         TrajIO io = traj_io(P.B, (kOrderWork ? P.B.items[1] : P.B.items[0]).as<int32_t>(), cnt);
         io.counter = P.seg_ctr.as<int32_t>() + li;
         io.evals = P.seg_evals.as<unsigned long long>() + 4 * li;
-        if (l.own) {
+        if (l.kind == OWN) {
             const cosims_mol *M = P.jobs[P.order[l.i0]].mol;
             const bool tiled = tile_mode(M, prec);
             GrpFn fn = tiled ? tile_kernel(prec, g.ft, g.L) : group_kernel(prec, g.ft, g.L);
@@ -2304,6 +2459,18 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
             G.last = last;
             const int64_t grid = group_grid((const void *)fn, g.W, lds, bpc, P.cus, cnt, g.L);
             hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(g.W), lds, x, G);
+        } else if (l.kind == LDS_MULTI) {
+            const MultiShape &m = ms[li];
+            LdsMultiArgs G{};
+            G.io = io;  // items: the whole sorted work list; each job's range from ends
+            G.mols = P.table.as<MolRec>();
+            G.pos_job = P.pos_job.as<int32_t>();
+            G.ends = ends;
+            G.pos_ctr = P.pos_ctr.as<int32_t>();
+            G.wg_pos = P.wg_pos.as<int32_t>() + m.wg0;
+            G.pos0 = l.i0;
+            G.npos = l.i1 - l.i0;
+            hipLaunchKernelGGL(m.fn, dim3((unsigned)m.grid), dim3(g.W), m.lds, x, G);
         } else {
             MultiFn fn = multi_kernel(prec, g.ft, g.L);
             if (!fn) return fail(COSIMS_E_INVALID, "no table kernel for this force type / lane count");
@@ -2362,7 +2529,7 @@ int launch_batch(BatchPlan &P, int prec, int bpc, hipStream_t s, GroupRun &tot, 
 
 // Size-class autotuner of a batch: for every size class present, the
 // lanes x workgroup search of tune_shapes on that class's jobs alone.
-int tune_batch(const cosims_batch_job *jobs, int32_t njobs, int prec, ShapeTable &out, bool shared = false)
+int tune_batch(const cosims_batch_job *jobs, int32_t njobs, int prec, ShapeTable &out, bool shared = true)
 {
     if (!jobs || njobs <= 0) return fail(COSIMS_E_INVALID, "cosims_tune_batch: no jobs");
     Stream st;
@@ -2736,11 +2903,11 @@ int cosims_run_batch(cosims_batch_job *jobs, int32_t njobs, const cosims_batch_c
         bool given = false;
         for (int c = 0; c < COSIMS_SIZE_CLASSES; c++) given |= cfg->class_lanes[c] != 0;
         if (!given)
-            if (int rc = tune_batch(jobs, njobs, prec, shape, cfg->shared != 0)) return rc;
+            if (int rc = tune_batch(jobs, njobs, prec, shape, cfg->own_launches == 0)) return rc;
     }
     BatchPlan P;
     if (int rc = plan_batch(jobs, njobs, 0, shape, P)) return rc;
-    P.shared = cfg->shared != 0;
+    P.shared = cfg->own_launches == 0;
     Stream st;
     HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
     if (int rc = stage_batch(P, st.s)) return rc;

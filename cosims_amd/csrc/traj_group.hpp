@@ -760,6 +760,7 @@ __device__ __forceinline__ double grp_potential(PD X, int npad, bool charged, do
 template <int PREC, int FT, int L>
 struct LdsSource {
     static constexpr bool kCoop = false;
+    static constexpr bool kMulti = false;
     const GroupArgs &A;
     const char *smem;
     __device__ void bind(int) {}
@@ -793,6 +794,7 @@ struct LdsSource {
 template <int PREC, int FT, int L>
 struct TableSource {
     static constexpr bool kCoop = false;
+    static constexpr bool kMulti = false;
     const MultiArgs &A;
     int *lists;  // this wavefront's cluster lists in LDS (clustered types, L = 64)
     int mi = 0;
@@ -852,6 +854,7 @@ struct TableSource {
 template <int PREC, int FT, int L>
 struct TileSource {
     static constexpr bool kCoop = true;
+    static constexpr bool kMulti = false;
     const GroupArgs &A;
     char *smem;
     __device__ void bind(int) {}
@@ -918,6 +921,208 @@ struct TileSource {
     }
 };
 
+// Molecule source of k_traj_lds_multi: the jobs of one launch shape, each
+// workgroup working on one job at a time with that job's molecule staged in
+// its LDS (the layout of LdsSource).  A job is a position of the batch's work
+// list with a range of trajectories of its own, ordered longest expected
+// first; the workgroup's groups refill from the range of the staged job
+// only.  When that range is exhausted and no wavefront of the workgroup still
+// holds one of its trajectories, one wavefront (the stager) picks the job
+// with the most trajectories left, stages its molecule and publishes it; the
+// others wait (s_sleep) and then refill from the new range.  Lane j of a
+// group sums atoms j, j+L, ... with the force call of LdsSource, so every
+// trajectory gives the bits it gives in a launch of its own.
+//
+// Why no wavefront can read the LDS while it is restaged: a wavefront
+// evaluates forces only while one of its groups holds a trajectory, a group
+// takes a trajectory only from the range of the job its wavefront last saw
+// published, and the stager stages only after its own refill found that range
+// exhausted (ranges only shrink) and with `active`, the wavefronts counted in
+// before their last refill and not yet counted out, at zero.
+struct WgCtl {
+    int pos;     // position whose molecule is in LDS; -1: nothing left
+    int gen;     // publications so far
+    int active;  // wavefronts that may hold a trajectory of the staged job
+    int stager;  // 1 while a wavefront restages
+};
+constexpr int kCtlBytes = 64;  // LDS before the molecule
+
+struct LdsMultiArgs {
+    TrajIO io;               // io.items: the whole work list; counter / n_items per job (pos_ctr, ends)
+    const MolRec *mols;      // molecule table, indexed by job
+    const int32_t *pos_job;  // job of each work-list position
+    const int32_t *ends;     // end of each position's range of io.items (written on the device)
+    int32_t *pos_ctr;        // per-position refill counters, zero at launch
+    const int32_t *wg_pos;   // first position of each workgroup
+    int pos0, npos;          // this launch's positions
+};
+
+template <int PREC, int FT, int L>
+struct LdsMultiSource {
+    static constexpr bool kCoop = false;
+    static constexpr bool kMulti = true;
+    const LdsMultiArgs &A;
+    char *mol;
+    WgCtl *ctl;
+    const MolRec *rec = nullptr;
+    int gen = 0;
+    bool counted = false;
+    __device__ void bind(int) {}
+    __device__ int pairs() const { return rec->m.n; }
+    __device__ double mu() const { return rec->m.mu; }
+    __device__ double dt0() const { return rec->dt0; }
+    __device__ bool exits(double px, double py, double pz) const
+    {
+        return PREC == 1 ? outside(rec->m, px, py, pz) : outside_div(rec->m, px, py, pz);
+    }
+    __device__ void force(bool, int j, double ex, double ey, double ez, double &fx, double &fy, double &fz) const
+    {
+        const int npad = rec->npad;
+        if constexpr (PREC == 1) {
+            const f4n *xq = (const f4n *)mol;
+            grp_force32<FT, L>(xq, (const f2n *)(xq + npad), npad, j, (float)ex, (float)ey, (float)ez,
+                               rec->m.dipole_scalar, fx, fy, fz);
+        } else {
+            grp_force64<FT, L>((const double *)mol, npad, j, ex, ey, ez, rec->m.dipole_scalar, fx, fy, fz);
+        }
+    }
+    __device__ double potential(bool, int j, double ex, double ey, double ez) const
+    {
+        return grp_potential<L>(glb(rec->soa), rec->npad, rec->m.charged, rec->m.dipole_scalar, j, ex, ey, ez);
+    }
+    // refill from position p's range (p wave-uniform)
+    __device__ void use(int p, TrajIO &io)
+    {
+        p = __builtin_amdgcn_readfirstlane(p);
+        rec = A.mols + A.pos_job[p];
+        const int b = p ? A.ends[p - 1] : 0;
+        io.items = A.io.items + b;
+        io.n_items = A.ends[p] - b;
+        io.counter = A.pos_ctr + p;
+    }
+    __device__ void count_in()
+    {
+        if (!counted) {
+            if ((threadIdx.x & 63) == 0) atomicAdd(&ctl->active, 1);
+            counted = true;
+        }
+    }
+    __device__ void count_out()
+    {
+        if (counted) {
+            if ((threadIdx.x & 63) == 0) atomicSub(&ctl->active, 1);
+            counted = false;
+        }
+    }
+    // the molecule of job rec into LDS by this wavefront alone
+    __device__ void stage_wave(const MolRec *r) const
+    {
+        const int lane = threadIdx.x & 63, npad = r->npad;
+        if constexpr (PREC == 1) {
+            const GLB f4n *gx = (const GLB f4n *)glb(r->xq);
+            const GLB f2n *ga = (const GLB f2n *)glb(r->se);
+            f4n *lx = (f4n *)mol;
+            f2n *la = (f2n *)(lx + npad);
+            for (int i = lane; i < npad; i += 4 * 64) {  // npad is a multiple of 64
+                f4n x[4];
+                f2n a[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int k = min(i + 64 * u, npad - 64 + lane);
+                    x[u] = gx[k];
+                    a[u] = ga[k];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (i + 64 * u < npad) {
+                        lx[i + 64 * u] = x[u];
+                        la[i + 64 * u] = a[u];
+                    }
+            }
+        } else {
+            const GLB double *gs = glb(r->soa);
+            double *ls = (double *)mol;
+            const int tot = 6 * npad;
+            for (int i = lane; i < tot; i += 4 * 64) {
+                double x[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) x[u] = gs[min(i + 64 * u, tot - 64 + lane)];
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (i + 64 * u < tot) ls[i + 64 * u] = x[u];
+            }
+        }
+    }
+    // position with the most trajectories left (-1: none), wave-uniform
+    __device__ int pick() const
+    {
+        const int lane = threadIdx.x & 63;
+        int best = -1, left = 0;
+        for (int q = lane; q < A.npos; q += 64) {
+            const int p = A.pos0 + q;
+            const int b = p ? A.ends[p - 1] : 0;
+            const int l = A.ends[p] - b - __hip_atomic_load(A.pos_ctr + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (l > left) {
+                left = l;
+                best = p;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {  // largest left, then lowest position
+            const int l2 = __shfl_xor(left, off), b2 = __shfl_xor(best, off);
+            if (l2 > left || (l2 == left && l2 > 0 && b2 < best)) {
+                left = l2;
+                best = b2;
+            }
+        }
+        return __builtin_amdgcn_readfirstlane(left > 0 ? best : -1);
+    }
+    // This wavefront's groups are idle and its job's range is exhausted:
+    // wait for the next publication (restaging it when the workgroup holds no
+    // trajectory), then refill from the new range.  false: nothing is left.
+    __device__ bool next_job(TrajIO &io)
+    {
+        const int lane = threadIdx.x & 63;
+        for (;;) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const int g = __builtin_amdgcn_readfirstlane(__hip_atomic_load(&ctl->gen, __ATOMIC_ACQUIRE,
+                                                                          __HIP_MEMORY_SCOPE_WORKGROUP));
+            if (g != gen) {
+                gen = g;
+                const int p = __builtin_amdgcn_readfirstlane(
+                    __hip_atomic_load(&ctl->pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+                if (p < 0) return false;
+                use(p, io);
+                return true;
+            }
+            const int act = __builtin_amdgcn_readfirstlane(
+                __hip_atomic_load(&ctl->active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+            if (act == 0) {
+                int won = 0;
+                if (lane == 0) won = atomicCAS(&ctl->stager, 0, 1) == 0;
+                won = __builtin_amdgcn_readfirstlane(won);
+                if (won) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    const int g2 = __builtin_amdgcn_readfirstlane(
+                        __hip_atomic_load(&ctl->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+                    if (g2 == gen) {  // nobody published since this wavefront last looked
+                        const int p = pick();
+                        if (p >= 0) stage_wave(A.mols + A.pos_job[p]);
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                        if (lane == 0) {
+                            __hip_atomic_store(&ctl->pos, p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            __hip_atomic_fetch_add(&ctl->gen, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    if (lane == 0) __hip_atomic_store(&ctl->stager, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    continue;
+                }
+            }
+            __builtin_amdgcn_s_sleep(8);
+        }
+    }
+};
+
 enum : int { PH_IDLE = 0, PH_INIT = 1, PH_STEP = 2, PH_CHECK = 3 };
 
 // Integration steps past which a trajectory counts as long (the 99th
@@ -928,7 +1133,7 @@ constexpr int kLongSteps = 1024;
 // (src/diffEqSolver.cpp:81-211) for the group of lanes this lane belongs
 // to, refilled from the work list until it is exhausted.
 template <int L, class Src>
-__device__ __forceinline__ void group_loop(const TrajIO &A, Src &S)
+__device__ __forceinline__ void group_loop(TrajIO A, Src &S)
 {
     const int lane = threadIdx.x & 63;
     const int g0 = lane & ~(L - 1);   // first lane of my group
@@ -946,6 +1151,7 @@ __device__ __forceinline__ void group_loop(const TrajIO &A, Src &S)
             const bool need = (phase == PH_IDLE);
             const unsigned long long mask = __ballot(need && j == 0);
             if (mask) {
+                if constexpr (Src::kMulti) S.count_in();  // before taking trajectories of the staged job
                 const int cnt = __popcll(mask);
                 const int leader = __ffsll((unsigned long long)mask) - 1;
                 int base = 0;
@@ -973,7 +1179,16 @@ __device__ __forceinline__ void group_loop(const TrajIO &A, Src &S)
         if constexpr (Src::kCoop) {
             if (!__syncthreads_or(phase != PH_IDLE)) break;
         } else {
-            if (!__any(phase != PH_IDLE)) break;
+            if (!__any(phase != PH_IDLE)) {
+                if constexpr (Src::kMulti) {
+                    S.count_out();
+                    if (S.next_job(A)) {
+                        exhausted = false;
+                        continue;
+                    }
+                }
+                break;
+            }
             // a wavefront carrying a long trajectory issues first: a launch
             // ends with its longest trajectories, and the SIMD's other
             // wavefronts fill the cycles around them (issue order only)
@@ -1116,7 +1331,7 @@ template <int PREC>
 __device__ __forceinline__ void stage_molecule(const GroupArgs &A, char *smem)
 {
     const int npad = A.npad;
-    const double *S = A.soa;
+    const GLB double *S = glb(A.soa);
     if constexpr (PREC == 1) {
         float4 *xq = (float4 *)smem;
         float2 *se = (float2 *)(xq + npad);
@@ -1172,6 +1387,37 @@ __global__ void __launch_bounds__(1024) k_force_rate(GroupArgs A, int evals, dou
         acc += fx + fy + fz;
     }
     if (acc == -1.0e300) *sink = acc;
+}
+
+// The jobs of one launch shape in one persistent launch, each workgroup
+// working on one job at a time with that job's molecule in its LDS
+// (LdsMultiSource).  Dynamic LDS: the control block, then the largest
+// molecule of the launch.
+template <int PREC, int FT, int L>
+__global__ void __launch_bounds__(1024) k_traj_lds_multi(LdsMultiArgs A)
+{
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    WgCtl *ctl = (WgCtl *)smem;
+    char *mol = smem + kCtlBytes;
+    const int p0 = A.wg_pos[blockIdx.x];
+    const MolRec *r0 = A.mols + A.pos_job[p0];
+    {
+        GroupArgs G{};
+        G.npad = r0->npad;
+        G.soa = r0->soa;
+        stage_molecule<PREC>(G, mol);
+    }
+    if (threadIdx.x == 0) {
+        ctl->pos = p0;
+        ctl->gen = 0;
+        ctl->active = 0;
+        ctl->stager = 0;
+    }
+    __syncthreads();
+    LdsMultiSource<PREC, FT, L> src{A, mol, ctl};
+    TrajIO io = A.io;
+    src.use(p0, io);
+    group_loop<L>(io, src);
 }
 
 // Molecules beyond one workgroup's LDS, exact force types: the molecule

@@ -113,7 +113,7 @@ SIZE_CLASSES = 5
 class BatchCfg(C.Structure):
     _fields_ = [("prec", C.c_int32), ("lanes", C.c_int32), ("wg_size", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("class_lanes", C.c_int32 * SIZE_CLASSES), ("class_wg", C.c_int32 * SIZE_CLASSES),
-                ("shared", C.c_int32)]
+                ("own_launches", C.c_int32)]
 
 
 class BatchStats(C.Structure):
@@ -456,16 +456,19 @@ def tune_batch(jobs, prec=PREC_MIXED):
 
 
 def run_batch(jobs, prec=PREC_MIXED, lanes=0, wg_size=0, blocks_per_cu=0, class_lanes=None, class_wg=None,
-              shared=False):
-    """All jobs in one call: a launch of its own for each job of an exact
-    force type with enough work (LDS-staged or tiled kernel), shared
-    table-kernel launches for the rest; shared=True puts every job of a
-    launch shape in one persistent launch.  lanes > 0 fixes the lanes per
+              own_launches=False):
+    """All jobs in one call.  By default the jobs of a launch shape whose
+    molecule fits LDS share one persistent launch (each workgroup stages the
+    molecule of the job it works on), larger exact-type jobs run tiled
+    launches of their own and clustered jobs share table-kernel launches;
+    own_launches=True gives each exact-type job with enough work a launch of
+    its own (LDS-staged or tiled) instead.  lanes > 0 fixes the lanes per
     trajectory, 0 applies the size rule, -1 uses class_lanes / class_wg
     (tuned on the batch when not given).
     Returns (per-job result dicts like Molecule.run, batch statistics)."""
     arr, keep = _batch_jobs(jobs)
-    cfg = BatchCfg(prec=prec, lanes=lanes, wg_size=wg_size, blocks_per_cu=blocks_per_cu, shared=int(shared))
+    cfg = BatchCfg(prec=prec, lanes=lanes, wg_size=wg_size, blocks_per_cu=blocks_per_cu,
+                   own_launches=int(own_launches))
     for c in range(SIZE_CLASSES):
         cfg.class_lanes[c] = (class_lanes or [0] * SIZE_CLASSES)[c]
         cfg.class_wg[c] = (class_wg or [0] * SIZE_CLASSES)[c]

@@ -234,15 +234,18 @@ int cosims_run(const cosims_mol *m, const cosims_params *p, const cosims_run_cfg
 /* Many CCS runs in one call.  The reference runs one molecule per process
  * (src/main.cpp:119-140, one monteTraj::runTraj each); here all jobs are
  * sampled into one work list, ordered by launch shape (force type x lanes
- * per trajectory x workgroup size), larger molecules first.  With
- * cfg.shared, the molecules of uneven size of a launch shape share one
- * persistent launch, each group of lanes reading the atoms of its own
- * trajectory's molecule through L2; by default a job of an exact type with
- * enough work runs a launch of its own with its molecule in LDS (faster),
- * and only the rest share launches.  The launches of a batch run
- * concurrently on separate streams.  Each job's results are those
- * cosims_run gives for the same molecule, seed, job id and launch shape, in
- * either mode.
+ * per trajectory x workgroup size), larger molecules first.  By default
+ * the jobs of a launch shape whose molecule fits one workgroup's
+ * LDS share one persistent launch: each workgroup stages the molecule of the
+ * job it works on in LDS, refills its groups from that job's trajectories
+ * (longest expected first), and when they are spent moves to the job with
+ * the most trajectories left, restaging; larger molecules of the exact types
+ * run tiled launches of their own and the clustered types share launches
+ * that read the molecules through L2.  With cfg.own_launches, a job of an
+ * exact type with enough work runs a launch of its own (LDS-staged or
+ * tiled) and only the rest share launches.  The launches of a batch run concurrently on
+ * separate streams.  Each job's results are those cosims_run gives for the
+ * same molecule, seed, job id and launch shape, in either mode.
  *
  * Lanes per trajectory follow the molecule's size: size class k holds the
  * molecules whose force loop sweeps n units (atoms, or clusters for the
@@ -269,10 +272,10 @@ typedef struct cosims_batch_cfg {
     int32_t blocks_per_cu;    /* 0 => occupancy limit */
     int32_t class_lanes[COSIMS_SIZE_CLASSES];
     int32_t class_wg[COSIMS_SIZE_CLASSES];
-    int32_t shared;           /* 0: a job of an exact type whose work is large enough runs a launch of
-                                 its own (its molecule staged in LDS, or tiled through LDS); 1: every
-                                 job of a launch shape shares one persistent launch, each group
-                                 reading its trajectory's molecule through L2 */
+    int32_t own_launches;     /* 0 (default): the jobs of a launch shape within LDS share one persistent
+                                 launch, each workgroup staging the molecule of the job it works on;
+                                 1: a job of an exact type whose work is large enough runs a launch
+                                 of its own (its molecule staged in LDS, or tiled through LDS) */
 } cosims_batch_cfg;
 
 typedef struct cosims_batch_stats {
@@ -287,7 +290,7 @@ typedef struct cosims_batch_stats {
 int cosims_run_batch(cosims_batch_job *jobs, int32_t njobs, const cosims_batch_cfg *cfg,
                      cosims_batch_stats *stats);
 /* Per size class, the lanes and workgroup size the tuner picks for these
- * jobs (0 for classes without jobs). */
+ * jobs in the default (shared-launch) mode (0 for classes without jobs). */
 int cosims_tune_batch(const cosims_batch_job *jobs, int32_t njobs, int32_t prec, int32_t *class_lanes,
                       int32_t *class_wg);
 

@@ -7,8 +7,8 @@ reference's golden record, the oracle or the engine's own single runs.
   tiled kernel (exact type, molecule streamed through LDS) and the 64-lane
   clustered table kernel (median |dchi| <= 1e-8 rad), and mixed precision
   against FP64 on paired initial conditions (CCS within 0.1 %).
-* 256 conformers, N log-uniform in 50-5,000: one call, default (own
-  launches) and shared modes; every conformer's per-integral CCS equals
+* 256 conformers, N log-uniform in 50-5,000: one call, default (shared
+  LDS launches) and own-launch modes; every conformer's per-integral CCS equals
   cosims_run of that conformer alone bit for bit; every 16th conformer at
   50,000 trajectories in mixed precision within 0.1 % of FP64 (paired).
 * A replayed Poisson stream of 12 requests (N 109-18,935 atoms, He 222-382
@@ -159,12 +159,14 @@ def conformers():
 
 def test_conformer_batch_equals_single_runs(conformers):
     """256 uneven conformers, 500 trajectories each, size-rule lanes: one
-    call in the default mode (own launches for the larger jobs, shared
-    launches for the tiny ones) and in the shared mode; every conformer's
-    per-integral CCS equals cosims_run of it alone with its launch shape."""
+    call in the default mode (one shared persistent launch per size class,
+    each workgroup staging the molecule of the conformer it works on) and
+    with own launches (a launch each for the larger jobs, shared table
+    launches for the tiny ones); every conformer's per-integral CCS equals
+    cosims_run of it alone with its launch shape."""
     jobs = [(m, dict(job=k, iters=1, traj=500)) for k, m in enumerate(conformers)]
-    own, st = E.run_batch(jobs, prec=E.PREC_MIXED, lanes=0)
-    shared, st2 = E.run_batch(jobs, prec=E.PREC_MIXED, lanes=0, shared=True)
+    own, st = E.run_batch(jobs, prec=E.PREC_MIXED, lanes=0, own_launches=True)
+    shared, st2 = E.run_batch(jobs, prec=E.PREC_MIXED, lanes=0)
     assert st["n_traj"] == st2["n_traj"] == 256 * 500
     assert st2["launches"] <= E.SIZE_CLASSES
     for k, (m, spec) in enumerate(jobs):

@@ -88,7 +88,8 @@ def ion2000():
 
 @pytest.mark.parametrize("prec", [E.PREC_MIXED, E.PREC_FP64])
 def test_batch_of_one_equals_run(ion128, prec):
-    """Table kernel (batch) and LDS kernel (cosims_run) give identical results."""
+    """A batch of one job (shared LDS launch) and cosims_run (LDS kernel)
+    give identical results."""
     shape = dict(lanes=8, wg_size=256)
     single = ion128.run(prec, job=11, **shape)
     (b,), st = E.run_batch([(ion128, dict(job=11))], prec=prec, **shape)
@@ -97,10 +98,12 @@ def test_batch_of_one_equals_run(ion128, prec):
     assert b["n_hit"] == single["n_hit"] and b["total_steps"] == single["total_steps"]
 
 
-def test_batch_alone_equals_mix(ion128, ion2000):
+@pytest.mark.parametrize("own", [False, True], ids=["shared", "own"])
+def test_batch_alone_equals_mix(ion128, ion2000, own):
     """Uneven molecules and two force types in one call: each job's result
-    is the same alone and in the mix.  The exact-type jobs fit LDS and run
-    one LDS-kernel launch each; the clustered job runs the table kernel."""
+    is the same alone and in the mix.  The exact-type jobs fit LDS: one
+    shared LDS launch for all four (default) or one LDS-kernel launch each
+    (own launches); the clustered job runs the table kernel."""
     g = golden("acau_charged")
     ac1 = golden_engine_mol(g, [1, 1, 1, 1])   # clustered, type 1
     ac4 = golden_engine_mol(g, [1, 0, 0, 1])   # exact, type 4
@@ -108,12 +111,12 @@ def test_batch_alone_equals_mix(ion128, ion2000):
             (ac1, dict(job=3, iters=2, traj=1500)), (ac4, dict(job=4, iters=1, traj=2500)),
             (ion128, dict(job=5, iters=1, traj=1000))]
     shape = dict(lanes=16, wg_size=512)
-    mix, st = E.run_batch(jobs, prec=E.PREC_MIXED, **shape)
-    assert st["launches"] == 5
+    mix, st = E.run_batch(jobs, prec=E.PREC_MIXED, own_launches=own, **shape)
+    assert st["launches"] == (5 if own else 2)
     assert st["n_traj"] == sum(j[1].get("iters", j[0].params.iter) * j[1].get("traj", j[0].params.traj)
                                for j in jobs)
     for k, job in enumerate(jobs):
-        (alone,), _ = E.run_batch([job], prec=E.PREC_MIXED, **shape)
+        (alone,), _ = E.run_batch([job], prec=E.PREC_MIXED, own_launches=own, **shape)
         assert np.array_equal(mix[k]["omega"], alone["omega"]), k
         assert mix[k]["wmean"] == alone["wmean"]
 
@@ -197,11 +200,12 @@ def test_large_molecule_beyond_lds(ion8000):
     assert abs(mix["wmean"] - lane["wmean"]) / lane["wmean"] <= 1e-3
 
 
-def test_batch_mixes_table_and_lds_launches(ion8000, ion128, ion2000):
-    """One launch shape, molecules on both sides of the LDS limit: the
-    8,000-atom job runs the table kernel, the others one LDS launch each;
-    every result equals the job alone and cosims_run with the same lanes."""
-    shape = dict(lanes=16, wg_size=256)
+def test_batch_mixes_tiled_and_lds_launches(ion8000, ion128, ion2000):
+    """One launch shape, molecules on both sides of the LDS limit, own
+    launches: the 8,000-atom job runs the tiled kernel, the others one LDS
+    launch each; every result equals the job alone and cosims_run with the
+    same lanes."""
+    shape = dict(lanes=16, wg_size=256, own_launches=True)
     jobs = [(ion128, dict(job=41, iters=1, traj=1500)), (ion8000, dict(job=42, iters=1, traj=600)),
             (ion2000, dict(job=43, iters=2, traj=800))]
     for prec in (E.PREC_MIXED, E.PREC_FP64):
@@ -210,19 +214,21 @@ def test_batch_mixes_table_and_lds_launches(ion8000, ion128, ion2000):
         for k, job in enumerate(jobs):
             (alone,), _ = E.run_batch([job], prec=prec, **shape)
             assert np.array_equal(mix[k]["omega"], alone["omega"]), (prec, k)
-            single = job[0].run(prec, job=job[1]["job"], iters=job[1]["iters"], traj=job[1]["traj"], **shape)
+            single = job[0].run(prec, job=job[1]["job"], iters=job[1]["iters"], traj=job[1]["traj"], lanes=16,
+                                wg_size=256)
             assert np.array_equal(mix[k]["omega"], single["omega"]), (prec, k)
             assert mix[k]["total_steps"] == single["total_steps"]
 
 
 def test_batch_tiny_jobs_share_a_table_launch(ion128, ion2000):
-    """Jobs below 65,536 trajectories x atoms share one table-kernel launch
-    (a launch each would cost more than their work); a larger job keeps its
-    own.  Every result equals cosims_run (LDS kernel) bit for bit."""
+    """Own launches: jobs below 65,536 trajectories x atoms share one
+    table-kernel launch (a launch each would cost more than their work); a
+    larger job keeps its own.  Every result equals cosims_run (LDS kernel)
+    bit for bit."""
     shape = dict(lanes=8, wg_size=256)
     jobs = [(ion128, dict(job=50 + k, iters=1, traj=300)) for k in range(4)]
     jobs.append((ion2000, dict(job=60, iters=1, traj=1000)))
-    mix, st = E.run_batch(jobs, prec=E.PREC_MIXED, **shape)
+    mix, st = E.run_batch(jobs, prec=E.PREC_MIXED, own_launches=True, **shape)
     assert st["launches"] == 2
     for k, (m, spec) in enumerate(jobs):
         single = m.run(E.PREC_MIXED, job=spec["job"], iters=1, traj=spec["traj"], **shape)
@@ -232,16 +238,35 @@ def test_batch_tiny_jobs_share_a_table_launch(ion128, ion2000):
 
 @pytest.mark.parametrize("prec", [E.PREC_MIXED, E.PREC_FP64])
 def test_batch_shared_launch_equals_own_launches(ion128, ion2000, ion8000, prec):
-    """shared=True: every job of a launch shape in one persistent launch
-    (molecules of 128, 2,000 and 8,000 atoms read through L2); the results
-    are those of the default own launches (LDS-staged, tiled) bit for bit."""
+    """The default shared mode: the jobs of a launch shape within LDS (128
+    and 2,000 atoms) in one persistent launch, each workgroup staging the
+    molecule of the job it works on; the 8,000-atom job keeps a tiled launch
+    of its own.  The results are those of own launches bit for bit."""
     shape = dict(lanes=16, wg_size=256)
     jobs = [(ion128, dict(job=71, iters=1, traj=1500)), (ion2000, dict(job=72, iters=1, traj=1000)),
             (ion8000, dict(job=73, iters=1, traj=400))]
-    shared, st = E.run_batch(jobs, prec=prec, shared=True, **shape)
-    assert st["launches"] == 1
-    own, st2 = E.run_batch(jobs, prec=prec, **shape)
+    shared, st = E.run_batch(jobs, prec=prec, **shape)
+    assert st["launches"] == 2
+    own, st2 = E.run_batch(jobs, prec=prec, own_launches=True, **shape)
     assert st2["launches"] == 3
     for k in range(len(jobs)):
         assert np.array_equal(shared[k]["omega"], own[k]["omega"]), (prec, k)
         assert shared[k]["total_steps"] == own[k]["total_steps"], (prec, k)
+
+
+@pytest.mark.parametrize("prec", [E.PREC_MIXED, E.PREC_FP64])
+def test_batch_shared_lds_restaging(ion128, ion2000, prec):
+    """24 jobs of one launch shape, 128 and 2,000 atoms, 300 to 2,531
+    trajectories each, in one shared persistent launch: workgroups start on
+    jobs in proportion to their work, spend them and restage the molecule of
+    the job with the most trajectories left, many times over.  Every job's
+    result equals cosims_run of it alone bit for bit."""
+    shape = dict(lanes=16, wg_size=512)
+    jobs = [(ion2000 if k % 3 == 0 else ion128, dict(job=200 + k, iters=1, traj=300 + 97 * k)) for k in range(24)]
+    shared, st = E.run_batch(jobs, prec=prec, **shape)
+    assert st["launches"] == 1
+    assert st["n_traj"] == sum(spec["traj"] for _, spec in jobs)
+    for k, (m, spec) in enumerate(jobs):
+        single = m.run(prec, job=spec["job"], iters=1, traj=spec["traj"], **shape)
+        assert np.array_equal(shared[k]["omega"], single["omega"]), (prec, k)
+        assert shared[k]["total_steps"] == single["total_steps"], (prec, k)
