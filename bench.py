@@ -29,6 +29,7 @@ FLOP_PAIR, FLOP_EPI, FLOP_VERLET = 58, 21, 50   # algorithmic FLOPs, type-4 pair
 # TFLOP/s, MI355X FP32 vector peak.  Reachable without packed FP32: gfx950 issues an FP32 VALU
 # instruction whose operands are all VGPRs in 2 cycles a wavefront (one with an SGPR operand in 4)
 PEAK_FP32 = 157.3
+PEAK_FP64 = 78.6  # TFLOP/s, MI355X FP64 vector peak
 # HBM bytes per launch of the timed mixed kernel (rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0,
 # scale factors from a 256 MiB device copy; 16 lanes, workgroup 256), from
 # profiles/traj_grp_mixed_ion2000.md
@@ -283,14 +284,15 @@ def clustered_workload(E, synth, traj, oracle_sample):
         runs[label] = (m, r)
         out[label] = {"force_type": inf["force_type"], "clusters": inf["ncl"], "setup_s": setup,
                       "trajectories_per_s": traj / dt, "ccs": r["wmean"], "ccs_err": r["werr"],
-                      "lanes": r["lanes"], "wg_size": r["wg_size"], "kernel_s": r["traj_kernel_ms"][0] / 1e3}
+                      "lanes": r["lanes"], "wg_size": r["wg_size"], "kernel_s": r["traj_kernel_ms"][0] / 1e3,
+                      **launch_roofline(r, PEAK_FP32)}
     m4, r4 = runs["exact"]
     t1 = time.perf_counter()
     f = m4.run(E.PREC_FP64, job=0, keep=True, lanes=64, wg_size=512)
     dt = time.perf_counter() - t1
     log(f"c4 exact FP64: {dt:.2f} s, CCS {f['wmean']:.2f} +- {f['werr']:.2f}")
     out["exact_fp64"] = {"trajectories_per_s": traj / dt, "ccs": f["wmean"], "ccs_err": f["werr"],
-                         "lanes": f["lanes"]}
+                         "lanes": f["lanes"], **launch_roofline(f, PEAK_FP64)}
     for label in ("clustered", "exact"):
         r = runs[label][1]
         dd = r["omega"] - f["omega"]
@@ -306,7 +308,7 @@ def clustered_workload(E, synth, traj, oracle_sample):
     d1 = r1["omega"] - f1["omega"]
     log(f"c4 clustered FP64: {dt:.2f} s, CCS {f1['wmean']:.2f} +- {f1['werr']:.2f}")
     out["clustered_fp64"] = {"trajectories_per_s": traj / dt, "ccs": f1["wmean"], "ccs_err": f1["werr"],
-                             "lanes": f1["lanes"]}
+                             "lanes": f1["lanes"], **launch_roofline(f1, PEAK_FP64)}
     out["clustered"]["paired_rel_dev_vs_clustered_fp64"] = float(d1.mean() / f1["omega"].mean())
     out["clustered"]["paired_rel_se_vs_clustered_fp64"] = float(d1.std(ddof=1) / np.sqrt(d1.size)
                                                                 / f1["omega"].mean())
@@ -330,6 +332,18 @@ def clustered_workload(E, synth, traj, oracle_sample):
     for m, _ in runs.values():
         m.close()
     return out
+
+
+def launch_roofline(r, peak):
+    """Reference FLOPs of a run's trajectory launch (the library's per-branch
+    count, cosims_ccs.ref_flops) over its HIP-event time, against a peak."""
+    if r["n_rounds"] != 1 or not r["ref_flops"]:
+        return {}
+    tf = r["ref_flops"] / (r["traj_kernel_ms"][0] * 1e-3) / 1e12
+    cl = dict(zip(("clusters_examined", "far", "far_dipole", "atoms_charge_lj", "atoms_charge", "atoms_lj"),
+                  r["cl_counts"])) if any(r["cl_counts"]) else None
+    return {"achieved_tflops": tf, "peak_tflops": peak, "frac": tf / peak, "ref_flops": r["ref_flops"],
+            "branch_counts": cl}
 
 
 def copy_probe(nbytes=256 << 20, reps=3):

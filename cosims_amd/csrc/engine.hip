@@ -1278,6 +1278,11 @@ SampleArgs sample_args(const cosims_mol *M, uint64_t seed, uint32_t job)
     return A;
 }
 
+// Counters of TrajBufs::evals: the lane kernel's 2 per attempt round, or a
+// group launch's kEvalSlots.
+constexpr int kEvalBufSlots = 16;
+static_assert(kEvalBufSlots >= 2 * COSIMS_MAX_ROUNDS && kEvalBufSlots >= kEvalSlots, "evals slots");
+
 // Work buffers of one batch of trajectories (device).
 struct TrajBufs {
     DBuf p0, v0, g, status, chi, steps, tot, tries, maxtries, exitp, exitv2, e0, ergy, items[2], counters, evals,
@@ -1302,7 +1307,7 @@ struct TrajBufs {
         HIP_CHECK(items[0].alloc(sizeof(int32_t) * n));
         HIP_CHECK(items[1].alloc(sizeof(int32_t) * n));
         HIP_CHECK(counters.alloc(sizeof(int32_t) * 64));
-        HIP_CHECK(evals.alloc(sizeof(unsigned long long) * 2 * COSIMS_MAX_ROUNDS));
+        HIP_CHECK(evals.alloc(sizeof(unsigned long long) * kEvalBufSlots));
         return 0;
     }
 };
@@ -1451,7 +1456,7 @@ int run_rounds(const cosims_mol *M, int prec, int64_t n, TrajBufs &B, hipStream_
     hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, s, B.tries.as<int32_t>(), n, 0);
     hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, s, B.maxtries.as<int32_t>(), n, 8);
     hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, s, B.tot.as<int32_t>(), n, 0);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, B.evals.as<int32_t>(), 4 * COSIMS_MAX_ROUNDS, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, B.evals.as<int32_t>(), 2 * kEvalBufSlots, 0);
     HIP_CHECK(hipGetLastError());
     int32_t *ctr = B.counters.as<int32_t>();
     int32_t count = 0;
@@ -1641,7 +1646,7 @@ MultiFn multi_kernel(int prec, int ft, int L) { return prec == COSIMS_PREC_FP64 
 
 size_t multi_lds_bytes(int ft, int L, int wg)
 {
-    return multi_uses_lists(ft, L) ? (size_t)(wg / 64) * kListInts * sizeof(int) : 0;
+    return multi_uses_lists(ft, L) ? (size_t)(wg / 64) * (kListInts + kCountInts) * sizeof(int) : 0;
 }
 
 size_t group_lds_bytes(const cosims_mol *M, int prec)
@@ -1701,8 +1706,35 @@ int group_units(const Prepared &p)
 struct GroupRun {
     double ms = 0;
     int64_t steps = 0, evals = 0, pots = 0, pairs = 0;
+    int64_t cl[6] = {0, 0, 0, 0, 0, 0};  // clustered-type branch counts (ClCount order)
+    double flops = 0;                    // reference FLOPs (ref_flops)
     int grid = 0;
 };
+
+// Reference FLOPs of force evaluations, integration steps and potential
+// atom pairs of one force type (multipole order for the far clusters),
+// counted per branch as the reference writes them (the table in
+// cosims_ccs.ref_flops' comment).
+double ref_flops(int force_type, int order, int64_t evals, int64_t steps, int64_t pot_pairs, int64_t pairs,
+                 const int64_t *cl)
+{
+    const double hdr = (double)cl[0], far = (double)cl[1], fdip = (double)cl[2], exlj = (double)cl[3],
+                 exq = (double)cl[4], lj = (double)cl[5];
+    double f = 0;
+    switch (force_type) {
+    case 4: f = 58.0 * (double)pairs + 21.0 * (double)evals; break;
+    case 6: f = 22.0 * (double)pairs; break;
+    case 1:
+    case 2:
+        f = 12.0 * (hdr - far) + 49.0 * far + 67.0 * fdip + (order >= 2 ? 89.0 * far : 0.0) +
+            62.0 * exlj + 48.0 * exq + 22.0 * lj + 27.0 * (double)evals;
+        break;
+    case 3: f = 10.0 * hdr + 62.0 * exlj + 48.0 * exq + 21.0 * (double)evals; break;
+    default: f = 10.0 * hdr + 22.0 * lj; break;  // type 5
+    }
+    const bool charged = force_type != 5 && force_type != 6;
+    return f + 50.0 * (double)steps + (charged ? 24.0 : 15.0) * (double)pot_pairs;
+}
 
 TrajIO traj_io(TrajBufs &B, const int32_t *items, int n_items)
 {
@@ -1739,7 +1771,7 @@ int timed_group_launch(void (*fn)(Args), const Args &args, int64_t grid, int wg,
 {
     int32_t *ctr = B.counters.as<int32_t>();
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ctr + 1, 2, 0);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, B.evals.as<int32_t>(), 4 * COSIMS_MAX_ROUNDS, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, B.evals.as<int32_t>(), 2 * kEvalBufSlots, 0);
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
@@ -1750,7 +1782,7 @@ int timed_group_launch(void (*fn)(Args), const Args &args, int64_t grid, int wg,
     hipLaunchKernelGGL(k_chi, dim3((unsigned)((n_items + 255) / 256)), dim3(256), 0, s, args.io.items, n_items,
                        args.io.status, args.io.cchi, args.io.chi, (const int32_t *)nullptr, (const int32_t *)nullptr);
     HIP_CHECK(hipGetLastError());
-    unsigned long long ev[4];
+    unsigned long long ev[kEvalSlots];
     HIP_CHECK(hipMemcpyAsync(ev, B.evals.p, sizeof(ev), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     float ms = 0;
@@ -1762,6 +1794,7 @@ int timed_group_launch(void (*fn)(Args), const Args &args, int64_t grid, int wg,
     out.evals = (int64_t)ev[1];
     out.pots = (int64_t)ev[2];
     out.pairs = (int64_t)ev[3];
+    for (int k = 0; k < 6; k++) out.cl[k] = (int64_t)ev[4 + k];
     out.grid = (int)grid;
     return 0;
 }
@@ -2072,6 +2105,7 @@ struct BatchPlan {
     int64_t n = 0;
     int cus = 0;
     int nlaunch = 0;                     // launches of the last launch_enqueue
+    std::vector<int> launch_i0;          // first work-list position of each of them
     bool shared = false;                 // every job of a launch shape in one persistent launch
     TrajBufs B;
     DBuf tmol, table, seg_ctr, seg_evals, job_end, order_scr;
@@ -2318,9 +2352,11 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
     const int nl = (int)plan.size();
     if (nl > kMaxLaunches) return fail(COSIMS_E_INVALID, "batch: too many launches");
     P.nlaunch = nl;
+    P.launch_i0.resize(nl);
+    for (int li = 0; li < nl; li++) P.launch_i0[li] = plan[li].i0;
     if (!P.seg_ctr.p) {
         HIP_CHECK(P.seg_ctr.alloc(sizeof(int32_t) * kMaxLaunches));
-        HIP_CHECK(P.seg_evals.alloc(sizeof(unsigned long long) * 4 * kMaxLaunches));
+        HIP_CHECK(P.seg_evals.alloc(sizeof(unsigned long long) * kEvalSlots * kMaxLaunches));
     }
     if (kOrderWork && nl > P.order_cap) {
         HIP_CHECK(P.order_scr.alloc(sizeof(int32_t) * 2 * kOrderBins * nl));
@@ -2340,7 +2376,8 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
     if (nl > 0) {
         const int nc = std::max(1, (nl + 63) / 64);
         hipLaunchKernelGGL(k_fill_i32, dim3(nc), dim3(64), 0, s, P.seg_ctr.as<int32_t>(), (int64_t)nl, 0);
-        hipLaunchKernelGGL(k_fill_i32, dim3(8 * nc), dim3(64), 0, s, P.seg_evals.as<int32_t>(), (int64_t)8 * nl, 0);
+        hipLaunchKernelGGL(k_fill_i32, dim3(2 * kEvalSlots * nc), dim3(64), 0, s, P.seg_evals.as<int32_t>(),
+                           (int64_t)2 * kEvalSlots * nl, 0);
     }
     std::vector<int> order(nl);
     for (int i = 0; i < nl; i++) order[i] = i;
@@ -2442,7 +2479,7 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
         const int32_t *last = ends + l.i1 - 1;
         TrajIO io = traj_io(P.B, (kOrderWork ? P.B.items[1] : P.B.items[0]).as<int32_t>(), cnt);
         io.counter = P.seg_ctr.as<int32_t>() + li;
-        io.evals = P.seg_evals.as<unsigned long long>() + 4 * li;
+        io.evals = P.seg_evals.as<unsigned long long>() + kEvalSlots * li;
         if (l.kind == OWN) {
             const cosims_mol *M = P.jobs[P.order[l.i0]].mol;
             const bool tiled = tile_mode(M, prec);
@@ -2500,9 +2537,9 @@ int launch_enqueue(BatchPlan &P, int prec, int bpc, hipStream_t s, int &launches
 int launch_finish(BatchPlan &P, hipStream_t s, GroupRun &tot)
 {
     const int nseg = P.nlaunch;
-    std::vector<unsigned long long> ev(4 * (size_t)std::max(1, nseg));
+    std::vector<unsigned long long> ev(kEvalSlots * (size_t)std::max(1, nseg));
     if (nseg > 0)
-        HIP_CHECK(hipMemcpyAsync(ev.data(), P.seg_evals.p, sizeof(unsigned long long) * 4 * nseg,
+        HIP_CHECK(hipMemcpyAsync(ev.data(), P.seg_evals.p, sizeof(unsigned long long) * kEvalSlots * nseg,
                                  hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     float ms = 0;
@@ -2510,10 +2547,19 @@ int launch_finish(BatchPlan &P, hipStream_t s, GroupRun &tot)
     tot.ms += ms;
     HIP_CHECK(hipEventElapsedTime(&P.sample_ms, P.es0, P.es1));
     for (int i = 0; i < nseg; i++) {
-        tot.steps += (int64_t)ev[4 * i];
-        tot.evals += (int64_t)ev[4 * i + 1];
-        tot.pots += (int64_t)ev[4 * i + 2];
-        tot.pairs += (int64_t)ev[4 * i + 3];
+        const unsigned long long *e = ev.data() + (size_t)kEvalSlots * i;
+        int64_t cl[6];
+        for (int k = 0; k < 6; k++) {
+            cl[k] = (int64_t)e[4 + k];
+            tot.cl[k] += cl[k];
+        }
+        tot.steps += (int64_t)e[0];
+        tot.evals += (int64_t)e[1];
+        tot.pots += (int64_t)e[2];
+        tot.pairs += (int64_t)e[3];
+        const BatchPlan::Job &J = P.jobs[P.order[P.launch_i0[i]]];
+        tot.flops += ref_flops(J.ft, J.mol->prep.multipole_order, (int64_t)e[1], (int64_t)e[0], (int64_t)e[2],
+                               (int64_t)e[3], cl);
     }
     return 0;
 }
@@ -2769,7 +2815,10 @@ int cosims_run(const cosims_mol *M, const cosims_params *P, const cosims_run_cfg
         out->round_items[0] = hits;
         out->round_steps[0] = G.steps;
         out->round_evals[0] = G.evals;
-        out->pot_evals = G.pots * (int64_t)M->prep.n;
+        out->pot_evals = G.pots;
+        for (int k = 0; k < 6; k++) out->cl_counts[k] = G.cl[k];
+        out->ref_flops = ref_flops(M->prep.force_type, M->prep.multipole_order, G.evals, G.steps, G.pots, G.pairs,
+                                   G.cl);
     }
     out->lanes = L;
     out->wg_size = wg;
@@ -2794,10 +2843,17 @@ int cosims_run(const cosims_mol *M, const cosims_params *P, const cosims_run_cfg
     finish_ccs(status.data(), sums.data(), iters, T, P->print_rate, omega, omega_err, out);
     out->n_traj = n;
     const int ft = M->prep.force_type;
-    const int64_t npair = (ft == 4 || ft == 6) ? M->prep.n : 0;  // clustered types: branch-dependent
+    // clustered types: the member atoms their branches evaluated
+    const int64_t npair = (ft == 4 || ft == 6) ? M->prep.n : 0;
     int64_t evals = 0;
     for (int r = 0; r < out->n_rounds; r++) evals += out->round_evals[r];
-    out->pair_evals = npair * evals;
+    out->pair_evals = npair * evals + out->cl_counts[3] + out->cl_counts[4] + out->cl_counts[5];
+    if (L == 1 && npair > 0) {  // the lane kernel counts steps and evaluations, not potential pairs
+        int64_t steps = 0;
+        for (int r = 0; r < out->n_rounds; r++) steps += out->round_steps[r];
+        const int64_t no_cl[6] = {0, 0, 0, 0, 0, 0};
+        out->ref_flops = ref_flops(ft, M->prep.multipole_order, evals, steps, 0, out->pair_evals, no_cl);
+    }
     out->run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return 0;
 }
@@ -2961,7 +3017,8 @@ int cosims_run_batch(cosims_batch_job *jobs, int32_t njobs, const cosims_batch_c
     stats->n_traj = P.n;
     stats->force_evals = R.evals;
     stats->pot_evals = R.pots;
-    stats->pair_evals = R.pairs;
+    stats->pair_evals = R.pairs + R.cl[3] + R.cl[4] + R.cl[5];
+    stats->ref_flops = R.flops;
     stats->run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return 0;
 }

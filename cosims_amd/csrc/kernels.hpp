@@ -175,6 +175,14 @@ __device__ __forceinline__ void acc_type6(const MolDev &m, T px, T py, T pz, T &
     ox = tx; oy = ty; oz = tz;
 }
 
+// Branch counts of the clustered types' force evaluations, this lane's
+// share: clusters examined, far clusters (multipole terms), far clusters with
+// dipole terms, member atoms with charges and LJ, with charges only, with
+// LJ only.  They price a launch in the reference's FLOPs per branch.
+struct ClCount {
+    unsigned hdr = 0, far = 0, fdip = 0, exlj = 0, exq = 0, lj = 0;
+};
+
 template <typename T>
 struct ClAcc {
     T fX = 0, fY = 0, fZ = 0, sXY = 0, sXZ = 0, sYZ = 0, aXY = 0, aXZ = 0, aYZ = 0, dXX = 0, dYY = 0, dZZ = 0,
@@ -315,33 +323,55 @@ __device__ __forceinline__ void cl_far(const MolDev &m, const CAS ClusterT<T> &H
 // src/forceAccel.cpp:244-438 (type 1, multipoleCutoff), :470-626 (type 2,
 // multipoleOnly), :655-742 (type 3, dispersionOnly), :819-851 (type 5,
 // dispersionNoCharge).
-template <typename T, int FT>
-__device__ __forceinline__ void cl_one(const MolDev &m, const CAS ClusterT<T> &H, T px, T py, T pz, ClAcc<T> &A)
+// With CNT the branches taken are counted into C.
+template <typename T, int FT, bool CNT = false>
+__device__ __forceinline__ void cl_one(const MolDev &m, const CAS ClusterT<T> &H, T px, T py, T pz, ClAcc<T> &A,
+                                       ClCount &C)
 {
     T dx0 = px - H.px, dx1 = py - H.py, dx2 = pz - H.pz;
+    const unsigned na = (unsigned)(H.end - H.start);
+    if (CNT) C.hdr++;
     if constexpr (FT == 1 || FT == 2) {
         T xx = dx0 * dx0, yy = dx1 * dx1, zz = dx2 * dx2;
         T r2 = xx + yy + zz;
         const int s = H.start, e = H.end;
         if (r2 > H.mpr) {
             cl_far<T>(m, H, dx0, dx1, dx2, r2, A);
+            if (CNT) {
+                C.far++;
+                C.fdip += (m.order >= 1 && !H.rep) ? 1u : 0u;
+            }
             if (FT == 1) {
-                if (!(r2 > H.dpr)) cl_lj<T, true>(m, s, e, H.qs6, H.qe24, px, py, pz, A);
+                if (!(r2 > H.dpr)) {
+                    cl_lj<T, true>(m, s, e, H.qs6, H.qe24, px, py, pz, A);
+                    if (CNT) C.lj += na;
+                }
             } else {
                 cl_lj<T, false>(m, s, e, T(0), T(0), px, py, pz, A);
+                if (CNT) C.lj += na;
             }
         } else if (FT == 2 || !(r2 > H.dpr)) {
             cl_exact<T, true>(m, s, e, px, py, pz, A);
+            if (CNT) C.exlj += na;
         } else {
             cl_exact<T, false>(m, s, e, px, py, pz, A);
+            if (CNT) C.exq += na;
         }
     } else {
         T r2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
         if constexpr (FT == 3) {
-            if (!(r2 > H.dpr)) cl_exact<T, true>(m, H.start, H.end, px, py, pz, A);
-            else cl_exact<T, false>(m, H.start, H.end, px, py, pz, A);
+            if (!(r2 > H.dpr)) {
+                cl_exact<T, true>(m, H.start, H.end, px, py, pz, A);
+                if (CNT) C.exlj += na;
+            } else {
+                cl_exact<T, false>(m, H.start, H.end, px, py, pz, A);
+                if (CNT) C.exq += na;
+            }
         } else {
-            if (!(r2 > H.dpr)) cl_lj<T, false>(m, H.start, H.end, T(0), T(0), px, py, pz, A);
+            if (!(r2 > H.dpr)) {
+                cl_lj<T, false>(m, H.start, H.end, T(0), T(0), px, py, pz, A);
+                if (CNT) C.lj += na;
+            }
         }
     }
 }
@@ -368,18 +398,20 @@ __device__ __forceinline__ void cl_epilogue(const MolDev &m, ClAcc<T> &A, T &ox,
 // Clustered force types over clusters first, first+stride, ...: stride 1 is
 // the reference's cluster order; the group kernel splits clusters over the
 // lanes of a group and sums the partial accumulators afterwards.
-template <typename T, int FT>
-__device__ __forceinline__ void cl_sum(const MolDev &m, int first, int stride, T px, T py, T pz, ClAcc<T> &A)
+template <typename T, int FT, bool CNT = false>
+__device__ __forceinline__ void cl_sum(const MolDev &m, int first, int stride, T px, T py, T pz, ClAcc<T> &A,
+                                       ClCount &C)
 {
     const auto H = Layout<T>::heads(m);
-    for (int n = first; n < m.ncl; n += stride) cl_one<T, FT>(m, H[n], px, py, pz, A);
+    for (int n = first; n < m.ncl; n += stride) cl_one<T, FT, CNT>(m, H[n], px, py, pz, A, C);
 }
 
 template <typename T, int FT>
 __device__ __forceinline__ void acc_clustered(const MolDev &m, T px, T py, T pz, T &ox, T &oy, T &oz)
 {
     ClAcc<T> A;
-    cl_sum<T, FT>(m, 0, 1, px, py, pz, A);
+    ClCount C;
+    cl_sum<T, FT>(m, 0, 1, px, py, pz, A, C);
     cl_epilogue<T, FT>(m, A, ox, oy, oz);
 }
 

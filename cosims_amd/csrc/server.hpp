@@ -350,7 +350,8 @@ int collect_slice(cosims_server *S, Slice &X)
     S->st.launches += X.launches;
     S->st.traj += P.n;
     S->st.kernel_ms += R.ms;
-    S->st.pair_evals += R.pairs;
+    S->st.pair_evals += R.pairs + R.cl[3] + R.cl[4] + R.cl[5];
+    S->st.ref_flops += R.flops;
     S->st.force_evals += R.evals;
     S->st.total_steps += R.steps;
     return 0;

@@ -54,9 +54,50 @@ struct TrajIO {
     double *chi, *cchi, *ergy;             // cchi: cos(chi) of accepted trajectories
     int32_t *steps, *totsteps;
     int32_t max_steps;
-    unsigned long long *evals;             // [0] steps, [1] force evaluations, [2] potential evaluations,
-                                           // [3] atom pairs of the exact force types' evaluations
+    unsigned long long *evals;             // [0] steps, [1] force evaluations, [2] atom pairs of the
+                                           // potential (getPotential) evaluations,
+                                           // [3] atom pairs of the exact force types' evaluations,
+                                           // [4..9] clustered-type branch counts (ClCount order)
 };
+constexpr int kEvalSlots = 10;             // counters of one launch in TrajIO::evals
+
+// Branch counts of one wavefront's clustered force evaluations (the 64-lane
+// clustered kernels), kept in LDS so that they hold no registers: row k
+// (ClCount order) has an entry per lane; rows 0-2 (clusters examined, far,
+// far with dipole terms) are counted wave-wide into lane 0's entry, rows
+// 3-5 (member atoms by branch) by each lane into its own.
+constexpr int kCountInts = 6 * 64;  // LDS ints per wavefront
+struct WaveCount {
+    unsigned *row;
+    __device__ void clear() const
+    {
+        const int lane = threadIdx.x & 63;
+#pragma unroll
+        for (int k = 0; k < 6; k++) row[k * 64 + lane] = 0;
+    }
+    __device__ void lane_add(int k, unsigned v) const { row[k * 64 + (threadIdx.x & 63)] += v; }
+    __device__ void wave_add(int k, unsigned v) const  // v wave-uniform
+    {
+        if ((threadIdx.x & 63) == 0) row[k * 64] += v;
+    }
+};
+
+// Adds this wavefront's six branch counts to evals[4..9] (all lanes call it).
+__device__ __forceinline__ void flush_counts(unsigned long long *evals, const WaveCount &W)
+{
+    unsigned long long v[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = W.row[k * 64 + (threadIdx.x & 63)];
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) v[k] += __shfl_xor(v[k], off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < 6; k++)
+            if (v[k]) atomicAdd(evals + 4 + k, v[k]);
+    }
+}
 
 struct GroupArgs {
     TrajIO io;
@@ -391,7 +432,8 @@ __device__ __forceinline__ void grp_clustered(const MolDev &m, int j, T px, T py
                                               double &oz)
 {
     ClAcc<T> A;
-    cl_sum<T, FT>(m, j, L, px, py, pz, A);
+    ClCount C;  // not kept: branch counts come from the 64-lane kernels only
+    cl_sum<T, FT>(m, j, L, px, py, pz, A, C);
     A.sumX = gsum<L>(A.sumX); A.sumY = gsum<L>(A.sumY); A.sumZ = gsum<L>(A.sumZ);
     if constexpr (FT != 5) {
         A.fX = gsum<L>(A.fX); A.fY = gsum<L>(A.fY); A.fZ = gsum<L>(A.fZ);
@@ -420,8 +462,9 @@ constexpr int kListInts = 2 * 64 * kChunk;  // LDS ints per wavefront
 
 template <typename T, int FT>
 __device__ __forceinline__ void wave_clustered(const MolDev &m, int ncl, int lane, T px, T py, T pz, int *lists,
-                                               double &ox, double &oy, double &oz)
+                                               double &ox, double &oy, double &oz, const WaveCount &W)
 {
+    unsigned n_hdr = 0, n_far = 0, n_fdip = 0;  // wave-uniform
     ClAcc<T> A;
     const auto H = Layout<T>::heads(m);
     int *ex_list = lists, *lj_list = lists + 64 * kChunk;
@@ -432,6 +475,7 @@ __device__ __forceinline__ void wave_clustered(const MolDev &m, int ncl, int lan
         for (int q = 0; q < kChunk; q++) {
             const int c = base + q * 64 + lane;
             int kex = 0, klj = 0;  // exact: 1 charges, 2 charges + LJ; LJ: 1 per-atom, 2 fixed parameters
+            bool fr = false, dp = false;
             if (c < ncl) {
                 const CAS ClusterT<T> &h = H[c];
                 const T dx0 = px - h.px, dx1 = py - h.py, dx2 = pz - h.pz;
@@ -439,6 +483,8 @@ __device__ __forceinline__ void wave_clustered(const MolDev &m, int ncl, int lan
                 if constexpr (FT == 1 || FT == 2) {
                     if (r2 > h.mpr) {
                         cl_far<T>(m, h, dx0, dx1, dx2, r2, A);
+                        fr = true;
+                        dp = m.order >= 1 && !h.rep;
                         if (FT == 2) klj = 1;
                         else if (!(r2 > h.dpr)) klj = 2;
                     } else {
@@ -455,6 +501,9 @@ __device__ __forceinline__ void wave_clustered(const MolDev &m, int ncl, int lan
             if (klj) lj_list[nlj + __popcll(ml & below)] = 4 * c + klj;
             nex += __popcll(me);
             nlj += __popcll(ml);
+            n_hdr += __popcll(__ballot(c < ncl));
+            n_far += __popcll(__ballot(fr));
+            n_fdip += __popcll(__ballot(dp));
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -463,17 +512,22 @@ __device__ __forceinline__ void wave_clustered(const MolDev &m, int ncl, int lan
             const int v = ex_list[e];
             const CAS ClusterT<T> &h = H[v >> 2];
             cl_exact_rt<T>(m, h.start, h.end, (v & 3) == 2, px, py, pz, A);
+            W.lane_add((v & 3) == 2 ? 3 : 4, (unsigned)(h.end - h.start));
         }
         for (int e = lane; e < nlj; e += 64) {
             const int v = lj_list[e];
             const CAS ClusterT<T> &h = H[v >> 2];
             cl_lj_rt<T>(m, h.start, h.end, (v & 3) == 2, h.qs6, h.qe24, px, py, pz, A);
+            W.lane_add(5, (unsigned)(h.end - h.start));
         }
         // the next chunk overwrites the lists: every lane is past its reads
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+    W.wave_add(0, n_hdr);
+    W.wave_add(1, n_far);
+    W.wave_add(2, n_fdip);
     A.sumX = gsum<64>(A.sumX); A.sumY = gsum<64>(A.sumY); A.sumZ = gsum<64>(A.sumZ);
     if constexpr (FT != 5) {
         A.fX = gsum<64>(A.fX); A.fY = gsum<64>(A.fY); A.fZ = gsum<64>(A.fZ);
@@ -614,12 +668,13 @@ __device__ __forceinline__ void cl32_members(const MolDev &m, int k0, int k1, f2
 // cluster and its members.  Entries are 4 c + kind.
 template <bool EXACT>
 __device__ __forceinline__ void cl32_flush(const MolDev &m, const int *ring, int head, int cnt, int lane, float px,
-                                           float py, float pz, float (&s)[16])
+                                           float py, float pz, float (&s)[16], const WaveCount &W)
 {
     if (lane >= cnt) return;
     const int v = ring[(head + lane) & (kRing - 1)];
     const int c = v >> 2, kind = v & 3;
     const i2n se = ((const GLB i2n *)glb(m.cse))[c];
+    W.lane_add(!EXACT ? 5 : (kind == 2 ? 3 : 4), (unsigned)(se.y - se.x));
     if constexpr (EXACT) {
         if (kind == 2) cl32_members<1>(m, se.x, se.y, f2n{0.0f, 0.0f}, px, py, pz, s);
         else cl32_members<0>(m, se.x, se.y, f2n{0.0f, 0.0f}, px, py, pz, s);
@@ -635,8 +690,10 @@ __device__ __forceinline__ void cl32_flush(const MolDev &m, const int *ring, int
 
 template <int FT>
 __device__ __forceinline__ void wave_clustered32(const MolDev &m, int ncl, int lane, float px, float py, float pz,
-                                                 int *lists, double &ox, double &oy, double &oz)
+                                                 int *lists, double &ox, double &oy, double &oz,
+                                                 const WaveCount &W)
 {
+    unsigned n_hdr = 0, n_far = 0, n_fdip = 0;  // wave-uniform
     constexpr bool kFar = FT == 1 || FT == 2;
     // s: LJ force 0-2, field 3-5, M xx yy zz xy xz yz 6-11, trace 12, antisymmetric xy xz yz 13-15
     float s[16];
@@ -665,6 +722,7 @@ __device__ __forceinline__ void wave_clustered32(const MolDev &m, int ncl, int l
             if (kFar) n1 = G1[cn];
         }
         int kex = 0, klj = 0;  // exact: 1 charges, 2 charges + LJ; LJ: 1 per-atom, 2 fixed parameters
+        bool fr = false;
         if (c < ncl) {
             const float dx = px - h.x, dy = py - h.y, dz = pz - h.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
@@ -673,6 +731,7 @@ __device__ __forceinline__ void wave_clustered32(const MolDev &m, int ncl, int l
             if constexpr (kFar) {
                 if (r2 > h.w) {
                     cl32_far(g, dx, dy, dz, r2, s);
+                    fr = true;
                     if (FT == 2) klj = 1;
                     else if (disp) klj = 2;
                 } else {
@@ -687,24 +746,31 @@ __device__ __forceinline__ void wave_clustered32(const MolDev &m, int ncl, int l
         const unsigned long long me = __ballot(kex != 0), ml = __ballot(klj != 0);
         if (kex) ex_ring[(ex_head + ex_n + __popcll(me & below)) & (kRing - 1)] = 4 * c + kex;
         if (klj) lj_ring[(lj_head + lj_n + __popcll(ml & below)) & (kRing - 1)] = 4 * c + klj;
+        // the dipole is zeroed for a centre-replaced cluster and at order 0
+        n_hdr += __popcll(__ballot(c < ncl));
+        n_far += __popcll(__ballot(fr));
+        n_fdip += __popcll(__ballot(fr && (g.y != 0.0f || g.z != 0.0f || g.w != 0.0f)));
         ex_n += __popcll(me);
         lj_n += __popcll(ml);
         wave_sync_lds();
         if (ex_n >= 64) {
-            cl32_flush<true>(m, ex_ring, ex_head, 64, lane, px, py, pz, s);
+            cl32_flush<true>(m, ex_ring, ex_head, 64, lane, px, py, pz, s, W);
             ex_head = (ex_head + 64) & (kRing - 1);
             ex_n -= 64;
         }
         if (lj_n >= 64) {
-            cl32_flush<false>(m, lj_ring, lj_head, 64, lane, px, py, pz, s);
+            cl32_flush<false>(m, lj_ring, lj_head, 64, lane, px, py, pz, s, W);
             lj_head = (lj_head + 64) & (kRing - 1);
             lj_n -= 64;
         }
         // the next slice's appends reuse the flushed slots: every lane is past its reads
         wave_sync_lds();
     }
-    if (ex_n) cl32_flush<true>(m, ex_ring, ex_head, ex_n, lane, px, py, pz, s);
-    if (lj_n) cl32_flush<false>(m, lj_ring, lj_head, lj_n, lane, px, py, pz, s);
+    if (ex_n) cl32_flush<true>(m, ex_ring, ex_head, ex_n, lane, px, py, pz, s, W);
+    if (lj_n) cl32_flush<false>(m, lj_ring, lj_head, lj_n, lane, px, py, pz, s, W);
+    W.wave_add(0, n_hdr);
+    W.wave_add(1, n_far);
+    W.wave_add(2, n_fdip);
     wave_sync_lds();
 #pragma unroll
     for (int k = 0; k < 16; k++) {
@@ -764,7 +830,9 @@ struct LdsSource {
     const GroupArgs &A;
     const char *smem;
     __device__ void bind(int) {}
+    __device__ void flush(const TrajIO &) const {}
     __device__ int pairs() const { return (FT == 4 || FT == 6) ? A.m.n : 0; }
+    __device__ int atoms() const { return A.m.n; }
     __device__ double mu() const { return A.m.mu; }
     __device__ double dt0() const { return A.dt0; }
     __device__ bool exits(double px, double py, double pz) const
@@ -796,11 +864,17 @@ struct TableSource {
     static constexpr bool kCoop = false;
     static constexpr bool kMulti = false;
     const MultiArgs &A;
-    int *lists;  // this wavefront's cluster lists in LDS (clustered types, L = 64)
+    int *lists;    // this wavefront's cluster lists in LDS (clustered types, L = 64)
+    WaveCount W;   // this wavefront's branch counts in LDS (clustered types, L = 64)
     int mi = 0;
     __device__ void bind(int t) { mi = A.tmol ? A.tmol[t] : 0; }
+    __device__ void flush(const TrajIO &io) const
+    {
+        if constexpr (FT != 4 && FT != 6 && L == 64) flush_counts(io.evals, W);
+    }
     __device__ const MolRec &rec() const { return A.mols[mi]; }
     __device__ int pairs() const { return (FT == 4 || FT == 6) ? rec().m.n : 0; }
+    __device__ int atoms() const { return rec().m.n; }
     __device__ double mu() const { return rec().m.mu; }
     __device__ double dt0() const { return rec().dt0; }
     __device__ bool exits(double px, double py, double pz) const
@@ -822,11 +896,11 @@ struct TableSource {
             const int ncl = active ? r.m.ncl : 0;
             if constexpr (PREC == 1) {
                 if (r.m.order <= 1)
-                    wave_clustered32<FT>(r.m, ncl, j, (float)ex, (float)ey, (float)ez, lists, fx, fy, fz);
+                    wave_clustered32<FT>(r.m, ncl, j, (float)ex, (float)ey, (float)ez, lists, fx, fy, fz, W);
                 else
-                    wave_clustered<float, FT>(r.m, ncl, j, (float)ex, (float)ey, (float)ez, lists, fx, fy, fz);
+                    wave_clustered<float, FT>(r.m, ncl, j, (float)ex, (float)ey, (float)ez, lists, fx, fy, fz, W);
             } else
-                wave_clustered<double, FT>(r.m, ncl, j, ex, ey, ez, lists, fx, fy, fz);
+                wave_clustered<double, FT>(r.m, ncl, j, ex, ey, ez, lists, fx, fy, fz, W);
         } else {
             MolDev m = r.m;
             if (!active) m.ncl = 0;
@@ -858,7 +932,9 @@ struct TileSource {
     const GroupArgs &A;
     char *smem;
     __device__ void bind(int) {}
+    __device__ void flush(const TrajIO &) const {}
     __device__ int pairs() const { return A.m.n; }
+    __device__ int atoms() const { return A.m.n; }
     __device__ double mu() const { return A.m.mu; }
     __device__ double dt0() const { return A.dt0; }
     __device__ bool exits(double px, double py, double pz) const
@@ -968,7 +1044,9 @@ struct LdsMultiSource {
     int gen = 0;
     bool counted = false;
     __device__ void bind(int) {}
+    __device__ void flush(const TrajIO &) const {}
     __device__ int pairs() const { return rec->m.n; }
+    __device__ int atoms() const { return rec->m.n; }
     __device__ double mu() const { return rec->m.mu; }
     __device__ double dt0() const { return rec->dt0; }
     __device__ bool exits(double px, double py, double pz) const
@@ -1143,8 +1221,8 @@ __device__ __forceinline__ void group_loop(TrajIO A, Src &S)
     double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0, ax = 0, ay = 0, az = 0, dt = 0, rdot = 0;
     double e0 = 0, chi = 0;
     bool exhausted = false, prio = false;
-    unsigned n_steps = 0, n_evals = 0, n_pots = 0;
-    unsigned long long n_pairs = 0;
+    unsigned n_steps = 0, n_evals = 0;
+    unsigned long long n_pairs = 0, n_pots = 0;
 
     for (;;) {
         if (!exhausted) {
@@ -1225,7 +1303,7 @@ __device__ __forceinline__ void group_loop(TrajIO A, Src &S)
         if (j == 0) {
             n_evals += needF;
             n_steps += (phase == PH_STEP);
-            n_pots += needU;
+            if (needU) n_pots += (unsigned)S.atoms();
             if (needF) n_pairs += (unsigned)S.pairs();
         }
 
@@ -1323,6 +1401,7 @@ __device__ __forceinline__ void group_loop(TrajIO A, Src &S)
         atomicAdd(A.evals + 2, p);
         atomicAdd(A.evals + 3, q);
     }
+    S.flush(A);
 }
 
 // The molecule of A into LDS for LdsSource: FP32 x, y, z, q | A, B, or the
@@ -1433,18 +1512,23 @@ __global__ void __launch_bounds__(1024) k_traj_tile(GroupArgs A)
     group_loop<L>(io, src);
 }
 
-// Dynamic LDS of k_traj_multi: the cluster lists of each wavefront when
-// clustered types run one group per wavefront, else none.
+// Dynamic LDS of k_traj_multi: the cluster lists of each wavefront and then
+// every wavefront's branch counts when clustered types run one group per
+// wavefront, else none.
 __host__ __device__ constexpr bool multi_uses_lists(int ft, int L) { return ft != 4 && ft != 6 && L == 64; }
 
 template <int PREC, int FT, int L>
 __global__ void __launch_bounds__(1024) k_traj_multi(MultiArgs A)
 {
     extern __shared__ int lds_lists[];
-    int *lists = multi_uses_lists(FT, L) ? lds_lists + (threadIdx.x >> 6) * kListInts : nullptr;
+    constexpr bool kLists = multi_uses_lists(FT, L);
+    int *lists = kLists ? lds_lists + (threadIdx.x >> 6) * kListInts : nullptr;
+    WaveCount W{kLists ? (unsigned *)(lds_lists + (blockDim.x >> 6) * kListInts) + (threadIdx.x >> 6) * kCountInts
+                       : nullptr};
+    if (kLists) W.clear();
     TrajIO io = A.io;
     resolve_range(io, A.first, A.last);
-    TableSource<PREC, FT, L> src{A, lists};
+    TableSource<PREC, FT, L> src{A, lists, W};
     group_loop<L>(io, src);
 }
 

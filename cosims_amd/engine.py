@@ -99,7 +99,8 @@ class Ccs(C.Structure):
                [("run_ms", C.c_double), ("n_rounds", C.c_int32),
                 ("traj_kernel_ms", C.c_double * MAX_ROUNDS), ("round_steps", C.c_int64 * MAX_ROUNDS),
                 ("round_evals", C.c_int64 * MAX_ROUNDS), ("round_items", C.c_int64 * MAX_ROUNDS), ("check_kernel_ms", C.c_double),
-                ("sample_kernel_ms", C.c_double), ("lanes", C.c_int32), ("wg_size", C.c_int32)]
+                ("sample_kernel_ms", C.c_double), ("lanes", C.c_int32), ("wg_size", C.c_int32),
+                ("cl_counts", C.c_int64 * 6), ("ref_flops", C.c_double)]
 
 
 class BatchJob(C.Structure):
@@ -121,7 +122,7 @@ class BatchStats(C.Structure):
                 ("class_lanes", C.c_int32 * SIZE_CLASSES), ("class_wg", C.c_int32 * SIZE_CLASSES),
                 ("kernel_ms", C.c_double), ("sample_ms", C.c_double), ("run_ms", C.c_double)] + \
                [(k, C.c_int64) for k in ("n_traj", "n_hit", "total_steps", "force_evals", "pot_evals",
-                                          "pair_evals")]
+                                          "pair_evals")] + [("ref_flops", C.c_double)]
 
 
 class ServerCfg(C.Structure):
@@ -148,7 +149,7 @@ class ServerStats(C.Structure):
     _fields_ = [(k, C.c_int64) for k in ("submitted", "done", "failed", "queued", "active", "slices", "launches",
                                           "traj", "pair_evals", "force_evals", "total_steps")] + \
                [(k, C.c_double) for k in ("kernel_ms", "busy_ms", "now_ms")] + \
-               [(k, C.c_int64) for k in ("bytes_in_use", "bytes_peak", "budget")]
+               [(k, C.c_int64) for k in ("bytes_in_use", "bytes_peak", "budget")] + [("ref_flops", C.c_double)]
 
 
 def lib():
@@ -419,6 +420,7 @@ class Molecule:
         out = {k: getattr(res, k) for k, _ in Ccs._fields_}
         for k in ("traj_kernel_ms", "round_steps", "round_evals", "round_items"):
             out[k] = list(out[k])[: res.n_rounds]
+        out["cl_counts"] = list(res.cl_counts)
         out.update(omega=om, omega_err=err)
         if keep:
             out.update(chi=chi, status=status, g=g)
@@ -477,7 +479,7 @@ def run_batch(jobs, prec=PREC_MIXED, lanes=0, wg_size=0, blocks_per_cu=0, class_
     out = []
     for om, err, res in keep:
         d = {k: getattr(res, k) for k, _ in Ccs._fields_ if k not in ("traj_kernel_ms", "round_steps",
-                                                                     "round_evals", "round_items")}
+                                                                     "round_evals", "round_items", "cl_counts")}
         d.update(omega=om, omega_err=err)
         out.append(d)
     stats = {k: getattr(st, k) for k, _ in BatchStats._fields_}

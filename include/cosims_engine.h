@@ -226,6 +226,24 @@ typedef struct cosims_ccs {
     int64_t round_items[COSIMS_MAX_ROUNDS];   /* trajectories queued for each launch */
     double check_kernel_ms, sample_kernel_ms;
     int32_t lanes, wg_size;                   /* launch shape used */
+    /* Clustered force types (group kernels), over every force evaluation:
+     * clusters examined, far clusters (multipole terms), far clusters with
+     * dipole terms, member atoms with charges and LJ, with charges only,
+     * with LJ only. */
+    int64_t cl_counts[6];
+    /* Reference FLOPs of the trajectory launches, counted per branch as the
+     * reference writes them: type 4 58 a pair + 21 an evaluation
+     * (src/forceAccel.cpp:764-802), type 6 22 a pair (:865-880); type 1/2
+     * 12 a near cluster's header, 116 a far cluster with dipole terms (49
+     * without, +89 at order 2), 62 an atom with charges and LJ, 48 with
+     * charges only, 22 with LJ only, 27 an evaluation (:244-442); type 3 10
+     * a header, 62 / 48 an atom, 21 an evaluation (:655-746); type 5 10 a
+     * header, 22 an atom (:819-851); plus 50 an integration step
+     * (src/diffEqSolver.cpp:131-174) and 24 (charged) or 15 an atom of
+     * getPotential (:954-972).  The lane-per-trajectory kernel counts
+     * neither potential pairs nor branches: its figure leaves out the
+     * potential, and is 0 for the clustered types. */
+    double ref_flops;
 } cosims_ccs;
 /* omega/omega_err: iter entries each. */
 int cosims_run(const cosims_mol *m, const cosims_params *p, const cosims_run_cfg *cfg, double *omega,
@@ -284,7 +302,9 @@ typedef struct cosims_batch_stats {
     double kernel_ms;         /* first launch to last launch end (HIP events) */
     double sample_ms, run_ms; /* sampling launches; whole call */
     int64_t n_traj, n_hit, total_steps, force_evals, pot_evals;
-    int64_t pair_evals;       /* atom pairs of the exact force types' force evaluations */
+    int64_t pair_evals;       /* atom pairs of the force evaluations (exact types: every atom;
+                                 clustered types: the member atoms their branches evaluate) */
+    double ref_flops;         /* reference FLOPs of all launches (as cosims_ccs.ref_flops) */
 } cosims_batch_stats;
 
 int cosims_run_batch(cosims_batch_job *jobs, int32_t njobs, const cosims_batch_cfg *cfg,
@@ -365,6 +385,7 @@ typedef struct cosims_server_stats {
     int64_t slices, launches, traj, pair_evals, force_evals, total_steps;
     double kernel_ms, busy_ms, now_ms;
     int64_t bytes_in_use, bytes_peak, budget;
+    double ref_flops;         /* reference FLOPs of every launch (as cosims_ccs.ref_flops) */
 } cosims_server_stats;
 
 int cosims_server_create(const cosims_server_cfg *cfg, cosims_server **out);

@@ -270,3 +270,45 @@ def test_batch_shared_lds_restaging(ion128, ion2000, prec):
         single = m.run(prec, job=spec["job"], iters=1, traj=spec["traj"], **shape)
         assert np.array_equal(shared[k]["omega"], single["omega"]), (prec, k)
         assert shared[k]["total_steps"] == single["total_steps"], (prec, k)
+
+
+def test_clustered_branch_counts():
+    """The 64-lane clustered kernels count their branches: every force
+    evaluation examines every cluster (clusters examined = clusters x
+    evaluations exactly); far clusters and those with dipole terms are
+    subsets; with the default radii (R_d >= R_m) no atom takes the
+    charges-only branch; mixed precision and FP64 take the same branches on
+    the same initial conditions except on the few chaotic trajectories; the
+    reference FLOPs follow the per-branch table."""
+    g = golden("acau_charged")
+    m = golden_engine_mol(g, [1, 1, 1, 1])
+    inf = m.info()
+    assert inf["force_type"] == 1 and inf["ncl"] > 0
+    kw = dict(job=5, iters=1, traj=4000, lanes=64, wg_size=256)
+    runs = {p: m.run(p, **kw) for p in (E.PREC_MIXED, E.PREC_FP64)}
+    for p, r in runs.items():
+        hdr, far, fdip, exlj, exq, lj = r["cl_counts"]
+        evals = sum(r["round_evals"])
+        assert hdr == inf["ncl"] * evals, (p, hdr, evals)
+        assert 0 < fdip <= far < hdr
+        assert exq == 0 and exlj > 0 and lj > 0
+        assert r["pair_evals"] == exlj + exq + lj
+        steps = sum(r["round_steps"])
+        want = (12 * (hdr - far) + 49 * far + 67 * fdip + 62 * exlj + 48 * exq + 22 * lj + 27 * evals
+                + 50 * steps + 24 * r["pot_evals"])
+        assert abs(r["ref_flops"] - want) <= 1e-9 * want
+    a, b = runs[E.PREC_MIXED]["cl_counts"], runs[E.PREC_FP64]["cl_counts"]
+    for k in range(6):
+        assert abs(a[k] - b[k]) <= 0.01 * max(1, b[k]), (k, a, b)
+
+
+def test_exact_type_reference_flops(ion2000):
+    """Exact type 4: 58 FLOP a pair + 21 an evaluation + 50 a step + 24 a
+    potential atom pair; pairs = atoms x evaluations."""
+    r = ion2000.run(E.PREC_MIXED, job=3, iters=1, traj=3000, lanes=16, wg_size=256)
+    evals, steps = sum(r["round_evals"]), sum(r["round_steps"])
+    assert r["pair_evals"] == 2000 * evals
+    assert r["pot_evals"] % 2000 == 0 and r["pot_evals"] >= 2000 * r["n_hit"]
+    want = 58 * r["pair_evals"] + 21 * evals + 50 * steps + 24 * r["pot_evals"]
+    assert abs(r["ref_flops"] - want) <= 1e-9 * want
+    assert r["cl_counts"] == [0] * 6
