@@ -97,9 +97,11 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a, dtype=float), q)) if len(a) else None
 
 
-def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, fp64_max_atoms=0, **server_kw):
+def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, fp64_max_atoms=0, fp64_max_requests=40,
+            **server_kw):
     """Poisson request stream through the CCS server; latency = completion -
-    submission on the server's clock (setup, queueing and trajectories)."""
+    submission on the server's clock (setup, queueing and trajectories).
+    The arrival times are replayed in real time."""
     import tempfile
     reqs = synth.serving_stream(count=n_requests, rate=rate, seed=seed)
     d = tempfile.mkdtemp(prefix="cosims_serve_")
@@ -150,7 +152,7 @@ def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, fp64_max_at
         # iter = blocks, the request's lanes) in FP64, for the requests of up to fp64_max_atoms atoms
         dev = []
         for k, r in enumerate(res):
-            if reqs[k]["n"] > fp64_max_atoms:
+            if reqs[k]["n"] > fp64_max_atoms or len(dev) >= fp64_max_requests:
                 continue
             m = E.Molecule.from_file(os.path.join(d, f"req{k}.mfj"), params=params(k))
             f = m.run(E.PREC_FP64, job=k, iters=r["blocks"], traj=10000, lanes=r["lanes"], wg_size=W)
@@ -164,11 +166,17 @@ def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, fp64_max_at
     lat = [r["t_done_ms"] - r["t_submit_ms"] for r in res]
     makespan = (max(r["t_done_ms"] for r in res) - min(r["t_submit_ms"] for r in res)) / 1e3
     traj = sum(r["n_traj"] for r in res)
+    # latency against arrival order: p50 and p99 of each fifth of the stream (a backlog shows as a
+    # rise from fifth to fifth)
+    fifths = np.array_split(np.arange(len(lat)), 5)
+    by_arrival = [{"requests": f"{int(f[0])}-{int(f[-1])}", "p50": pct([lat[i] for i in f], 50),
+                   "p99": pct([lat[i] for i in f], 99)} for f in fifths if len(f)]
     return {
         "workload": f"Poisson {rate} req/s, {n_requests} requests, N log-uniform 100-20,000, "
                     f"q round(N/800)+1 e, T 200-400 K, to {100 * target:.1f}% relative SE",
         "requests": n_requests, "offered_rate": rate, "served_rate": n_requests / makespan,
         "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "mean": float(np.mean(lat)), "max": max(lat)},
+        "latency_by_arrival_fifth_ms": by_arrival,
         "setup_ms_p50": pct([r["setup_ms"] for r in res], 50),
         "queue_ms_p50": pct([r["t_admit_ms"] - r["t_ready_ms"] for r in res], 50),
         "trajectories_per_s": traj / makespan, "trajectories": traj, "makespan_s": makespan,
@@ -177,12 +185,13 @@ def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, fp64_max_at
         "slices": st["slices"], "kernel_s": st["kernel_ms"] / 1e3, "busy_s": st["busy_ms"] / 1e3,
         # reference FLOPs of every trajectory issued (blocks past a stopping point included) over the
         # time some slice was in flight
-        "achieved_tflops_busy": (st["pair_evals"] * FLOP_PAIR + st["force_evals"] * FLOP_EPI
-                                 + st["total_steps"] * FLOP_VERLET) / (st["busy_ms"] * 1e-3) / 1e12,
+        "achieved_tflops_busy": st["ref_flops"] / (st["busy_ms"] * 1e-3) / 1e12,
         "lanes_per_trajectory": "by size class", "class_lanes": server_kw.get("class_lanes"), "wg_size": W,
         "class_wg": server_kw["class_wg"],
         "prior_rel_sd": server_kw["prior_rel_sd"], "issue_fraction": server_kw["issue_fraction"],
         "policy": "shortest remaining work first" if policy == E.SERVE_SHORTEST_FIRST else "equal share",
+        "slice_launches": "a launch of its own per request" if server_kw.get("own_launches")
+                          else "shared, molecules staged in LDS",
         "ccs_vs_fp64": vs64,
     }
 
@@ -374,8 +383,13 @@ def main():
                     help="lanes per trajectory of the mixed run (0: the autotuner's; fixes the shape for profiling)")
     ap.add_argument("--wg", type=int, default=0, help="workgroup size of the mixed run (0: the autotuner's)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--serve-requests", type=int, default=40, help="requests of the serving stream (0: none)")
+    ap.add_argument("--serve-requests", type=int, default=200, help="requests of the serving stream (0: none)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-sustained-rate", type=float, default=2.5,
+                    help="offered rate of a second stream the GPU can sustain (0: none)")
+    ap.add_argument("--serve-sustained-requests", type=int, default=100)
+    ap.add_argument("--serve-own-launches", type=int, default=0,
+                    help="serving slices with a launch of their own per request (1) or shared LDS launches (0)")
     ap.add_argument("--serve-target", type=float, default=0.005)
     ap.add_argument("--serve-fp64-atoms", type=int, default=1000,
                     help="rerun served requests of up to this many atoms in FP64 over the same blocks (0: none)")
@@ -384,7 +398,7 @@ def main():
     ap.add_argument("--serve-class-lanes", type=str, default="8,16,32,64,64",
                     help="lanes per trajectory per size class of the serving stream (0: the size rule, "
                          "4 x 2^class)")
-    ap.add_argument("--serve-compare", type=int, default=1,
+    ap.add_argument("--serve-compare", type=int, default=0,
                     help="also run the stream with equal-share slices (1) or not (0)")
     ap.add_argument("--workload", choices=("headline", "batch", "clustered"), default="headline",
                     help="headline: the 2,000-atom ion + serving stream (the JSON contract line); batch: 256 uneven "
@@ -525,13 +539,24 @@ def main():
     if a.serve_requests > 0:
         cwg = [int(x) for x in a.serve_class_wg.split(",")]
         cl = [int(x) for x in a.serve_class_lanes.split(",")]
+        own = bool(a.serve_own_launches)
         sv = serving(E, synth, a.serve_requests, a.serve_rate, a.serve_target, fp64_max_atoms=a.serve_fp64_atoms,
-                     class_wg=cwg, class_lanes=cl)
+                     class_wg=cwg, class_lanes=cl, own_launches=own)
         out["serving"] = sv
-        out["served_latency_ms"] = {"p50": sv["latency_ms"]["p50"], "p99": sv["latency_ms"]["p99"]}
+        # the config's stream (20 req/s, several times what one GPU serves: its percentiles measure
+        # the backlog) and a stream at a rate the GPU sustains
+        out["served_latency_ms"] = {"p50": sv["latency_ms"]["p50"], "p99": sv["latency_ms"]["p99"],
+                                    "offered_rate": a.serve_rate, "requests": a.serve_requests}
+        if a.serve_sustained_rate > 0 and a.serve_sustained_requests > 0:
+            su = serving(E, synth, a.serve_sustained_requests, a.serve_sustained_rate, a.serve_target, seed=6000,
+                         class_wg=cwg, class_lanes=cl, own_launches=own)
+            out["serving_sustained"] = su
+            out["served_latency_ms"]["sustained"] = {"p50": su["latency_ms"]["p50"], "p99": su["latency_ms"]["p99"],
+                                                     "offered_rate": a.serve_sustained_rate,
+                                                     "requests": a.serve_sustained_requests}
         if a.serve_compare:
             eq = serving(E, synth, a.serve_requests, a.serve_rate, a.serve_target, policy=E.SERVE_EQUAL_SHARE,
-                         class_wg=cwg, class_lanes=cl)
+                         class_wg=cwg, class_lanes=cl, own_launches=own)
             out["serving_equal_share"] = {k: eq[k] for k in ("latency_ms", "served_rate", "trajectories_per_s",
                                                              "makespan_s", "slices", "trajectories_issued")}
     print(json.dumps(out), flush=True)

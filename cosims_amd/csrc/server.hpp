@@ -5,14 +5,17 @@
 //   submit -> setup worker (molecule setup, boundary marches on the GPU,
 //   device upload) -> ready -> admitted (slot and device-memory budget) ->
 //   active -> done.
-// The scheduler thread runs slices: one persistent table-kernel launch per
-// force type over blocks of trajectories of every active request (a block
-// is one CCS integral, Philox iteration = block index).  Requests that
-// arrive while a slice runs join the next slice.  After each slice every
-// request's new blocks are taken in order and the stopping rule is applied
-// after each block, so a request's result depends only on its own blocks:
-// it is the result cosims_run gives over the same blocks with the server's
-// launch shape, whatever else shared the launches.
+// The scheduler thread runs slices: persistent launches over blocks of
+// trajectories of every active request (a block is one CCS integral, Philox
+// iteration = block index), by default one per launch shape with each
+// workgroup staging the molecule of the request it works on.  Requests that
+// arrive while a slice runs join the next slice.  As a slice completes, each
+// block's CCS is computed on the device (k_block_ccs) and the stopping rule
+// is applied on the device block by block in block order (k_stop); the host
+// reads back only the blocks taken and the retirement flags.  A request's
+// result depends only on its own blocks: it is the result cosims_run gives
+// over the same blocks with the server's launch shape, whatever else shared
+// the launches.
 #pragma once
 
 #include <condition_variable>
@@ -23,6 +26,134 @@
 namespace {
 
 constexpr int64_t kTrajBytes = 150;  // device bytes per in-flight trajectory (TrajBufs + work list)
+
+// ------------------------------------------------ per-block CCS on the device
+// One CCS integral of a served request (a block): its trajectory statuses in
+// the slice's buffers, its sums from k_reduce, and where its CCS and standard
+// error go (src/monteTraj.cpp:378-386).
+struct BlockDesc {
+    const int8_t *st;
+    const double *sums;   // kSums
+    double *out;          // omega, omega_err
+    int32_t T, upd;       // trajectories; progress-update period (the denominator's reset)
+};
+
+// The denominator of one thread's integral (src/monteTraj.cpp:456-521):
+// T minus the trajectories since the last progress update whose success flag
+// is off, a miss repeating the flag of the trajectory before it.  The last
+// window [w0, T) is scanned in parallel: each thread takes a chunk, finds
+// its last non-miss status, and with the last non-miss status before its
+// chunk (an exclusive scan over the chunks, or before w0) counts the
+// trajectories of its chunk that see the flag off.  One workgroup a block.
+__global__ void __launch_bounds__(256) k_block_ccs(const BlockDesc *D)
+{
+    __shared__ int8_t last[256];
+    __shared__ int8_t carry;
+    __shared__ int orbit;
+    const BlockDesc d = D[blockIdx.x];
+    const int T = d.T, upd = d.upd, tid = threadIdx.x;
+    const int w0 = (T / upd) * upd;  // orbit counts reset at every multiple of upd
+    const int len = T - w0;
+    if (tid == 0) {
+        int8_t c = -1;  // no non-miss yet: the flag is off
+        for (int j = w0 - 1; j >= 0; j--)
+            if (d.st[j] != ST_MISS) {
+                c = d.st[j];
+                break;
+            }
+        carry = c;
+        orbit = 0;
+    }
+    const int chunk = (len + 255) / 256;
+    const int a = min(len, tid * chunk), b = min(len, a + chunk);
+    int8_t mine = -1;
+    for (int i = a; i < b; i++)
+        if (d.st[w0 + i] != ST_MISS) mine = d.st[w0 + i];
+    last[tid] = mine;
+    __syncthreads();
+    int8_t in = carry;  // last non-miss status before this chunk
+    for (int t = tid - 1; t >= 0; t--)
+        if (last[t] >= 0) {
+            in = last[t];
+            break;
+        }
+    bool succ = in == ST_OK;
+    int off = 0;
+    for (int i = a; i < b; i++) {
+        const int8_t s = d.st[w0 + i];
+        if (s != ST_MISS) succ = s == ST_OK;
+        off += succ ? 0 : 1;
+    }
+    if (off) atomicAdd(&orbit, off);
+    __syncthreads();
+    if (tid == 0) {
+        const double den = (double)(T - orbit);
+        const double tot = d.sums[0], tot2 = d.sums[1];
+        d.out[0] = tot / den;
+        d.out[1] = sqrt((tot2 / den - (tot / den) * (tot / den)) / den);
+    }
+}
+
+// Running state of a served request on the device: blocks taken into its
+// result, whether it has met the stopping rule, and the inverse-variance
+// sums of src/monteTraj.cpp:412-425 over its blocks in block order.
+struct ReqDev {
+    int32_t used, finished, min_blocks, max_blocks;
+    double target, we, wm;
+};
+
+// The stopping rule of one request over the blocks of a collected slice
+// (`upto`: one past its last block there; earlier blocks were collected
+// before): blocks are taken in order from `used`, Sum 1/err^2 and
+// Sum omega/err^2 accumulated as final_stats accumulates them, and the
+// request stops at the first block from min_blocks on where werr / wmean <=
+// target, or at max_blocks.  Each taken block's CCS, error and sums are
+// copied to the slice's staging record for the host, after the integration
+// steps of all the request's blocks in the slice.
+struct StopDesc {
+    ReqDev *state;
+    const double *blk;    // omega, omega_err per block
+    const double *sums;   // kSums per block
+    int32_t it0, upto;    // the slice's blocks of this request: [it0, upto)
+    double *stage;        // steps of the slice's blocks, then (2 + kSums) doubles per slice block
+    int32_t *head;        // used before, used after, finished
+};
+
+__global__ void __launch_bounds__(64) k_stop(const StopDesc *D, int n)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const StopDesc d = D[e];
+    double steps = 0;
+    for (int k = d.it0; k < d.upto; k++) steps += d.sums[kSums * k + 3];
+    d.stage[0] = steps;
+    ReqDev s = *d.state;
+    const int u0 = s.used;
+    if (!s.finished && s.used != d.it0) {  // an earlier block of this request was never collected
+        d.head[0] = u0;
+        d.head[1] = u0;
+        d.head[2] = -1;
+        return;
+    }
+    for (int k = s.used; k < d.upto && !s.finished; k++) {
+        const double om = d.blk[2 * k], er = d.blk[2 * k + 1];
+        s.we += 1.0 / (er * er);
+        s.wm += om / (er * er);
+        s.used = k + 1;
+        double *o = d.stage + 1 + (size_t)(2 + kSums) * (k - d.it0);
+        o[0] = om;
+        o[1] = er;
+        for (int q = 0; q < kSums; q++) o[2 + q] = d.sums[kSums * k + q];
+        if (s.used >= s.min_blocks) {
+            const double werr = 1.0 / sqrt(s.we), wmean = s.wm / s.we;
+            if (werr / wmean <= s.target || s.used >= s.max_blocks) s.finished = 1;
+        }
+    }
+    *d.state = s;
+    d.head[0] = u0;
+    d.head[1] = s.used;
+    d.head[2] = s.finished;
+}
 
 struct SJob {
     int64_t ticket = 0;
@@ -41,6 +172,8 @@ struct SJob {
     std::vector<double> omega, omega_err, sums;  // per block
     bool finished = false;
     DBuf dsums;                         // per-block sums on the device (max_blocks x kSums)
+    DBuf dblk, dstate;                  // per-block CCS and error; the stopping rule's state (ReqDev)
+    ReqDev init{};                      // host copy of the initial state (its upload is asynchronous)
     size_t bytes = 0;                   // charged against the budget from admission ...
     int inflight = 0;                   // ... until it has retired and no slice in flight holds it
     bool released = false;
@@ -239,15 +372,36 @@ std::vector<SliceEntry> plan_slice(cosims_server *S, const std::vector<std::shar
     return out;
 }
 
+// Per pipeline place: the descriptors of a slice's per-block CCS and
+// stopping-rule kernels and the staging record the host reads back.
+struct SliceDev {
+    DBuf bd, sd, stage, head;
+    size_t bd_cap = 0, sd_cap = 0, stage_cap = 0, head_cap = 0;
+    std::vector<BlockDesc> bdh;
+    std::vector<StopDesc> sdh;
+    std::vector<double> stage_h;
+    std::vector<int32_t> head_h;
+};
+
 // A slice in flight: its plan, device buffers and stream.
 struct Slice {
     std::vector<SliceEntry> e;
     BatchPlan *P = nullptr;
+    SliceDev *D = nullptr;
     hipStream_t s = nullptr;
     int launches = 0;
     double t_issue = 0;
     bool live() const { return !e.empty(); }
 };
+
+template <class T>
+int reserve_dev(DBuf &b, size_t &cap, size_t n)
+{
+    if (cap >= n) return 0;
+    HIP_CHECK(b.alloc(sizeof(T) * n));
+    cap = n;
+    return 0;
+}
 
 // Issues one slice (sampling, launches, per-block reduction into each
 // request's device sums) without waiting for the launches.
@@ -260,8 +414,14 @@ int issue_slice(cosims_server *S, Slice &X)
     for (SliceEntry &e : X.e) {
         SJob &J = *e.job;
         if (e.it0 + e.nb > c.max_blocks) return fail(COSIMS_E_INVALID, "server: block count beyond max_blocks");
-        if (!J.dsums.p && J.dsums.alloc(sizeof(double) * kSums * c.max_blocks) != hipSuccess)
-            return fail(COSIMS_E_NOMEM, "server: per-request sums");
+        if (!J.dsums.p) {
+            if (J.dsums.alloc(sizeof(double) * kSums * c.max_blocks) != hipSuccess ||
+                J.dblk.alloc(sizeof(double) * 2 * c.max_blocks) != hipSuccess ||
+                J.dstate.alloc(sizeof(ReqDev)) != hipSuccess)
+                return fail(COSIMS_E_NOMEM, "server: per-request device state");
+            J.init = ReqDev{0, 0, c.min_blocks, c.max_blocks, J.target, 0.0, 0.0};
+            HIP_CHECK(hipMemcpyAsync(J.dstate.p, &J.init, sizeof(ReqDev), hipMemcpyHostToDevice, X.s));
+        }
         BatchPlan::Job b{};
         b.mol = J.mol;
         b.seed = J.params.seed;
@@ -282,6 +442,7 @@ int issue_slice(cosims_server *S, Slice &X)
         J.inflight++;
     }
     if (P.n > INT32_MAX / 4) return fail(COSIMS_E_INVALID, "server: slice too large");
+    P.shared = c.own_launches == 0;
     if (int rc = order_plan(P)) return rc;
     if (int rc = stage_batch(P, X.s)) return rc;
     if (int rc = launch_enqueue(P, c.prec, 0, X.s, X.launches)) return rc;
@@ -296,53 +457,84 @@ int issue_slice(cosims_server *S, Slice &X)
         hipLaunchKernelGGL(k_reduce, dim3(J.iters), dim3(256), 0, X.s, RA);
         HIP_CHECK(hipGetLastError());
     }
+    // each block's CCS and standard error on the device (the host reads no statuses)
+    SliceDev &D = *X.D;
+    D.bdh.clear();
+    for (size_t i = 0; i < X.e.size(); i++) {
+        const auto &J = P.jobs[i];
+        SJob &R = *X.e[i].job;
+        const int64_t pr = J.mol->params.print_rate;
+        const int32_t upd = (int32_t)(pr > 0 ? pr : std::max<int64_t>(1, (int64_t)(J.T / 5.0)));
+        for (int b = 0; b < J.iters; b++)
+            D.bdh.push_back({P.B.status.as<int8_t>() + J.off + (int64_t)b * J.T,
+                             R.dsums.as<double>() + kSums * (X.e[i].it0 + b), R.dblk.as<double>() + 2 * (X.e[i].it0 + b),
+                             (int32_t)J.T, upd});
+    }
+    if (int rc = reserve_dev<BlockDesc>(D.bd, D.bd_cap, D.bdh.size())) return rc;
+    HIP_CHECK(hipMemcpyAsync(D.bd.p, D.bdh.data(), sizeof(BlockDesc) * D.bdh.size(), hipMemcpyHostToDevice, X.s));
+    hipLaunchKernelGGL(k_block_ccs, dim3((unsigned)D.bdh.size()), dim3(256), 0, X.s, D.bd.as<BlockDesc>());
+    HIP_CHECK(hipGetLastError());
     return 0;
 }
 
-// Waits for a slice and applies the stopping rule block by block in block
-// order to every request in it (blocks of finished requests are ignored).
+// Waits for a slice, then applies the stopping rule on the device (k_stop:
+// every request of the slice, its blocks in block order; blocks of finished
+// requests are ignored) and reads back one staging record: the blocks each
+// request took, their CCS, errors and sums, and whether it has stopped.
 int collect_slice(cosims_server *S, Slice &X)
 {
     const auto &c = S->cfg;
     BatchPlan &P = *X.P;
+    SliceDev &D = *X.D;
     GroupRun R;
     if (int rc = launch_finish(P, X.s, R)) return rc;
-    // copies on the slice's own stream: the next slice keeps running meanwhile
-    std::vector<int8_t> status(P.n);
-    std::vector<std::vector<double>> all(X.e.size());
-    HIP_CHECK(hipMemcpyAsync(status.data(), P.B.status.p, P.n, hipMemcpyDeviceToHost, X.s));
-    for (size_t i = 0; i < X.e.size(); i++) {
-        all[i].resize((size_t)kSums * X.e[i].nb);
-        HIP_CHECK(hipMemcpyAsync(all[i].data(), X.e[i].job->dsums.as<double>() + kSums * X.e[i].it0,
-                                 sizeof(double) * all[i].size(), hipMemcpyDeviceToHost, X.s));
+    const size_t ne = X.e.size();
+    D.sdh.resize(ne);
+    size_t nd = 0;  // staging doubles
+    for (size_t i = 0; i < ne; i++) nd += 1 + (size_t)(2 + kSums) * X.e[i].nb;
+    if (int rc = reserve_dev<double>(D.stage, D.stage_cap, std::max<size_t>(1, nd))) return rc;
+    if (int rc = reserve_dev<int32_t>(D.head, D.head_cap, 3 * std::max<size_t>(1, ne))) return rc;
+    if (int rc = reserve_dev<StopDesc>(D.sd, D.sd_cap, std::max<size_t>(1, ne))) return rc;
+    size_t off = 0;
+    for (size_t i = 0; i < ne; i++) {
+        SJob &J = *X.e[i].job;
+        D.sdh[i] = StopDesc{J.dstate.as<ReqDev>(), J.dblk.as<double>(), J.dsums.as<double>(), X.e[i].it0,
+                            X.e[i].it0 + X.e[i].nb, D.stage.as<double>() + off, D.head.as<int32_t>() + 3 * i};
+        off += 1 + (size_t)(2 + kSums) * X.e[i].nb;
     }
+    HIP_CHECK(hipMemcpyAsync(D.sd.p, D.sdh.data(), sizeof(StopDesc) * ne, hipMemcpyHostToDevice, X.s));
+    hipLaunchKernelGGL(k_stop, dim3((unsigned)((ne + 63) / 64)), dim3(64), 0, X.s, D.sd.as<StopDesc>(), (int)ne);
+    HIP_CHECK(hipGetLastError());
+    D.stage_h.resize(std::max<size_t>(1, nd));
+    D.head_h.resize(3 * ne);
+    HIP_CHECK(hipMemcpyAsync(D.stage_h.data(), D.stage.p, sizeof(double) * nd, hipMemcpyDeviceToHost, X.s));
+    HIP_CHECK(hipMemcpyAsync(D.head_h.data(), D.head.p, sizeof(int32_t) * 3 * ne, hipMemcpyDeviceToHost, X.s));
     HIP_CHECK(hipStreamSynchronize(X.s));
-    double units = 0;
-    for (size_t i = 0; i < X.e.size(); i++) {
+    off = 0;
+    double units = 0;  // force-loop work of the slice (integration steps x atoms, or x clusters)
+    for (size_t i = 0; i < ne; i++) {
         SJob &J = *X.e[i].job;
         const auto &B = P.jobs[i];
-        const std::vector<double> &sums = all[i];
-        for (int b = 0; b < X.e[i].nb; b++) units += sums[kSums * b + 3] * B.units;
-        for (int b = 0; b < X.e[i].nb && !J.finished; b++) {
-            if (X.e[i].it0 + b != J.used) return fail(COSIMS_E_INVALID, "server: blocks out of order");
-            double om = 0, er = 0;
-            integral_ccs(status.data() + B.off + (int64_t)b * B.T, sums.data() + kSums * b, B.T,
-                         J.mol->params.print_rate, om, er);
-            J.omega.push_back(om);
-            J.omega_err.push_back(er);
-            J.sums.insert(J.sums.end(), sums.begin() + kSums * b, sums.begin() + kSums * (b + 1));
-            J.used++;
-            if (J.used >= c.min_blocks) {
-                cosims_ccs tmp{};
-                final_stats(J.omega.data(), J.omega_err.data(), J.sums.data(), J.used, c.block_traj, &tmp);
-                if (tmp.werr / tmp.wmean <= J.target || J.used >= c.max_blocks) {
-                    J.finished = true;
-                    J.ccs = tmp;
-                    J.ccs.lanes = B.L;
-                    J.ccs.wg_size = B.W;
-                }
-            }
+        const int32_t *h = D.head_h.data() + 3 * i;
+        const double *rec = D.stage_h.data() + off;
+        units += rec[0] * B.units;
+        if (h[2] < 0 || h[0] != J.used) return fail(COSIMS_E_INVALID, "server: blocks out of order");
+        for (int k = h[0]; k < h[1]; k++) {
+            const double *o = rec + 1 + (2 + kSums) * (size_t)(k - X.e[i].it0);
+            J.omega.push_back(o[0]);
+            J.omega_err.push_back(o[1]);
+            J.sums.insert(J.sums.end(), o + 2, o + 2 + kSums);
         }
+        J.used = h[1];
+        if (h[2] == 1 && !J.finished) {
+            J.finished = true;
+            cosims_ccs tmp{};
+            final_stats(J.omega.data(), J.omega_err.data(), J.sums.data(), J.used, c.block_traj, &tmp);
+            J.ccs = tmp;
+            J.ccs.lanes = B.L;
+            J.ccs.wg_size = B.W;
+        }
+        off += 1 + (size_t)(2 + kSums) * X.e[i].nb;
     }
     std::lock_guard<std::mutex> lk(S->mu);
     if (R.ms > 0 && units > 0) S->units_per_ms = 0.5 * S->units_per_ms + 0.5 * units / R.ms;
@@ -387,6 +579,7 @@ void scheduler(cosims_server *S)
 {
     const int kDepth = S->cfg.pipeline_depth;
     BatchPlan plans[kMaxDepth];
+    SliceDev devs[kMaxDepth];
     Stream streams[kMaxDepth];
     bool ok = true;
     for (auto &x : streams) ok &= hipStreamCreateWithFlags(&x.s, hipStreamNonBlocking) == hipSuccess;
@@ -423,6 +616,7 @@ void scheduler(cosims_server *S)
             X.e = plan_slice(S, act);
             if (!X.live()) break;
             X.P = &plans[q];
+            X.D = &devs[q];
             X.s = streams[q].s;
             X.t_issue = S->now_ms();
             rc = issue_slice(S, X);

@@ -132,7 +132,7 @@ class ServerCfg(C.Structure):
                 ("slice_ms", C.c_double), ("policy", C.c_int32), ("min_slice_traj", C.c_int64),
                 ("pipeline_depth", C.c_int32), ("issue_fraction", C.c_double),
                 ("prior_rel_sd", C.c_double), ("class_lanes", C.c_int32 * SIZE_CLASSES),
-                ("class_wg", C.c_int32 * SIZE_CLASSES)]
+                ("class_wg", C.c_int32 * SIZE_CLASSES), ("own_launches", C.c_int32)]
 
 
 class Request(C.Structure):
@@ -495,12 +495,13 @@ class Server:
     def __init__(self, prec=PREC_MIXED, lanes=0, wg_size=512, hbm_budget=0, max_active=64, min_blocks=3,
                  max_blocks=1000, setup_threads=0, block_traj=10000, slice_traj=0, slice_ms=0.0,
                  policy=SERVE_SHORTEST_FIRST, min_slice_traj=0, pipeline_depth=0, issue_fraction=0.0,
-                 prior_rel_sd=0.0, class_lanes=None, class_wg=None):
+                 prior_rel_sd=0.0, class_lanes=None, class_wg=None, own_launches=False):
         self.cfg = ServerCfg(prec=prec, lanes=lanes, wg_size=wg_size, hbm_budget=hbm_budget, max_active=max_active,
                              min_blocks=min_blocks, max_blocks=max_blocks, setup_threads=setup_threads,
                              block_traj=block_traj, slice_traj=slice_traj, slice_ms=slice_ms, policy=policy,
                              min_slice_traj=min_slice_traj, pipeline_depth=pipeline_depth,
-                             issue_fraction=issue_fraction, prior_rel_sd=prior_rel_sd)
+                             issue_fraction=issue_fraction, prior_rel_sd=prior_rel_sd,
+                             own_launches=int(own_launches))
         for c in range(SIZE_CLASSES):
             self.cfg.class_lanes[c] = (class_lanes or [0] * SIZE_CLASSES)[c]
             self.cfg.class_wg[c] = (class_wg or [0] * SIZE_CLASSES)[c]

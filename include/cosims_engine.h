@@ -322,10 +322,11 @@ int cosims_tune_batch(const cosims_batch_job *jobs, int32_t njobs, int32_t prec,
  * admitted while an active slot and the device-memory budget allow, and run
  * in blocks: one CCS integral of block_traj trajectories each, Philox
  * (params.seed, job) with iteration = block index.  Each slice is one
- * persistent launch per force type over blocks of every active request;
- * requests arriving meanwhile join the next slice.  A request retires at the
- * first block, from min_blocks on, where werr / wmean <= target (or at
- * max_blocks); its result is that of cosims_run with iter = blocks,
+ * persistent launch per launch shape over blocks of every active request;
+ * requests arriving meanwhile join the next slice.  Each block's CCS and the
+ * stopping rule are computed on the device as the slice completes: a request
+ * retires at the first block, from min_blocks on, where werr / wmean <=
+ * target (or at max_blocks); its result is that of cosims_run with iter = blocks,
  * traj = block_traj and the request's launch shape (fixed lanes, or the size
  * rule of cosims_run_batch), whatever shared the launches.  Per-block sums
  * stay on the device until read. */
@@ -355,6 +356,9 @@ typedef struct cosims_server_cfg {
     int32_t class_lanes[COSIMS_SIZE_CLASSES]; /* with lanes 0: lanes of each size class (e.g. from
                                                  cosims_tune_batch); 0 => the size rule */
     int32_t class_wg[COSIMS_SIZE_CLASSES];    /* workgroup size of each class; 0 => wg_size */
+    int32_t own_launches;     /* 0: a slice's requests of a launch shape share one persistent launch with
+                                 their molecules staged in LDS (as cosims_run_batch's default); 1: a
+                                 launch of its own per request with enough work */
 } cosims_server_cfg;
 
 /* Slice composition: shortest remaining work first (requests ordered by

@@ -245,3 +245,15 @@ def test_server_large_molecule_equals_run():
     run = m.run(E.PREC_MIXED, job=30, iters=r["blocks"], traj=BLOCK, **SHAPE)
     assert np.array_equal(r["omega"], run["omega"])
     assert r["wmean"] == run["wmean"]
+
+
+def test_server_own_launches_equal_shared(mixed_results):
+    """Slices with a launch of their own per request give the results of
+    the default shared LDS launches bit for bit."""
+    res, _ = mixed_results
+    with _server(own_launches=True) as srv:
+        out = [srv.wait(t) for t in _submit_all(srv, REQUESTS)]
+    for r, ref in zip(out, res):
+        assert r["blocks"] == ref["blocks"]
+        assert np.array_equal(r["omega"], ref["omega"])
+        assert r["wmean"] == ref["wmean"]
