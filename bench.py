@@ -385,9 +385,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--serve-requests", type=int, default=200, help="requests of the serving stream (0: none)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
-    ap.add_argument("--serve-sustained-rate", type=float, default=2.5,
-                    help="offered rate of a second stream the GPU can sustain (0: none)")
-    ap.add_argument("--serve-sustained-requests", type=int, default=100)
+    ap.add_argument("--serve-sustained-rate", type=float, default=1.2,
+                    help="offered rate of a second stream the GPU can sustain (0: none); one MI355X serves "
+                         "1.6-2.1 req/s of this mix")
+    ap.add_argument("--serve-sustained-requests", type=int, default=60)
     ap.add_argument("--serve-own-launches", type=int, default=0,
                     help="serving slices with a launch of their own per request (1) or shared LDS launches (0)")
     ap.add_argument("--serve-target", type=float, default=0.005)
