@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -555,25 +556,47 @@ int device_init(Device &d)
     return 0;
 }
 
-// RAII device buffer; alloc() replaces any previous allocation.
+// RAII device buffer; alloc() replaces any previous allocation.  borrow()
+// points it at memory it does not own (a slice of the static scratch).
 struct DBuf {
     void *p = nullptr;
+    bool owned = false;
     DBuf() = default;
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
     ~DBuf() { reset(); }
     void reset()
     {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
+        owned = false;
     }
     hipError_t alloc(size_t bytes)
     {
         reset();
-        return hipMalloc(&p, bytes ? bytes : 16);
+        const hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+        owned = e == hipSuccess;
+        if (!owned) p = nullptr;
+        return e;
+    }
+    void borrow(void *q)
+    {
+        reset();
+        p = q;
     }
     template <typename T> T *as() const { return (T *)p; }
 };
+
+// Points b at the 256-byte aligned slice of `bytes` at or after offset *off
+// of [base, base + cap) and advances *off past it; false when it does not fit.
+bool carve_one(DBuf &b, uint8_t *base, size_t cap, size_t *off, size_t bytes)
+{
+    const size_t o = (*off + 255) / 256 * 256;
+    *off = o + bytes;
+    if (o + bytes > cap) return false;
+    b.borrow(base + o);
+    return true;
+}
 
 struct Stream {
     hipStream_t s = nullptr;
@@ -832,6 +855,7 @@ struct cosims_mol {
     // lanes per trajectory, workgroup size
     mutable std::mutex tune_mu;
     mutable std::map<int, std::pair<int, int>> tuned;
+    mutable std::atomic<bool> on_device{false};  // device copy made (ensure_device succeeded)
     ~cosims_mol() { if (dmem) (void)hipFree(dmem); }
 };
 
@@ -996,6 +1020,7 @@ int ensure_device(const cosims_mol *M)
         M->up_rc = device_init(M->dev);
         if (M->up_rc == 0) M->up_rc = upload(M);
         if (M->up_rc) M->up_err = g_err;
+        else M->on_device.store(true);
     });
     if (M->up_rc) g_err = M->up_err;
     return M->up_rc;
@@ -1309,6 +1334,31 @@ struct TrajBufs {
         HIP_CHECK(counters.alloc(sizeof(int32_t) * 64));
         HIP_CHECK(evals.alloc(sizeof(unsigned long long) * kEvalBufSlots));
         return 0;
+    }
+    // The group-kernel path's buffers for n trajectories as 256-byte aligned
+    // slices of [base, base + cap) from offset *off on (advanced past them);
+    // the lane kernel's attempt buffers (tries, maxtries, exitp, e0) stay
+    // unset.  false when they do not fit.
+    bool carve(uint8_t *base, size_t cap, size_t *off, int64_t n)
+    {
+        bool ok = true;
+        auto take = [&](DBuf &b, size_t bytes) { ok = ok && carve_one(b, base, cap, off, bytes); };
+        take(bin, n);
+        take(order, sizeof(int32_t) * 2 * kOrderBins);
+        take(p0, sizeof(double) * 3 * n);
+        take(v0, sizeof(double) * 3 * n);
+        take(g, sizeof(double) * n);
+        take(status, n);
+        take(chi, sizeof(double) * n);
+        take(steps, sizeof(int32_t) * n);
+        take(tot, sizeof(int32_t) * n);
+        take(exitv2, sizeof(double) * n);
+        take(ergy, sizeof(double) * n);
+        take(items[0], sizeof(int32_t) * n);
+        take(items[1], sizeof(int32_t) * n);
+        take(counters, sizeof(int32_t) * 64);
+        take(evals, sizeof(unsigned long long) * kEvalBufSlots);
+        return ok;
     }
 };
 
@@ -1763,22 +1813,25 @@ int64_t group_grid(const void *fn, int wg, size_t lds, int bpc, int cus, int64_t
     return std::max<int64_t>(1, std::min<int64_t>(grid, (n_items + groups_per_wg - 1) / groups_per_wg));
 }
 
-// One persistent launch of fn(args) with HIP-event timing on stream s, then
-// acos of the accepted exits and the launch's counters.
+// One persistent launch of fn(args) on stream s, timed with HIP events
+// unless `timed` is off (out.ms = 0: no events are created), then acos of
+// the accepted exits and the launch's counters.
 template <class Args>
 int timed_group_launch(void (*fn)(Args), const Args &args, int64_t grid, int wg, size_t lds, hipStream_t s,
-                       TrajBufs &B, int n_items, GroupRun &out)
+                       TrajBufs &B, int n_items, GroupRun &out, bool timed = true)
 {
     int32_t *ctr = B.counters.as<int32_t>();
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ctr + 1, 2, 0);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, B.evals.as<int32_t>(), 2 * kEvalBufSlots, 0);
-    hipEvent_t e0, e1;
-    HIP_CHECK(hipEventCreate(&e0));
-    HIP_CHECK(hipEventCreate(&e1));
-    HIP_CHECK(hipEventRecord(e0, s));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timed) {
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        HIP_CHECK(hipEventRecord(e0, s));
+    }
     hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(wg), lds, s, args);
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipEventRecord(e1, s));
+    if (timed) HIP_CHECK(hipEventRecord(e1, s));
     hipLaunchKernelGGL(k_chi, dim3((unsigned)((n_items + 255) / 256)), dim3(256), 0, s, args.io.items, n_items,
                        args.io.status, args.io.cchi, args.io.chi, (const int32_t *)nullptr, (const int32_t *)nullptr);
     HIP_CHECK(hipGetLastError());
@@ -1786,9 +1839,11 @@ int timed_group_launch(void (*fn)(Args), const Args &args, int64_t grid, int wg,
     HIP_CHECK(hipMemcpyAsync(ev, B.evals.p, sizeof(ev), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     float ms = 0;
-    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (timed) {
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
     out.ms = ms;
     out.steps = (int64_t)ev[0];
     out.evals = (int64_t)ev[1];
@@ -1802,7 +1857,7 @@ int timed_group_launch(void (*fn)(Args), const Args &args, int64_t grid, int wg,
 // Table-kernel launch over the item list items[0..n_items): mols is a device
 // array of MolRec, tmol the molecule of each trajectory (NULL: molecule 0).
 int run_multi(const MolRec *mols, const int32_t *tmol, int ft, int prec, int cus, const int32_t *items, int n_items,
-              TrajBufs &B, hipStream_t s, int L, int wg, int bpc, GroupRun &out)
+              TrajBufs &B, hipStream_t s, int L, int wg, int bpc, GroupRun &out, bool timed = true)
 {
     MultiFn fn = multi_kernel(prec, ft, L);
     if (!fn) return fail(COSIMS_E_INVALID, "no table kernel for this force type / lane count");
@@ -1812,19 +1867,19 @@ int run_multi(const MolRec *mols, const int32_t *tmol, int ft, int prec, int cus
     G.tmol = tmol;
     const size_t lds = multi_lds_bytes(ft, L, wg);
     const int64_t grid = group_grid((const void *)fn, wg, lds, bpc, cus, n_items, L);
-    return timed_group_launch(fn, G, grid, wg, lds, s, B, n_items, out);
+    return timed_group_launch(fn, G, grid, wg, lds, s, B, n_items, out, timed);
 }
 
 // One persistent group launch of one molecule over the item list
 // items[0 .. n_items) (B.items[0] when items is NULL).
 int run_group(const cosims_mol *M, int prec, int n_items, TrajBufs &B, hipStream_t s, int L, int wg, int bpc,
-              GroupRun &out, const int32_t *items = nullptr)
+              GroupRun &out, const int32_t *items = nullptr, bool timed = true)
 {
     if (!items) items = B.items[0].as<int32_t>();
     const bool tiled = tile_mode(M, prec);
     if (!tiled && !lds_mode(M, prec))
         return run_multi(M->rec_dev, nullptr, M->prep.force_type, prec, M->dev.cus, items, n_items, B, s, L, wg, bpc,
-                         out);
+                         out, timed);
     GrpFn fn = tiled ? tile_kernel(prec, M->prep.force_type, L) : group_kernel(prec, M->prep.force_type, L);
     if (!fn) return fail(COSIMS_E_INVALID, "no group kernel for this force type / lane count");
     GroupArgs G = group_args(M);
@@ -1836,10 +1891,11 @@ int run_group(const cosims_mol *M, int prec, int n_items, TrajBufs &B, hipStream
     HIP_CHECK(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     G.io = traj_io(B, items, n_items);
     const int64_t grid = group_grid((const void *)fn, wg, lds, bpc, M->dev.cus, n_items, L);
-    return timed_group_launch(fn, G, grid, wg, lds, s, B, n_items, out);
+    return timed_group_launch(fn, G, grid, wg, lds, s, B, n_items, out, timed);
 }
 
-// k_sample into B with hit compaction into B.items[0] (count in counters[0]).
+// k_sample into B with hit compaction into B.items[0] (count in counters[0]);
+// timed with HIP events when ms is set.
 int sample_into(const cosims_mol *M, uint64_t seed, uint32_t job, int64_t n, int64_t per_iter, TrajBufs &B,
                 hipStream_t s, float *ms)
 {
@@ -1849,6 +1905,11 @@ int sample_into(const cosims_mol *M, uint64_t seed, uint32_t job, int64_t n, int
     A.p0 = B.p0.as<double>(); A.v0 = B.v0.as<double>(); A.g = B.g.as<double>();
     A.status = B.status.as<int8_t>(); A.items = B.items[0].as<int32_t>(); A.n_items = B.counters.as<int32_t>();
     A.bin = B.bin.as<uint8_t>(); A.dt0 = M->params.dt;
+    if (!ms) {
+        hipLaunchKernelGGL(k_sample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, A);
+        HIP_CHECK(hipGetLastError());
+        return 0;
+    }
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
@@ -1885,6 +1946,79 @@ void integral_ccs(const int8_t *st8, const double *sums, int64_t T, int64_t prin
     const double tot = sums[0], tot2 = sums[1];
     omega = tot / d;
     omega_err = std::sqrt((tot2 / d - (tot / d) * (tot / d)) / d);
+}
+
+// integral_ccs on the device, one workgroup of 256 threads an integral.
+// The denominator of one thread's integral (src/monteTraj.cpp:456-521) is T
+// minus the trajectories since the last progress update whose success flag
+// is off, a miss repeating the flag of the trajectory before it.  The last
+// window [w0, T) is scanned in parallel: each thread takes a chunk, finds
+// its last non-miss status, and with the last non-miss status before its
+// chunk (an exclusive scan over the chunks, or before w0) counts the
+// trajectories of its chunk that see the flag off.
+__device__ void integral_ccs_wg(const int8_t *st, const double *sums, int T, int upd, double *omega,
+                                double *omega_err)
+{
+    __shared__ int8_t last[256];
+    __shared__ int8_t carry;
+    __shared__ int orbit;
+    const int tid = threadIdx.x;
+    const int w0 = (T / upd) * upd;  // orbit counts reset at every multiple of upd
+    const int len = T - w0;
+    if (tid == 0) {
+        int8_t c = -1;  // no non-miss yet: the flag is off
+        for (int j = w0 - 1; j >= 0; j--)
+            if (st[j] != ST_MISS) {
+                c = st[j];
+                break;
+            }
+        carry = c;
+        orbit = 0;
+    }
+    const int chunk = (len + 255) / 256;
+    const int a = min(len, tid * chunk), b = min(len, a + chunk);
+    int8_t mine = -1;
+    for (int i = a; i < b; i++)
+        if (st[w0 + i] != ST_MISS) mine = st[w0 + i];
+    last[tid] = mine;
+    __syncthreads();
+    int8_t in = carry;  // last non-miss status before this chunk
+    for (int t = tid - 1; t >= 0; t--)
+        if (last[t] >= 0) {
+            in = last[t];
+            break;
+        }
+    bool succ = in == ST_OK;
+    int off = 0;
+    for (int i = a; i < b; i++) {
+        const int8_t s = st[w0 + i];
+        if (s != ST_MISS) succ = s == ST_OK;
+        off += succ ? 0 : 1;
+    }
+    if (off) atomicAdd(&orbit, off);
+    __syncthreads();
+    if (tid == 0) {
+        const double den = (double)(T - orbit);
+        const double tot = sums[0], tot2 = sums[1];
+        *omega = tot / den;
+        *omega_err = sqrt((tot2 / den - (tot / den) * (tot / den)) / den);
+    }
+}
+
+// The iters integrals of a run of T trajectories each, workgroup = integral.
+__global__ void __launch_bounds__(256) k_iter_ccs(const int8_t *status, const double *sums, int T, int upd,
+                                                  double *omega, double *omega_err)
+{
+    const int it = blockIdx.x;
+    integral_ccs_wg(status + (int64_t)it * T, sums + kSums * it, T, upd, omega + it, omega_err + it);
+}
+
+// Trajectory status codes as cosims_run_cfg.status gives them: 0 miss,
+// 1 success, 2 failure.
+__global__ void k_status_out(const int8_t *st, int64_t n, int32_t *out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = st[i] == ST_OK ? 1 : (st[i] == ST_MISS ? 0 : 2);
 }
 
 // Final statistics of monteTraj::runTraj (src/monteTraj.cpp:412-425) over
@@ -2004,15 +2138,24 @@ int tune_shapes(int units, RunFn run, int &L_out, int &W_out)
 // for (capped: 4096 per CU mixed, 1024 FP64), because the end-of-launch tail
 // weighs differently on small and large launches; results are cached on the
 // handle per precision and power-of-two size of that set.
-int autotune(const cosims_mol *M, int prec, int64_t n_hint, int &L_out, int &wg_out)
+// Run size the tuner times a run of n_hint trajectories at, and the key of
+// its tuned shape in cosims_mol::tuned (precision x power-of-two bucket).
+int64_t tune_size(const cosims_mol *M, int prec, int64_t n_hint, int *key)
 {
-    std::lock_guard<std::mutex> lk(M->tune_mu);
     const int pi = prec == COSIMS_PREC_FP64 ? 0 : 1;
     const int64_t cap = (int64_t)M->dev.cus * (pi ? 4096 : 1024);
     const int64_t n2 = std::max<int64_t>(16384, std::min<int64_t>(n_hint, cap));
     int bucket = 0;
     while ((int64_t(1) << (bucket + 1)) <= n2) bucket++;
-    const int key = 64 * pi + bucket;
+    *key = 64 * pi + bucket;
+    return n2;
+}
+
+int autotune(const cosims_mol *M, int prec, int64_t n_hint, int &L_out, int &wg_out)
+{
+    std::lock_guard<std::mutex> lk(M->tune_mu);
+    int key = 0;
+    const int64_t n2 = tune_size(M, prec, n_hint, &key);
     auto it = M->tuned.find(key);
     if (it != M->tuned.end()) {
         L_out = it->second.first;
@@ -2631,6 +2774,50 @@ int tune_batch(const cosims_batch_job *jobs, int32_t njobs, int prec, ShapeTable
 
 }  // namespace
 
+// ------------------------------------------------ allocation-free runs ---
+// Device scratch of cosims_run_noalloc: the group-kernel path's per-trajectory
+// buffers (102 bytes a trajectory: initial conditions, order buckets, status
+// and outcome, work lists, status codes out) plus the integrals' sums and
+// CCS, for at most COSIMS_NOALLOC_MAX_TRAJ trajectories in at most
+// COSIMS_NOALLOC_MAX_ITER integrals: 27.3 MB on every device the library runs
+// on, held until the process ends.  In host code `scratch` names a host-side
+// stand-in: kernels and copies take the device address hipGetSymbolAddress
+// returns for the current device (scratch_of).
+constexpr size_t kNoAllocBytes = (size_t)COSIMS_NOALLOC_MAX_TRAJ * 104 + 64 * 1024;
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+namespace {
+
+constexpr int kNoAllocDevices = 64;
+
+struct ScratchDev {
+    std::once_flag once;
+    hipError_t rc = hipSuccess;
+    uint8_t *ptr = nullptr;  // device address of this device's copy of scratch
+    std::mutex mu;           // calls on one device share its copy: they run one at a time
+};
+ScratchDev g_scratch[kNoAllocDevices];
+
+// The scratch of device dev (the current device), its address looked up
+// once, on the first call on that device.
+int scratch_of(int dev, ScratchDev *&out)
+{
+    if (dev < 0 || dev >= kNoAllocDevices) return fail(COSIMS_E_INVALID, "cosims_run_noalloc: device index too large");
+    ScratchDev &S = g_scratch[dev];
+    std::call_once(S.once, [&S] {
+        void *p = nullptr;
+        S.rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        S.ptr = S.rc == hipSuccess ? (uint8_t *)p : nullptr;
+    });
+    if (S.rc != hipSuccess || !S.ptr)
+        return fail(COSIMS_E_HIP, std::string("cosims_run_noalloc: hipGetSymbolAddress(scratch): ") +
+                                      hipGetErrorString(S.rc));
+    out = &S;
+    return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 int cosims_sample(const cosims_mol *M, uint64_t seed, uint32_t job, uint32_t iteration, int64_t first, int64_t n,
@@ -2854,6 +3041,123 @@ int cosims_run(const cosims_mol *M, const cosims_params *P, const cosims_run_cfg
         const int64_t no_cl[6] = {0, 0, 0, 0, 0, 0};
         out->ref_flops = ref_flops(ft, M->prep.multipole_order, evals, steps, 0, out->pair_evals, no_cl);
     }
+    out->run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+int cosims_run_noalloc(const cosims_mol *M, const cosims_params *P, const cosims_run_cfg *cfg, double *omega,
+                       double *omega_err, cosims_ccs *out)
+{
+    if (!M || !P || !cfg || !omega || !omega_err || !out)
+        return fail(COSIMS_E_INVALID, "cosims_run_noalloc: null argument");
+    const cosims_run_cfg c = *cfg;
+    const int prec = c.prec;
+    if (prec != COSIMS_PREC_FP64 && prec != COSIMS_PREC_MIXED) return fail(COSIMS_E_INVALID, "unknown precision");
+    const int iters = c.iter > 0 ? c.iter : P->iter;
+    const int64_t T = c.traj > 0 ? c.traj : P->traj;
+    if (iters <= 0 || T <= 0) return fail(COSIMS_E_INVALID, "cosims_run_noalloc: no trajectories");
+    if (iters > COSIMS_NOALLOC_MAX_ITER || T > COSIMS_NOALLOC_MAX_TRAJ || (int64_t)iters * T > COSIMS_NOALLOC_MAX_TRAJ)
+        return fail(COSIMS_E_INVALID, "cosims_run_noalloc: " + std::to_string(iters) + " x " + std::to_string(T) +
+                                          " trajectories exceed its static scratch (at most " +
+                                          std::to_string(COSIMS_NOALLOC_MAX_TRAJ) + " trajectories in at most " +
+                                          std::to_string(COSIMS_NOALLOC_MAX_ITER) + " integrals): use cosims_run");
+    const int64_t n = (int64_t)iters * T;
+    if (c.rng != COSIMS_RNG_PHILOX)
+        return fail(COSIMS_E_INVALID, "cosims_run_noalloc: Philox initial conditions only (the reference's stream "
+                                      "is drawn on the host)");
+    if (!M->on_device.load())
+        return fail(COSIMS_E_INVALID, "cosims_run_noalloc: molecule not on the device (call cosims_mol_upload first)");
+    int dev = -1;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev != M->dev.id)
+        return fail(COSIMS_E_INVALID, "cosims_run_noalloc: the molecule is on device " + std::to_string(M->dev.id) +
+                                          ", device " + std::to_string(dev) + " is current");
+    int L = c.lanes, wg = c.wg_size;
+    if (L == 0) {  // the shape cosims_run tuned for runs of this size, if it has
+        std::lock_guard<std::mutex> lk(M->tune_mu);
+        int key = 0;
+        (void)tune_size(M, prec, n, &key);
+        auto it = M->tuned.find(key);
+        if (it == M->tuned.end())
+            return fail(COSIMS_E_INVALID, "cosims_run_noalloc: no tuned launch shape for runs of this size: give "
+                                          "lanes and wg_size, or tune with cosims_run (lanes 0) first");
+        L = it->second.first;
+        if (wg <= 0) wg = it->second.second;
+    }
+    if (L != 4 && L != 8 && L != 16 && L != 32 && L != 64)
+        return fail(COSIMS_E_INVALID, "cosims_run_noalloc: lanes must be 0 (tuned before), 4, 8, 16, 32 or 64");
+    if (wg <= 0 || wg % 64 || wg > 1024 || wg % L)
+        return fail(COSIMS_E_INVALID, "cosims_run_noalloc: wg_size must be a multiple of 64 and of lanes, at most 1024");
+    ScratchDev *S = nullptr;
+    if (int rc = scratch_of(dev, S)) return rc;
+    std::lock_guard<std::mutex> lk(S->mu);
+    std::memset(out, 0, sizeof(*out));
+    const auto t0 = std::chrono::steady_clock::now();
+    TrajBufs B;
+    DBuf red, om, er, st32;
+    size_t off = 0;
+    bool fits = B.carve(S->ptr, kNoAllocBytes, &off, n);
+    fits = fits && carve_one(red, S->ptr, kNoAllocBytes, &off, sizeof(double) * kSums * iters);
+    fits = fits && carve_one(om, S->ptr, kNoAllocBytes, &off, sizeof(double) * iters);
+    fits = fits && carve_one(er, S->ptr, kNoAllocBytes, &off, sizeof(double) * iters);
+    fits = fits && carve_one(st32, S->ptr, kNoAllocBytes, &off, sizeof(int32_t) * n);
+    if (!fits) return fail(COSIMS_E_NOMEM, "cosims_run_noalloc: scratch layout larger than the static array");
+    // every buffer read below is written first in this call (sample_into and
+    // the launch clear their counters): nothing of the last call shows through
+    const hipStream_t s = hipStreamPerThread;
+    if (int rc = sample_into(M, P->seed, c.job, n, T, B, s, nullptr)) return rc;
+    int32_t hits = 0;
+    HIP_CHECK(hipMemcpyAsync(&hits, B.counters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (hits > 0) {
+        const int32_t *work = nullptr;
+        if (kOrderWork) {  // longest expected trajectories first, as cosims_run launches them
+            work = B.items[1].as<int32_t>();
+            if (int rc = order_items(B.items[0].as<int32_t>(), B.items[1].as<int32_t>(), nullptr, nullptr, hits,
+                                     B.bin.as<uint8_t>(), B.order.as<int32_t>(), s))
+                return rc;
+        }
+        GroupRun G;
+        if (int rc = run_group(M, prec, hits, B, s, L, wg, c.blocks_per_cu, G, work, false)) return rc;
+        out->n_rounds = 1;
+        out->round_items[0] = hits;
+        out->round_steps[0] = G.steps;
+        out->round_evals[0] = G.evals;
+        out->pot_evals = G.pots;
+        for (int k = 0; k < 6; k++) out->cl_counts[k] = G.cl[k];
+        out->ref_flops = ref_flops(M->prep.force_type, M->prep.multipole_order, G.evals, G.steps, G.pots, G.pairs,
+                                   G.cl);
+    }
+    out->lanes = L;
+    out->wg_size = wg;
+    ReduceArgs RA{};
+    RA.status = B.status.as<int8_t>(); RA.chi = B.chi.as<double>(); RA.ergy = B.ergy.as<double>();
+    RA.totsteps = B.tot.as<int32_t>(); RA.per_iter = T; RA.ccs_const = M->prep.ccs_const; RA.out = red.as<double>();
+    hipLaunchKernelGGL(k_reduce, dim3(iters), dim3(256), 0, s, RA);
+    HIP_CHECK(hipGetLastError());
+    const int64_t pr = P->print_rate;
+    const int upd = (int)(pr > 0 ? std::min<int64_t>(pr, INT32_MAX) : std::max<int64_t>(1, (int64_t)(T / 5.0)));
+    hipLaunchKernelGGL(k_iter_ccs, dim3(iters), dim3(256), 0, s, (const int8_t *)B.status.p,
+                       (const double *)red.p, (int)T, upd, om.as<double>(), er.as<double>());
+    HIP_CHECK(hipGetLastError());
+    if (c.status) {
+        hipLaunchKernelGGL(k_status_out, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           (const int8_t *)B.status.p, n, st32.as<int32_t>());
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(c.status, st32.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+    }
+    double sums[kSums * COSIMS_NOALLOC_MAX_ITER];
+    HIP_CHECK(hipMemcpyAsync(sums, red.p, sizeof(double) * kSums * iters, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(omega, om.p, sizeof(double) * iters, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(omega_err, er.p, sizeof(double) * iters, hipMemcpyDeviceToHost, s));
+    if (c.chi) HIP_CHECK(hipMemcpyAsync(c.chi, B.chi.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    if (c.g) HIP_CHECK(hipMemcpyAsync(c.g, B.g.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    final_stats(omega, omega_err, sums, iters, T, out);
+    out->n_traj = n;
+    const int ft = M->prep.force_type;
+    const int64_t npair = (ft == 4 || ft == 6) ? M->prep.n : 0;
+    out->pair_evals = npair * out->round_evals[0] + out->cl_counts[3] + out->cl_counts[4] + out->cl_counts[5];
     out->run_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return 0;
 }

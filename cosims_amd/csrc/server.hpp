@@ -38,60 +38,11 @@ struct BlockDesc {
     int32_t T, upd;       // trajectories; progress-update period (the denominator's reset)
 };
 
-// The denominator of one thread's integral (src/monteTraj.cpp:456-521):
-// T minus the trajectories since the last progress update whose success flag
-// is off, a miss repeating the flag of the trajectory before it.  The last
-// window [w0, T) is scanned in parallel: each thread takes a chunk, finds
-// its last non-miss status, and with the last non-miss status before its
-// chunk (an exclusive scan over the chunks, or before w0) counts the
-// trajectories of its chunk that see the flag off.  One workgroup a block.
+// One workgroup a block (integral_ccs_wg).
 __global__ void __launch_bounds__(256) k_block_ccs(const BlockDesc *D)
 {
-    __shared__ int8_t last[256];
-    __shared__ int8_t carry;
-    __shared__ int orbit;
     const BlockDesc d = D[blockIdx.x];
-    const int T = d.T, upd = d.upd, tid = threadIdx.x;
-    const int w0 = (T / upd) * upd;  // orbit counts reset at every multiple of upd
-    const int len = T - w0;
-    if (tid == 0) {
-        int8_t c = -1;  // no non-miss yet: the flag is off
-        for (int j = w0 - 1; j >= 0; j--)
-            if (d.st[j] != ST_MISS) {
-                c = d.st[j];
-                break;
-            }
-        carry = c;
-        orbit = 0;
-    }
-    const int chunk = (len + 255) / 256;
-    const int a = min(len, tid * chunk), b = min(len, a + chunk);
-    int8_t mine = -1;
-    for (int i = a; i < b; i++)
-        if (d.st[w0 + i] != ST_MISS) mine = d.st[w0 + i];
-    last[tid] = mine;
-    __syncthreads();
-    int8_t in = carry;  // last non-miss status before this chunk
-    for (int t = tid - 1; t >= 0; t--)
-        if (last[t] >= 0) {
-            in = last[t];
-            break;
-        }
-    bool succ = in == ST_OK;
-    int off = 0;
-    for (int i = a; i < b; i++) {
-        const int8_t s = d.st[w0 + i];
-        if (s != ST_MISS) succ = s == ST_OK;
-        off += succ ? 0 : 1;
-    }
-    if (off) atomicAdd(&orbit, off);
-    __syncthreads();
-    if (tid == 0) {
-        const double den = (double)(T - orbit);
-        const double tot = d.sums[0], tot2 = d.sums[1];
-        d.out[0] = tot / den;
-        d.out[1] = sqrt((tot2 / den - (tot / den) * (tot / den)) / den);
-    }
+    integral_ccs_wg(d.st, d.sums, d.T, d.upd, d.out, d.out + 1);
 }
 
 // Running state of a served request on the device: blocks taken into its

@@ -19,13 +19,15 @@ PREC_MIXED = 1
 RNG_PHILOX = 0
 RNG_REFERENCE = 1
 MAX_ROUNDS = 4
+NOALLOC_MAX_TRAJ = 262144  # cosims_run_noalloc's limits (COSIMS_NOALLOC_MAX_*)
+NOALLOC_MAX_ITER = 256
 
 EXPORTS = [
     "cosims_params_default", "cosims_params_read", "cosims_molfile_read", "cosims_molfile_free",
     "cosims_mol_create", "cosims_mol_create_prepared", "cosims_mol_destroy", "cosims_mol_upload", "cosims_mol_info_get",
     "cosims_mol_atoms", "cosims_mol_clusters", "cosims_accel", "cosims_potential", "cosims_sample",
     "cosims_sample_reference",
-    "cosims_verlet", "cosims_run", "cosims_run_batch", "cosims_tune_batch", "cosims_strerror",
+    "cosims_verlet", "cosims_run", "cosims_run_noalloc", "cosims_run_batch", "cosims_tune_batch", "cosims_strerror",
     "cosims_last_error", "cosims_device_info", "cosims_device_sync", "cosims_device_fma_rate", "cosims_force_rate",
     "cosims_server_create", "cosims_server_submit", "cosims_server_wait", "cosims_server_stats_get",
     "cosims_server_destroy",
@@ -178,6 +180,7 @@ def lib():
         L.cosims_sample_reference.argtypes = [vp, C.c_uint64, C.c_int32, C.c_int64, vp, vp, vp, vp]
         L.cosims_verlet.argtypes = [vp, C.c_int32, C.c_int64, vp, vp, vp, vp, vp, vp, vp]
         L.cosims_run.argtypes = [vp, C.POINTER(Params), C.POINTER(RunCfg), vp, vp, C.POINTER(Ccs)]
+        L.cosims_run_noalloc.argtypes = [vp, C.POINTER(Params), C.POINTER(RunCfg), vp, vp, C.POINTER(Ccs)]
         L.cosims_run_batch.argtypes = [C.POINTER(BatchJob), C.c_int32, C.POINTER(BatchCfg), C.POINTER(BatchStats)]
         L.cosims_server_create.argtypes = [C.POINTER(ServerCfg), C.POINTER(vp)]
         L.cosims_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_int64)]
@@ -404,7 +407,11 @@ class Molecule:
         return dict(chi=chi, ok=ok, steps=steps, ergy=ergy, total_steps=tot)
 
     def run(self, prec=PREC_MIXED, job=0, iters=0, traj=0, keep=False, params: Params | None = None,
-            lanes=0, wg_size=0, blocks_per_cu=0, rng=RNG_PHILOX):
+            lanes=0, wg_size=0, blocks_per_cu=0, rng=RNG_PHILOX, noalloc=False):
+        """monteTraj::runTraj: iters CCS integrals of traj trajectories.
+        noalloc=True calls cosims_run_noalloc (no device allocation: a static
+        scratch of at most NOALLOC_MAX_TRAJ trajectories in NOALLOC_MAX_ITER
+        integrals, group kernels, molecule uploaded first)."""
         p = params if params is not None else self.params
         it = iters or p.iter
         T = traj or p.traj
@@ -416,7 +423,8 @@ class Molecule:
             cfg.chi, cfg.status, cfg.g = _p(chi), _p(status), _p(g)
         om = np.empty(it); err = np.empty(it)
         res = Ccs()
-        _check(lib().cosims_run(self._h, C.byref(p), C.byref(cfg), _p(om), _p(err), C.byref(res)))
+        fn = lib().cosims_run_noalloc if noalloc else lib().cosims_run
+        _check(fn(self._h, C.byref(p), C.byref(cfg), _p(om), _p(err), C.byref(res)))
         out = {k: getattr(res, k) for k, _ in Ccs._fields_}
         for k in ("traj_kernel_ms", "round_steps", "round_evals", "round_items"):
             out[k] = list(out[k])[: res.n_rounds]

@@ -249,6 +249,28 @@ typedef struct cosims_ccs {
 int cosims_run(const cosims_mol *m, const cosims_params *p, const cosims_run_cfg *cfg, double *omega,
                double *omega_err, cosims_ccs *out);
 
+/* cosims_run without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * library calls no hipMalloc, hipFree, stream or event creation in it: the
+ * run's device buffers are slices of a static device array (27.3 MB on every
+ * device the library runs on, held until the process ends), its work goes to
+ * the calling thread's default stream (hipStreamPerThread), and the
+ * per-launch times (traj_kernel_ms, sample_kernel_ms) stay 0.  Limits: at
+ * most COSIMS_NOALLOC_MAX_TRAJ trajectories (iter x traj) in at most
+ * COSIMS_NOALLOC_MAX_ITER integrals (COSIMS_E_INVALID beyond), Philox
+ * initial conditions, the group kernels (lanes 4..64 and wg_size given, or
+ * lanes 0 for the shape cosims_run has tuned for runs of this size), a
+ * molecule already on the current device (cosims_mol_upload).  Results equal
+ * cosims_run's with the same configuration bit for bit.  Calls on one device
+ * share its array: they run one at a time (a lock per device); the array
+ * keeps the last call's contents, and every buffer a call reads it writes
+ * first. */
+#define COSIMS_NOALLOC_MAX_TRAJ 262144
+#define COSIMS_NOALLOC_MAX_ITER 256
+int cosims_run_noalloc(const cosims_mol *m, const cosims_params *p, const cosims_run_cfg *cfg, double *omega,
+                       double *omega_err, cosims_ccs *out);
+
 /* Many CCS runs in one call.  The reference runs one molecule per process
  * (src/main.cpp:119-140, one monteTraj::runTraj each); here all jobs are
  * sampled into one work list, ordered by launch shape (force type x lanes

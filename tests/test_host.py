@@ -21,6 +21,8 @@ def test_library_exports_every_declared_symbol():
     missing = [s for s in sorted(declared) if not hasattr(L, s)]
     assert not missing, missing
     assert declared == set(E.EXPORTS)
+    limits = dict(re.findall(r"#define COSIMS_NOALLOC_(MAX_\w+) (\d+)", hdr))
+    assert (int(limits["MAX_TRAJ"]), int(limits["MAX_ITER"])) == (E.NOALLOC_MAX_TRAJ, E.NOALLOC_MAX_ITER)
 
 
 def test_params_defaults_and_input_file(tmp_path):
