@@ -283,7 +283,29 @@ __device__ __forceinline__ void force32_epilogue(float (&s)[12], double ds, doub
     }
 }
 
-// Atoms from LDS (latency of a few dozen cycles: loaded where used).
+// The pairs of lane j with atoms j, j + L, ... < cnt in LDS (cnt a multiple
+// of L), the next atom read into registers of its own while the current
+// pair is computed: loaded where used, each LDS read waits out its latency
+// in full, and the loop runs 10-12 % slower at 4-6 wavefronts a SIMD.
+template <int FT, int L, typename P4, typename P2>
+__device__ __forceinline__ void lds_pairs32(P4 xq, P2 ab, int cnt, int j, float px, float py, float pz,
+                                            float (&s)[12])
+{
+    if (cnt <= 0) return;
+    f4n a = xq[j];
+    f2n c = ab[j];
+#pragma unroll 4
+    for (int kk = L; kk < cnt; kk += L) {
+        const f4n an = xq[kk + j];
+        const f2n cn = ab[kk + j];
+        pair32<FT>(a, c, px, py, pz, s);
+        a = an;
+        c = cn;
+    }
+    pair32<FT>(a, c, px, py, pz, s);
+}
+
+// Atoms from LDS.
 template <int FT, int L, typename P4, typename P2>
 __device__ __forceinline__ void grp_force32(P4 xq, P2 ab, int npad, int j, float px, float py, float pz, double ds,
                                             double &ox, double &oy, double &oz)
@@ -291,8 +313,7 @@ __device__ __forceinline__ void grp_force32(P4 xq, P2 ab, int npad, int j, float
     float s[12];
 #pragma unroll
     for (int k = 0; k < 12; k++) s[k] = 0.0f;
-#pragma unroll 4
-    for (int kk = 0; kk < npad; kk += L) pair32<FT>(xq[kk + j], ab[kk + j], px, py, pz, s);
+    lds_pairs32<FT, L>(xq, ab, npad, j, px, py, pz, s);
     force32_epilogue<FT, L>(s, ds, ox, oy, oz);
 }
 
@@ -963,10 +984,7 @@ struct TileSource {
                     ab[i] = ga[base + i];
                 }
                 __syncthreads();
-                if (wave) {
-#pragma unroll 4
-                    for (int kk = 0; kk < cnt; kk += L) pair32<FT>(xq[kk + j], ab[kk + j], px, py, pz, s);
-                }
+                if (wave) lds_pairs32<FT, L>(xq, ab, cnt, j, px, py, pz, s);
             }
             if (wave) force32_epilogue<FT, L>(s, A.m.dipole_scalar, fx, fy, fz);
         } else {
