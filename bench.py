@@ -98,10 +98,12 @@ def pct(a, q):
 
 
 def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, fp64_max_atoms=0, fp64_max_requests=40,
-            **server_kw):
+            clustered=False, **server_kw):
     """Poisson request stream through the CCS server; latency = completion -
     submission on the server's clock (setup, queueing and trajectories).
-    The arrival times are replayed in real time."""
+    The arrival times are replayed in real time.  clustered: the requests
+    keep the reference's default cut-offs (multipole and dispersion on: PCM
+    clustering, clustered force type) instead of exact electrostatics."""
     import tempfile
     reqs = synth.serving_stream(count=n_requests, rate=rate, seed=seed)
     d = tempfile.mkdtemp(prefix="cosims_serve_")
@@ -112,7 +114,8 @@ def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, fp64_max_at
         mols.append(E.read_molfile(path))
 
     def params(k):
-        return E.default_params(temp=reqs[k]["temp"], seed=seed + k, multipole=0, dispersion_cutoff=0, setup_gpu=1)
+        cut = 1 if clustered else 0
+        return E.default_params(temp=reqs[k]["temp"], seed=seed + k, multipole=cut, dispersion_cutoff=cut, setup_gpu=1)
     # launch shape: lanes per trajectory by molecule size (the size rule), fixed for every request
     W = 512
     # first issue sized for a per-trajectory relative spread of 2.0 (the spread of the CCS terms runs
@@ -173,11 +176,19 @@ def serving(E, synth, n_requests, rate, target, seed=5000, policy=0, fp64_max_at
                    "p99": pct([lat[i] for i in f], 99)} for f in fifths if len(f)]
     return {
         "workload": f"Poisson {rate} req/s, {n_requests} requests, N log-uniform 100-20,000, "
-                    f"q round(N/800)+1 e, T 200-400 K, to {100 * target:.1f}% relative SE",
+                    f"q round(N/800)+1 e, T 200-400 K, to {100 * target:.1f}% relative SE"
+                    + (", PCM-clustered force field (multipole and dispersion cut-offs on)" if clustered
+                       else ", exact electrostatics and LJ"),
         "requests": n_requests, "offered_rate": rate, "served_rate": n_requests / makespan,
         "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "mean": float(np.mean(lat)), "max": max(lat)},
         "latency_by_arrival_fifth_ms": by_arrival,
         "setup_ms_p50": pct([r["setup_ms"] for r in res], 50),
+        # setup phases of a request: boundary marches (GPU), PCM clustering (host threads), multipoles,
+        # device upload
+        "setup_phase_ms_p50": {k: pct([r[f"setup_{k}_ms"] for r in res], 50)
+                               for k in ("boundary", "cluster", "moments", "upload")},
+        "setup_phase_ms_max": {k: max(r[f"setup_{k}_ms"] for r in res)
+                               for k in ("boundary", "cluster", "moments", "upload")},
         "queue_ms_p50": pct([r["t_admit_ms"] - r["t_ready_ms"] for r in res], 50),
         "trajectories_per_s": traj / makespan, "trajectories": traj, "makespan_s": makespan,
         "trajectories_issued": st["traj"],
@@ -391,6 +402,9 @@ def main():
     ap.add_argument("--serve-sustained-requests", type=int, default=60)
     ap.add_argument("--serve-own-launches", type=int, default=0,
                     help="serving slices with a launch of their own per request (1) or shared LDS launches (0)")
+    ap.add_argument("--serve-clustered-requests", type=int, default=16,
+                    help="requests of a stream with the reference's default (clustered) force field (0: none)")
+    ap.add_argument("--serve-clustered-rate", type=float, default=1.0)
     ap.add_argument("--serve-target", type=float, default=0.005)
     ap.add_argument("--serve-fp64-atoms", type=int, default=1000,
                     help="rerun served requests of up to this many atoms in FP64 over the same blocks (0: none)")
@@ -560,6 +574,16 @@ def main():
                          class_wg=cwg, class_lanes=cl, own_launches=own)
             out["serving_equal_share"] = {k: eq[k] for k in ("latency_ms", "served_rate", "trajectories_per_s",
                                                              "makespan_s", "slices", "trajectories_issued")}
+        if a.serve_clustered_requests > 0:
+            # requests with the reference's default cut-offs: PCM clustering in each request's setup
+            # and the clustered force type in its trajectories
+            cu = serving(E, synth, a.serve_clustered_requests, a.serve_clustered_rate, a.serve_target, seed=7000,
+                         fp64_max_atoms=a.serve_fp64_atoms, fp64_max_requests=8, clustered=True, class_wg=cwg,
+                         class_lanes=cl, own_launches=own)
+            out["serving_clustered"] = cu
+            out["served_latency_ms"]["clustered"] = {"p50": cu["latency_ms"]["p50"], "p99": cu["latency_ms"]["p99"],
+                                                     "offered_rate": a.serve_clustered_rate,
+                                                     "requests": a.serve_clustered_requests}
     print(json.dumps(out), flush=True)
 
 

@@ -558,24 +558,39 @@ int device_init(Device &d)
 
 // RAII device buffer; alloc() replaces any previous allocation.  borrow()
 // points it at memory it does not own (a slice of the static scratch).
+// alloc_async() takes stream-ordered memory (hipMallocAsync on stream s),
+// given back with hipFreeAsync on the null stream: hipFree waits for the
+// whole device, so a buffer freed while persistent launches run on other
+// streams (molecule setup and request bookkeeping beside a serving slice)
+// would hold its thread until they end.  The owner frees it only after its
+// last use has completed.
 struct DBuf {
     void *p = nullptr;
-    bool owned = false;
+    bool owned = false, async = false;
     DBuf() = default;
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
     ~DBuf() { reset(); }
     void reset()
     {
-        if (p && owned) (void)hipFree(p);
+        if (p && owned) (void)(async ? hipFreeAsync(p, nullptr) : hipFree(p));
         p = nullptr;
         owned = false;
+        async = false;
     }
     hipError_t alloc(size_t bytes)
     {
         reset();
         const hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
         owned = e == hipSuccess;
+        if (!owned) p = nullptr;
+        return e;
+    }
+    hipError_t alloc_async(size_t bytes, hipStream_t s)
+    {
+        reset();
+        const hipError_t e = hipMallocAsync(&p, bytes ? bytes : 16, s);
+        owned = async = e == hipSuccess;
         if (!owned) p = nullptr;
         return e;
     }
@@ -734,12 +749,11 @@ __global__ void __launch_bounds__(64) k_chain_rays(ChainArgs A)
 
 // Boundary marches on the GPU for prepare(): the molecule's atoms are
 // uploaded once, then the chain of launches runs with one wait at its end.
+// Its buffers are stream-ordered (DBuf::alloc_async): a setup running beside
+// a serving slice does not wait for the slice at every free.
 struct GpuMarcher : BoundaryMarcher {
+    DBuf atoms_buf;
     void *atoms = nullptr;
-    ~GpuMarcher() override
-    {
-        if (atoms) (void)hipFree(atoms);
-    }
     int boundary(const Prepared &p, double mk, const BoundaryPlan &bp, MarchHit &zhit, std::vector<MarchHit> &rays,
                  std::string &err) override
     {
@@ -766,8 +780,12 @@ struct GpuMarcher : BoundaryMarcher {
             a[i].x = p.xyz[3 * i]; a[i].y = p.xyz[3 * i + 1]; a[i].z = p.xyz[3 * i + 2];
             a[i].q = p.q[i]; a[i].s6 = p.sig6[i]; a[i].e24 = p.eps24m[i]; a[i].e4 = p.eps4[i]; a[i].pad = 0;
         }
-        HIP_CHECK(hipMalloc(&atoms, sizeof(AtomF64) * p.n));
-        HIP_CHECK(hipMemcpy(atoms, a.data(), sizeof(AtomF64) * p.n, hipMemcpyHostToDevice));
+        Stream st;
+        HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+        HIP_CHECK(atoms_buf.alloc_async(sizeof(AtomF64) * p.n, st.s));
+        atoms = atoms_buf.p;
+        HIP_CHECK(hipMemcpyAsync(atoms, a.data(), sizeof(AtomF64) * p.n, hipMemcpyHostToDevice, st.s));
+        HIP_CHECK(hipStreamSynchronize(st.s));
         return 0;
     }
     int boundary_(const Prepared &p, double mk, const BoundaryPlan &bp, MarchHit &zhit, std::vector<MarchHit> &rays)
@@ -776,8 +794,8 @@ struct GpuMarcher : BoundaryMarcher {
         Stream st;
         HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
         DBuf dst, dh;
-        HIP_CHECK(dst.alloc(sizeof(ChainState)));
-        HIP_CHECK(dh.alloc(sizeof(MarchHit) * kChainMaxRays));
+        HIP_CHECK(dst.alloc_async(sizeof(ChainState), st.s));
+        HIP_CHECK(dh.alloc_async(sizeof(MarchHit) * kChainMaxRays, st.s));
         ChainArgs A{};
         A.m = mol_dev(p);
         A.bp = bp;
@@ -814,8 +832,8 @@ struct GpuMarcher : BoundaryMarcher {
         Stream st;
         HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
         DBuf dw, dh;
-        HIP_CHECK(dw.alloc(sizeof(March) * w.size()));
-        HIP_CHECK(dh.alloc(sizeof(MarchHit) * w.size()));
+        HIP_CHECK(dw.alloc_async(sizeof(March) * w.size(), st.s));
+        HIP_CHECK(dh.alloc_async(sizeof(MarchHit) * w.size(), st.s));
         HIP_CHECK(hipMemcpyAsync(dw.p, w.data(), sizeof(March) * w.size(), hipMemcpyHostToDevice, st.s));
         MarchArgs A{};
         A.m = mol_dev(p);
@@ -856,7 +874,9 @@ struct cosims_mol {
     mutable std::mutex tune_mu;
     mutable std::map<int, std::pair<int, int>> tuned;
     mutable std::atomic<bool> on_device{false};  // device copy made (ensure_device succeeded)
-    ~cosims_mol() { if (dmem) (void)hipFree(dmem); }
+    // stream-ordered (see DBuf::alloc_async): destroying a molecule does not
+    // wait for launches of other molecules
+    ~cosims_mol() { if (dmem) (void)hipFreeAsync(dmem, nullptr); }
 };
 
 namespace {
@@ -983,7 +1003,7 @@ int upload(const cosims_mol *M)
         lj_constants(soa[4 * npad + i], soa[5 * npad + i], A, B);
         se[i] = make_float2(A, B);
     }
-    HIP_CHECK(hipMalloc(&M->dmem, total));
+    HIP_CHECK(hipMallocAsync(&M->dmem, total, nullptr));
     M->dbytes = total;
     char *d = (char *)M->dmem;
     MolDev &m = M->md;
@@ -1213,6 +1233,9 @@ int cosims_mol_info_get(const cosims_mol *M, cosims_mol_info *o)
     o->fit_abc[0] = p.fit_a; o->fit_abc[1] = p.fit_b; o->fit_abc[2] = p.fit_c; o->fit_rmsd = p.fit_rmsd;
     o->n_boundary = p.n_boundary;
     o->setup_ms = M->setup_ms;
+    o->setup_boundary_ms = p.boundary_ms;
+    o->setup_cluster_ms = p.cluster_ms;
+    o->setup_moments_ms = p.moments_ms;
     return 0;
 }
 

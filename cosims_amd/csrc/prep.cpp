@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
@@ -541,6 +542,11 @@ void enlarge(const std::vector<V3> &P, Prepared &m)
     m.enlarge = a - aOld;
 }
 
+double ms_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 // ---------------------------------------------------------------- PCM --
 void eig_vals_pcm(double A[3][3], double ev[3])
 {
@@ -572,115 +578,173 @@ void eig_vecs_pcm(double A[3][3], const double l[3], double E[3][3])
     }
 }
 
-struct ClusterProps {
-    std::vector<V3> center;
-    std::vector<double> maxd;
-};
-
-void cluster_props(const std::vector<double> &xyz, const std::vector<std::vector<int>> &g, ClusterProps &cp)
+// Geometric centre and largest centre distance of one cluster
+// (src/PCM_Cluster.cpp:180-205): the first member only seeds minDist.
+void cluster_prop(const std::vector<double> &xyz, const std::vector<int> &G, V3 &center, double &maxd)
 {
-    cp.center.assign(g.size(), V3());
-    cp.maxd.assign(g.size(), 0.0);
-    for (size_t i = 0; i < g.size(); i++) {
-        V3 c;
-        for (int idx : g[i]) c = c + atom(xyz, idx);
-        c = c / (double)g[i].size();
-        cp.center[i] = c;
-        V3 md;
-        bool first = true;
-        for (int idx : g[i]) {
-            V3 d = c - atom(xyz, idx);
-            double dm = d.mag();
-            if (first) first = false;  // the first member only seeds minDist (src/PCM_Cluster.cpp:193-201)
-            else if (dm > md.mag()) md = d;
-        }
-        cp.maxd[i] = md.mag();
+    V3 c;
+    for (int idx : G) c = c + atom(xyz, idx);
+    c = c / (double)G.size();
+    center = c;
+    V3 md;
+    bool first = true;
+    for (int idx : G) {
+        V3 d = c - atom(xyz, idx);
+        double dm = d.mag();
+        if (first) first = false;
+        else if (dm > md.mag()) md = d;
     }
+    maxd = md.mag();
 }
 
-int form_clusters(Prepared &m, int mcs, std::string &err)
+// One PCM split of cluster G (src/PCM_Cluster.cpp:38-176): covariance about
+// the mean, eigenvalues and eigenvectors by eigen3D's closed forms, members
+// on either side of the plane normal to the dominant eigenvector (angle to it
+// at most pi/2: c1), both halves' centres and extents.  ok = false when a
+// half is empty.
+struct Split {
+    std::vector<int> c1, c2;
+    V3 cen1, cen2;
+    double md1 = 0, md2 = 0;
+    bool ok = true;
+};
+
+void split_cluster(const std::vector<double> &xyz, const std::vector<int> &G, Split &S)
+{
+    double mean[3] = {0, 0, 0}, cov[3][3], ev[3], E[3][3];
+    for (int idx : G)
+        for (int j = 0; j < 3; j++) mean[j] += xyz[3 * idx + j];
+    for (int j = 0; j < 3; j++) mean[j] = mean[j] / ((double)G.size());
+    for (int a = 0; a < 3; a++)
+        for (int b = a; b < 3; b++) {
+            double s = 0;
+            for (int idx : G) s += (xyz[3 * idx + a] - mean[a]) * (xyz[3 * idx + b] - mean[b]);
+            cov[a][b] = s;
+            if (a != b) cov[b][a] = s;
+        }
+    eig_vals_pcm(cov, ev);
+    eig_vecs_pcm(cov, ev, E);
+    int emax = 0;
+    for (int k = 0; k < 3; k++)
+        if (std::fabs(ev[k]) > ev[emax]) emax = k;
+    V3 c;
+    for (int idx : G) c = c + atom(xyz, idx);
+    c = c / ((double)G.size());
+    S.c1.clear();
+    S.c2.clear();
+    for (int idx : G) {
+        V3 d = atom(xyz, idx) - c;
+        double t1 = d.x * E[emax][0] + d.y * E[emax][1] + d.z * E[emax][2];
+        double t2 = d.mag() * std::sqrt(E[emax][0] * E[emax][0] + E[emax][1] * E[emax][1] + E[emax][2] * E[emax][2]);
+        double t3 = t1 / t2, ang;
+        if (t3 > 1) ang = 0;
+        else if (t3 < -1) ang = kPi;
+        else ang = std::acos(t1 / t2);
+        (ang <= kPi2 ? S.c1 : S.c2).push_back(idx);
+    }
+    S.ok = !S.c1.empty() && !S.c2.empty();
+    if (!S.ok) return;
+    cluster_prop(xyz, S.c1, S.cen1, S.md1);
+    cluster_prop(xyz, S.c2, S.cen2, S.md2);
+}
+
+// f(i) for i in [0, n) on up to `threads` host threads (0: all), taking
+// indices from a shared counter.
+template <class F>
+void parallel_for(int n, int threads, F &&f)
+{
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int nt = std::max(1, std::min(threads > 0 ? threads : hw, n));
+    if (nt == 1) {
+        for (int i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    auto work = [&] {
+        for (int i; (i = next.fetch_add(1)) < n;) f(i);
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(work);
+    work();
+    for (auto &t : th) t.join();
+}
+
+// PCM_Cluster::clusterAtoms (src/PCM_Cluster.cpp:38-249): passes over the
+// cluster list split every cluster wider than mcs (its first half in place,
+// its second appended, in list order) until none is; then orderClusters'
+// selection sort on centre z.  The splits of one pass are independent and
+// run on host threads; the list is assembled in the reference's order, so
+// membership and order do not depend on the thread count.  Clusters keep
+// the centre and extent computed when they were formed (the reference
+// recomputes them every pass, with the same result).
+int form_clusters(Prepared &m, int mcs, int threads, std::string &err)
 {
     std::vector<std::vector<int>> g(1);
     for (int i = 0; i < m.n; i++) g[0].push_back(i);
-    ClusterProps cp;
-    cluster_props(m.xyz, g, cp);
-    bool accept;
-    do {
-        accept = true;
-        size_t size = g.size();
-        for (size_t i = 0; i < size; i++) {
-            if (!(cp.maxd[i] > mcs)) continue;
-            const std::vector<int> &G = g[i];
-            double mean[3] = {0, 0, 0}, cov[3][3], ev[3], E[3][3];
-            for (int idx : G)
-                for (int j = 0; j < 3; j++) mean[j] += m.xyz[3 * idx + j];
-            for (int j = 0; j < 3; j++) mean[j] = mean[j] / ((double)G.size());
-            for (int a = 0; a < 3; a++)
-                for (int b = a; b < 3; b++) {
-                    double s = 0;
-                    for (int idx : G) s += (m.xyz[3 * idx + a] - mean[a]) * (m.xyz[3 * idx + b] - mean[b]);
-                    cov[a][b] = s;
-                    if (a != b) cov[b][a] = s;
-                }
-            eig_vals_pcm(cov, ev);
-            eig_vecs_pcm(cov, ev, E);
-            int emax = 0;
-            for (int k = 0; k < 3; k++)
-                if (std::fabs(ev[k]) > ev[emax]) emax = k;
-            V3 c;
-            for (int idx : G) c = c + atom(m.xyz, idx);
-            c = c / ((double)G.size());
-            std::vector<int> c1, c2;
-            for (int idx : G) {
-                V3 d = atom(m.xyz, idx) - c;
-                double t1 = d.x * E[emax][0] + d.y * E[emax][1] + d.z * E[emax][2];
-                double t2 = d.mag() * std::sqrt(E[emax][0] * E[emax][0] + E[emax][1] * E[emax][1] + E[emax][2] * E[emax][2]);
-                double t3 = t1 / t2, ang;
-                if (t3 > 1) ang = 0;
-                else if (t3 < -1) ang = kPi;
-                else ang = std::acos(t1 / t2);
-                (ang <= kPi2 ? c1 : c2).push_back(idx);
-            }
-            if (c1.empty() || c2.empty()) {
+    std::vector<V3> center(1);
+    std::vector<double> maxd(1);
+    cluster_prop(m.xyz, g[0], center[0], maxd[0]);
+    const int thr = m.n >= 2048 ? threads : 1;  // small molecules: thread start-up outweighs the work
+    std::vector<int> wide;
+    std::vector<Split> sp;
+    for (;;) {
+        wide.clear();
+        for (size_t i = 0; i < g.size(); i++)
+            if (maxd[i] > mcs) wide.push_back((int)i);
+        if (wide.empty()) break;
+        sp.assign(wide.size(), Split());
+        parallel_for((int)wide.size(), thr, [&](int k) { split_cluster(m.xyz, g[wide[k]], sp[k]); });
+        for (size_t k = 0; k < wide.size(); k++) {
+            Split &S = sp[k];
+            if (!S.ok) {
                 err = "PCM split produced an empty cluster (the reference stalls here)";
                 return COSIMS_E_UNSUPPORTED;
             }
-            g[i].swap(c1);
-            g.push_back(c2);
+            const int i = wide[k];
+            g[i].swap(S.c1);
+            center[i] = S.cen1;
+            maxd[i] = S.md1;
+            g.push_back(std::move(S.c2));
+            center.push_back(S.cen2);
+            maxd.push_back(S.md2);
         }
-        cluster_props(m.xyz, g, cp);
-        for (size_t i = 0; i < g.size(); i++)
-            if (cp.maxd[i] > mcs) {
-                accept = false;
-                break;
-            }
-    } while (!accept);
-    cluster_props(m.xyz, g, cp);
+    }
     const int ng = (int)g.size();
-    for (int k = 0; k < ng - 1; k++) {  // orderClusters: selection sort on centre z
-        int mi = k;
-        double mv = cp.center[k].z;
-        for (int i = k + 1; i < ng; i++)
-            if (cp.center[i].z < mv) {
-                mv = cp.center[i].z;
-                mi = i;
-            }
-        std::swap(cp.maxd[k], cp.maxd[mi]);
-        std::swap(cp.center[k], cp.center[mi]);
-        std::swap(g[k], g[mi]);
+    // orderClusters: selection sort on centre z (first minimum, swapped
+    // forward).  With all centre z distinct its result is the one ascending
+    // order, which an index sort gives in O(ng log ng); ties keep the
+    // selection sort, whose order of equal keys depends on its swaps.
+    std::vector<int> ord(ng);
+    for (int k = 0; k < ng; k++) ord[k] = k;
+    std::sort(ord.begin(), ord.end(), [&](int a, int b) { return center[a].z < center[b].z; });
+    bool distinct = true;
+    for (int k = 1; k < ng && distinct; k++) distinct = center[ord[k - 1]].z < center[ord[k]].z;
+    if (!distinct) {
+        for (int k = 0; k < ng; k++) ord[k] = k;
+        for (int k = 0; k < ng - 1; k++) {
+            int mi = k;
+            double mv = center[ord[k]].z;
+            for (int i = k + 1; i < ng; i++)
+                if (center[ord[i]].z < mv) {
+                    mv = center[ord[i]].z;
+                    mi = i;
+                }
+            std::swap(ord[k], ord[mi]);
+        }
     }
     m.ncl = ng;
     m.cl_start.assign(1, 0);
     m.cl_atoms.clear();
     m.cl_center.clear();
     m.cl_maxdist.clear();
-    for (int i = 0; i < ng; i++) {
+    for (int k = 0; k < ng; k++) {
+        const int i = ord[k];
         m.cl_atoms.insert(m.cl_atoms.end(), g[i].begin(), g[i].end());
         m.cl_start.push_back((int32_t)m.cl_atoms.size());
-        m.cl_center.push_back(cp.center[i].x);
-        m.cl_center.push_back(cp.center[i].y);
-        m.cl_center.push_back(cp.center[i].z);
-        m.cl_maxdist.push_back(cp.maxd[i]);
+        m.cl_center.push_back(center[i].x);
+        m.cl_center.push_back(center[i].y);
+        m.cl_center.push_back(center[i].z);
+        m.cl_maxdist.push_back(maxd[i]);
     }
     return 0;
 }
@@ -736,7 +800,9 @@ int complete_prepared(const Params &p, Prepared &m, std::string &err)
     m.vd = velocity_distribution(m.mu, p.temp);
     m.bmax = std::max(m.a, std::max(m.b, m.c));
     m.ccs_const = m.vd.const1 * (m.bmax * m.bmax);
+    const auto t0 = std::chrono::steady_clock::now();
     if (m.ncl > 0) cluster_moments(m);
+    m.moments_ms = ms_since(t0);
     return 0;
 }
 
@@ -790,7 +856,9 @@ int prepare(const Params &p, int n, const double *xyz, const double *mass, const
     m.dipole_scalar = kAlphaKe / m.mu;
     m.vd = velocity_distribution(m.mu, p.temp);
     std::vector<V3> pts;
+    auto t0 = std::chrono::steady_clock::now();
     rc = boundary_points(m, m.vd.min_kin, p.threads, bm, pts, err);
+    m.boundary_ms = ms_since(t0);
     if (rc) return rc;
     if (pts.size() < 4) {
         err = "potential boundary search found too few points";
@@ -800,7 +868,9 @@ int prepare(const Params &p, int n, const double *xyz, const double *mass, const
     fit_ellipsoid(pts, m);
     enlarge(pts, m);
     if (p.multipole || p.dispersion_cutoff) {
-        rc = form_clusters(m, p.max_cluster_size, err);
+        t0 = std::chrono::steady_clock::now();
+        rc = form_clusters(m, p.max_cluster_size, p.threads, err);
+        m.cluster_ms = ms_since(t0);
         if (rc) return rc;
     }
     return complete_prepared(p, m, err);

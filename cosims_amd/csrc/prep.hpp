@@ -78,6 +78,9 @@ struct Prepared {
     int ncl = 0;
     std::vector<int32_t> cl_start, cl_atoms, cl_replace;
     std::vector<double> cl_center, cl_maxdist, cl_pos, cl_q, cl_dip, cl_quad;
+    // wall time of the setup phases (prepare): boundary search, PCM
+    // clustering, cluster moments
+    double boundary_ms = 0, cluster_ms = 0, moments_ms = 0;
 };
 
 int force_type_of(bool charged, bool multipole, bool cutoff);

@@ -129,6 +129,7 @@ struct SJob {
     int inflight = 0;                   // ... until it has retired and no slice in flight holds it
     bool released = false;
     double t_submit = 0, t_ready = 0, t_admit = 0, t_done = 0, setup_ms = 0;
+    double setup_phase_ms[4] = {0, 0, 0, 0};  // boundary marches, PCM clustering, multipoles, upload
     cosims_ccs ccs{};
     ~SJob()
     {
@@ -186,7 +187,9 @@ void setup_worker(cosims_server *S)
         a.sigma_max = J->sigma_max;
         cosims_mol *m = nullptr;
         int rc = cosims_mol_create(&J->params, &a, &m);
+        const double t_up = S->now_ms();
         if (rc == 0) rc = ensure_device(m);
+        const double upload_ms = S->now_ms() - t_up;
         std::string msg = rc ? g_err : std::string();
         {
             std::lock_guard<std::mutex> lk(S->mu);
@@ -203,6 +206,10 @@ void setup_worker(cosims_server *S)
                 S->done_cv.notify_all();
             } else {
                 J->mol = m;
+                J->setup_phase_ms[0] = m->prep.boundary_ms;
+                J->setup_phase_ms[1] = m->prep.cluster_ms;
+                J->setup_phase_ms[2] = m->prep.moments_ms;
+                J->setup_phase_ms[3] = upload_ms;
                 // the molecule and its per-block sums; the slices' trajectory
                 // buffers are reserved once, outside the budget
                 J->bytes = m->dbytes + sizeof(double) * kSums * S->cfg.max_blocks;
@@ -366,9 +373,9 @@ int issue_slice(cosims_server *S, Slice &X)
         SJob &J = *e.job;
         if (e.it0 + e.nb > c.max_blocks) return fail(COSIMS_E_INVALID, "server: block count beyond max_blocks");
         if (!J.dsums.p) {
-            if (J.dsums.alloc(sizeof(double) * kSums * c.max_blocks) != hipSuccess ||
-                J.dblk.alloc(sizeof(double) * 2 * c.max_blocks) != hipSuccess ||
-                J.dstate.alloc(sizeof(ReqDev)) != hipSuccess)
+            if (J.dsums.alloc_async(sizeof(double) * kSums * c.max_blocks, X.s) != hipSuccess ||
+                J.dblk.alloc_async(sizeof(double) * 2 * c.max_blocks, X.s) != hipSuccess ||
+                J.dstate.alloc_async(sizeof(ReqDev), X.s) != hipSuccess)
                 return fail(COSIMS_E_NOMEM, "server: per-request device state");
             J.init = ReqDev{0, 0, c.min_blocks, c.max_blocks, J.target, 0.0, 0.0};
             HIP_CHECK(hipMemcpyAsync(J.dstate.p, &J.init, sizeof(ReqDev), hipMemcpyHostToDevice, X.s));
@@ -760,6 +767,10 @@ int cosims_server_wait(cosims_server *S, int64_t ticket, double timeout_ms, cosi
     out->t_done_ms = J->t_done;
     out->setup_ms = J->setup_ms;
     out->block_traj = S->cfg.block_traj;
+    out->setup_boundary_ms = J->setup_phase_ms[0];
+    out->setup_cluster_ms = J->setup_phase_ms[1];
+    out->setup_moments_ms = J->setup_phase_ms[2];
+    out->setup_upload_ms = J->setup_phase_ms[3];
     for (int k = 0; k < std::min<int>(cap, J->used); k++) {
         if (omega) omega[k] = J->omega[k];
         if (omega_err) omega_err[k] = J->omega_err[k];

@@ -85,7 +85,9 @@ class MolInfo(C.Structure):
                                            "vd_norm", "max_val", "cdf_frac", "ccs_const", "min_kin",
                                            "dispersion_radius", "multipole_radius")] + \
                [("rot", C.c_double * 9), ("max_xyz", C.c_double * 3), ("fit_abc", C.c_double * 3),
-                ("fit_rmsd", C.c_double), ("n_boundary", C.c_int32), ("setup_ms", C.c_double)]
+                ("fit_rmsd", C.c_double), ("n_boundary", C.c_int32), ("setup_ms", C.c_double),
+                ("setup_boundary_ms", C.c_double), ("setup_cluster_ms", C.c_double),
+                ("setup_moments_ms", C.c_double)]
 
 
 class RunCfg(C.Structure):
@@ -144,7 +146,9 @@ class Request(C.Structure):
 class Result(C.Structure):
     _fields_ = [("status", C.c_int32), ("blocks", C.c_int32), ("ccs", Ccs), ("rel_err", C.c_double),
                 ("t_submit_ms", C.c_double), ("t_ready_ms", C.c_double), ("t_admit_ms", C.c_double),
-                ("t_done_ms", C.c_double), ("setup_ms", C.c_double), ("block_traj", C.c_int64)]
+                ("t_done_ms", C.c_double), ("setup_ms", C.c_double), ("block_traj", C.c_int64),
+                ("setup_boundary_ms", C.c_double), ("setup_cluster_ms", C.c_double),
+                ("setup_moments_ms", C.c_double), ("setup_upload_ms", C.c_double)]
 
 
 class ServerStats(C.Structure):

@@ -157,6 +157,10 @@ typedef struct cosims_mol_info {
     double fit_abc[3], fit_rmsd; /* least-squares axes before enlargement */
     int32_t n_boundary;
     double setup_ms;
+    /* wall time of setup phases within setup_ms: the boundary search's
+     * potential marches, PCM clustering (params.threads host threads), the
+     * cluster multipoles */
+    double setup_boundary_ms, setup_cluster_ms, setup_moments_ms;
 } cosims_mol_info;
 int cosims_mol_info_get(const cosims_mol *m, cosims_mol_info *out);
 /* Prepared atoms (n each, xyz 3n) and clusters; any pointer may be NULL. */
@@ -404,6 +408,9 @@ typedef struct cosims_result {
     double t_submit_ms, t_ready_ms, t_admit_ms, t_done_ms;  /* server clock */
     double setup_ms;
     int64_t block_traj;
+    /* setup phases within setup_ms (as cosims_mol_info): boundary marches,
+     * PCM clustering, cluster multipoles; then the molecule's device upload */
+    double setup_boundary_ms, setup_cluster_ms, setup_moments_ms, setup_upload_ms;
 } cosims_result;
 
 typedef struct cosims_server_stats {

@@ -66,6 +66,10 @@ def test_server_requests_finish_at_target(mixed_results):
         assert r["rel_err"] <= t or r["blocks"] == 40, (f, r["rel_err"])
         assert r["t_submit_ms"] <= r["t_ready_ms"] <= r["t_admit_ms"] <= r["t_done_ms"]
         assert r["n_traj"] == r["blocks"] * BLOCK
+        # setup phases: the boundary marches always, PCM clustering for the clustered request only
+        phases = r["setup_boundary_ms"] + r["setup_cluster_ms"] + r["setup_moments_ms"] + r["setup_upload_ms"]
+        assert 0 < r["setup_boundary_ms"] and phases <= r["setup_ms"], f
+        assert (r["setup_cluster_ms"] > 0) == (p.get("multipole", 1) == 1), f
     assert st["slices"] >= 1 and st["traj"] >= sum(r["n_traj"] for r in res)
 
 

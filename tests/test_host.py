@@ -145,3 +145,26 @@ def test_reference_stream_matches_setUpTraj():
     assert np.array_equal(r["v0"][first][hf], np.array(s["v0_first"]).reshape(-1, 3)[hf])
     assert [math.fsum(r["p0"][h][:, k]) for k in range(3)] == s["hit_p0_fsum"]
     assert [math.fsum(r["v0"][h][:, k]) for k in range(3)] == s["hit_v0_fsum"]
+
+
+def test_pcm_threaded_splits_match_serial_oracle(tmp_path):
+    """PCM clustering of a 2,400-atom synthetic ion (radius 19 A), large
+    enough that the engine splits the clusters of each pass on 8 host
+    threads: membership, order, centres and extents equal the oracle's
+    serial restatement of PCM_Cluster::clusterAtoms (src/PCM_Cluster.cpp:
+    38-249) on the same prepared coordinates, and the phase is timed."""
+    from cosims_amd import synth
+    path = str(tmp_path / "ion2400.mfj")
+    synth.write_mfj(path, "ion2400", synth.synthetic_ion(2400, 19.0, 0.3, 8.0, seed=2401))
+    p = E.default_params(multipole=1, dispersion_cutoff=1, setup_gpu=0)
+    p.threads = 8
+    m = E.Molecule.from_file(path, params=p)
+    inf = m.info()
+    assert inf["ncl"] > 100 and inf["setup_cluster_ms"] > 0
+    assert inf["setup_boundary_ms"] + inf["setup_cluster_ms"] <= inf["setup_ms"]
+    cl, A = m.clusters(), m.atoms()
+    o = oracle.pcm_clusters(A["xyz"], A["q"], p.max_cluster_size)
+    assert np.array_equal(cl["start"], o["start"])
+    assert np.array_equal(cl["atoms"], o["atoms"])
+    assert np.array_equal(cl["center"], o["center"])
+    assert np.array_equal(cl["maxdist"], o["rad"])
