@@ -1776,6 +1776,14 @@ int group_units(const Prepared &p)
     return (ft == 4 || ft == 6) ? p.n : p.ncl;
 }
 
+// The clustered force types run one group per wavefront (64 lanes: far
+// clusters by lane, near and LJ clusters from ballot-built lists), whatever
+// their cluster count: with fewer lanes a trajectory each lane steps through
+// the branches of its own clusters, 4-14x slower on molecules of 300-5,000
+// atoms (38-671 clusters).  Their size rule, tuner and batch classes give
+// them 64 lanes.
+bool clustered_type(int ft) { return ft == 1 || ft == 2 || ft == 3 || ft == 5; }
+
 struct GroupRun {
     double ms = 0;
     int64_t steps = 0, evals = 0, pots = 0, pairs = 0;
@@ -2088,9 +2096,10 @@ void finish_ccs(const int8_t *status, const double *sums, int iters, int64_t T, 
 // (the faster run counts: launch times vary by a few per cent) on the large
 // set, where the end-of-launch tail weighs as in the real run (fewer lanes
 // per trajectory mean slower trajectories, hence longer tails, which a small
-// set over-weights).  run(set, L, W, ms) times one launch.
+// set over-weights).  run(set, L, W, ms) times one launch.  only64: the
+// clustered types, whose lane count is 64 (clustered_type): W alone.
 template <class RunFn>
-int tune_shapes(int units, RunFn run, int &L_out, int &W_out)
+int tune_shapes(int units, RunFn run, int &L_out, int &W_out, bool only64 = false)
 {
     struct Cand {
         int L, W;
@@ -2107,7 +2116,7 @@ int tune_shapes(int units, RunFn run, int &L_out, int &W_out)
         std::vector<std::pair<int, double>> sc;
         double best = 0;
         for (int L : Ls) {
-            if (L > 4 && units < 8 * L) continue;  // too few atoms (clusters) per lane
+            if (only64 ? L != 64 : (L > 4 && units < 8 * L)) continue;  // too few atoms per lane
             double ms = 0;
             if (int rc = run(2, L, 512, ms)) return rc;
             sc.push_back({L, ms});
@@ -2213,7 +2222,8 @@ int autotune(const cosims_mol *M, int prec, int64_t n_hint, int &L_out, int &wg_
         ms = R.ms;
         return 0;
     };
-    if (int rc = tune_shapes(group_units(M->prep), run, L_out, wg_out)) return rc;
+    if (int rc = tune_shapes(group_units(M->prep), run, L_out, wg_out, clustered_type(M->prep.force_type)))
+        return rc;
     M->tuned[key] = {L_out, wg_out};
     return 0;
 }
@@ -2243,6 +2253,20 @@ int size_class(int units)
     int k = 0;
     for (int L = lanes_rule(units); L > 4; L >>= 1) k++;
     return k;
+}
+
+// The clustered force types take the 64-lane size class (clustered_type).
+int job_class(const Prepared &p)
+{
+    return clustered_type(p.force_type) ? COSIMS_SIZE_CLASSES - 1 : size_class(group_units(p));
+}
+// Lanes of a job without a fixed lane count: 64 for the clustered types,
+// else its class's tuned lanes, else the size rule.
+int job_lanes(const Prepared &p, const int *class_lanes)
+{
+    if (clustered_type(p.force_type)) return 64;
+    const int c = size_class(group_units(p));
+    return class_lanes[c] > 0 ? class_lanes[c] : lanes_rule(group_units(p));
 }
 
 // A batch laid out on the device: per-trajectory buffers of all jobs back
@@ -2330,8 +2354,8 @@ struct ShapeTable {
     int class_wg[COSIMS_SIZE_CLASSES] = {0};
     void apply(BatchPlan::Job &b) const
     {
-        const int c = size_class(b.units);
-        b.L = lanes > 0 ? lanes : (class_lanes[c] ? class_lanes[c] : lanes_rule(b.units));
+        const int c = job_class(b.mol->prep);
+        b.L = lanes > 0 ? lanes : job_lanes(b.mol->prep, class_lanes);
         b.W = (lanes <= 0 && class_wg[c]) ? class_wg[c] : wg;
         if (b.W % b.L) b.W = std::max(b.W, b.L);
     }
@@ -2749,22 +2773,29 @@ int tune_batch(const cosims_batch_job *jobs, int32_t njobs, int prec, ShapeTable
     std::vector<cosims_batch_job> by_class[COSIMS_SIZE_CLASSES];
     for (int k = 0; k < njobs; k++) {
         if (!jobs[k].mol) return fail(COSIMS_E_INVALID, "cosims_tune_batch: job without molecule");
-        by_class[size_class(group_units(jobs[k].mol->prep))].push_back(jobs[k]);
+        by_class[job_class(jobs[k].mol->prep)].push_back(jobs[k]);
     }
     out.lanes = 0;
     for (int c = 0; c < COSIMS_SIZE_CLASSES; c++) {
         if (by_class[c].empty()) continue;
+        // the class's exact-type jobs decide its shape (its clustered jobs run 64 lanes
+        // whatever the table says); a class of clustered jobs alone tunes W at 64 lanes
+        std::vector<cosims_batch_job> exact;
+        for (const auto &j : by_class[c])
+            if (!clustered_type(j.mol->prep.force_type)) exact.push_back(j);
+        const bool only64 = exact.empty();
+        const std::vector<cosims_batch_job> &pool = only64 ? by_class[c] : exact;
         int units = 0;  // largest molecule of the class: the lane counts worth trying
-        for (const auto &j : by_class[c]) units = std::max(units, group_units(j.mol->prep));
+        for (const auto &j : pool) units = std::max(units, group_units(j.mol->prep));
         // tuning subsets: at most 8 jobs spread over the class, each with 1/8 (search), 1/2
         // (finalists) and 1/32 (screen) of the fewest trajectories a job of the subset has.
         // Jobs that fit LDS run one launch each, so the sets must be launch-sized like the
         // batch's: on small sets the end-of-launch tail would decide the shape.
-        const size_t m = by_class[c].size(), take = std::min<size_t>(8, m);
+        const size_t m = pool.size(), take = std::min<size_t>(8, m);
         std::vector<cosims_batch_job> sub;
         int64_t T = INT64_MAX;
         for (size_t i = 0; i < take; i++) {
-            const cosims_batch_job &j = by_class[c][i * m / take];
+            const cosims_batch_job &j = pool[i * m / take];
             sub.push_back(j);
             const int64_t it = j.iter > 0 ? j.iter : j.mol->params.iter;
             const int64_t tr = j.traj > 0 ? j.traj : j.mol->params.traj;
@@ -2788,7 +2819,7 @@ int tune_batch(const cosims_batch_job *jobs, int32_t njobs, int prec, ShapeTable
             return 0;
         };
         int L = 0, W = 0;
-        if (int rc = tune_shapes(units, run, L, W)) return rc;
+        if (int rc = tune_shapes(units, run, L, W, only64)) return rc;
         out.class_lanes[c] = L;
         out.class_wg[c] = W;
     }

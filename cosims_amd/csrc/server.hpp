@@ -391,8 +391,8 @@ int issue_slice(cosims_server *S, Slice &X)
         b.ft = J.mol->prep.force_type;
         b.units = group_units(J.mol->prep);
         b.it0 = (uint32_t)e.it0;
-        const int k = size_class(b.units);
-        b.L = c.lanes > 0 ? c.lanes : (c.class_lanes[k] > 0 ? c.class_lanes[k] : lanes_rule(b.units));
+        const int k = job_class(J.mol->prep);
+        b.L = c.lanes > 0 ? c.lanes : job_lanes(J.mol->prep, c.class_lanes);
         b.W = std::max(c.lanes <= 0 && c.class_wg[k] > 0 ? c.class_wg[k] : c.wg_size, b.L);
         P.n += b.n;
         P.jobs.push_back(b);

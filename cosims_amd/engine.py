@@ -454,11 +454,26 @@ def _batch_jobs(jobs):
 
 
 def size_class(units):
-    """Size class of a molecule whose force loop sweeps `units` atoms (or clusters)."""
+    """Size class of an exact-type molecule whose force loop sweeps `units` atoms."""
     L, k = 4, 0
     while L < 64 and units >= 48 * L:
         L, k = 2 * L, k + 1
     return k
+
+
+def job_class(info):
+    """Size class of a molecule (Molecule.info()): the clustered force types
+    take the 64-lane class."""
+    return SIZE_CLASSES - 1 if info["force_type"] in (1, 2, 3, 5) else size_class(info["n"])
+
+
+def job_lanes(info, class_lanes=None):
+    """Lanes a job runs without a fixed lane count: 64 for the clustered
+    types, else its class's entry of class_lanes, else the size rule."""
+    if info["force_type"] in (1, 2, 3, 5):
+        return 64
+    k = size_class(info["n"])
+    return (class_lanes or [0] * SIZE_CLASSES)[k] or 4 << k
 
 
 def tune_batch(jobs, prec=PREC_MIXED):

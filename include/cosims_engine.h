@@ -292,10 +292,13 @@ int cosims_run_noalloc(const cosims_mol *m, const cosims_params *p, const cosims
  * same molecule, seed, job id and launch shape, in either mode.
  *
  * Lanes per trajectory follow the molecule's size: size class k holds the
- * molecules whose force loop sweeps n units (atoms, or clusters for the
- * clustered types) with 48 * 4 * 2^(k-1) <= n < 48 * 4 * 2^k (class 0
- * below 192, class 4 from 1536 up); the size rule gives class k 4 * 2^k
- * lanes, the tuner times each class's choices. */
+ * exact-type molecules of n atoms with 48 * 4 * 2^(k-1) <= n < 48 * 4 * 2^k
+ * (class 0 below 192, class 4 from 1536 up); the size rule gives class k
+ * 4 * 2^k lanes, the tuner times each class's choices.  Molecules of the
+ * clustered types (1, 2, 3, 5) are in class 4 and run 64 lanes (one group a
+ * wavefront, the cluster-list kernel) unless `lanes` fixes every job: with
+ * fewer lanes they run 4-14x slower at 300-5,000 atoms.  Only the class's
+ * workgroup size applies to them. */
 #define COSIMS_SIZE_CLASSES 5
 
 typedef struct cosims_batch_job {

@@ -190,8 +190,8 @@ def test_server_size_rule_equals_run():
     for (f, p, j, t), r in zip(REQUESTS, res):
         m = E.Molecule.from_file(os.path.join(GOLDEN, f), params=_params(100 + j, **p))
         inf = m.info()
-        units = inf["n"] if inf["force_type"] in (4, 6) else inf["ncl"]
-        L = 4 << E.size_class(units)
+        L = E.job_lanes(inf)
+        assert L == 64 or inf["force_type"] in (4, 6)  # the clustered types run 64 lanes
         run = m.run(E.PREC_MIXED, job=j, iters=r["blocks"], traj=BLOCK, lanes=L, wg_size=512)
         assert np.array_equal(r["omega"], run["omega"]), (f, L)
 
@@ -207,8 +207,8 @@ def test_server_class_shapes_equal_run():
     for (f, p, j, t), r in zip(REQUESTS, res):
         m = E.Molecule.from_file(os.path.join(GOLDEN, f), params=_params(100 + j, **p))
         inf = m.info()
-        k = E.size_class(inf["n"] if inf["force_type"] in (4, 6) else inf["ncl"])
-        L = CL[k] or (4 << k)
+        k = E.job_class(inf)
+        L = E.job_lanes(inf, CL)
         W = max(CW[k] or 256, L)
         assert (r["lanes"], r["wg_size"]) == (L, W), (f, k)
         run = m.run(E.PREC_MIXED, job=j, iters=r["blocks"], traj=BLOCK, lanes=L, wg_size=W)
