@@ -2842,7 +2842,7 @@ static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
 
 namespace {
 
-constexpr int kNoAllocDevices = 64;
+constexpr int kMaxDevices = 64;
 
 struct ScratchDev {
     std::once_flag once;
@@ -2850,13 +2850,13 @@ struct ScratchDev {
     uint8_t *ptr = nullptr;  // device address of this device's copy of scratch
     std::mutex mu;           // calls on one device share its copy: they run one at a time
 };
-ScratchDev g_scratch[kNoAllocDevices];
+ScratchDev g_scratch[kMaxDevices];
 
 // The scratch of device dev (the current device), its address looked up
 // once, on the first call on that device.
 int scratch_of(int dev, ScratchDev *&out)
 {
-    if (dev < 0 || dev >= kNoAllocDevices) return fail(COSIMS_E_INVALID, "cosims_run_noalloc: device index too large");
+    if (dev < 0 || dev >= kMaxDevices) return fail(COSIMS_E_INVALID, "cosims_run_noalloc: device index too large");
     ScratchDev &S = g_scratch[dev];
     std::call_once(S.once, [&S] {
         void *p = nullptr;
@@ -2867,6 +2867,80 @@ int scratch_of(int dev, ScratchDev *&out)
         return fail(COSIMS_E_HIP, std::string("cosims_run_noalloc: hipGetSymbolAddress(scratch): ") +
                                       hipGetErrorString(S.rc));
     out = &S;
+    return 0;
+}
+
+// Trajectory buffers and stream cosims_run keeps between calls on a device
+// (sized for the largest run so far): a call takes them when no other call
+// holds them, else allocates its own.  Every buffer a run reads it writes
+// first, so nothing of the previous run shows through.
+struct RunPool {
+    std::mutex mu;
+    bool busy = false;
+    TrajBufs B;
+    int64_t cap = 0;               // trajectories B holds
+    DBuf aux;                      // per-integral sums, CCS and error
+    int cap_iters = 0;
+    Stream st;
+};
+// never destroyed: a static destructor's HIP calls would run after the
+// runtime has shut down at process exit
+RunPool *const g_run_pool = new RunPool[kMaxDevices];
+
+struct RunLease {
+    RunPool *pool = nullptr;       // the device's pool, or
+    TrajBufs own;                  // buffers of this call alone
+    DBuf own_aux;
+    Stream own_st;
+    TrajBufs *B = nullptr;
+    double *aux = nullptr;
+    hipStream_t s = nullptr;
+    RunLease() = default;
+    RunLease(const RunLease &) = delete;
+    ~RunLease()
+    {
+        if (pool) {
+            std::lock_guard<std::mutex> lk(pool->mu);
+            pool->busy = false;
+        }
+    }
+};
+
+// Buffers for n trajectories in iters integrals on device dev (current).
+int lease_run(int dev, int64_t n, int iters, RunLease &L)
+{
+    RunPool *P = (dev >= 0 && dev < kMaxDevices) ? g_run_pool + dev : nullptr;
+    if (P) {
+        std::lock_guard<std::mutex> lk(P->mu);
+        if (!P->busy) {
+            P->busy = true;
+            L.pool = P;
+        }
+    }
+    const size_t aux_bytes = sizeof(double) * (kSums + 2) * (size_t)iters;
+    if (L.pool) {
+        if (P->cap < n) {
+            P->cap = 0;
+            if (int rc = P->B.alloc(n)) return rc;
+            P->cap = n;
+        }
+        if (P->cap_iters < iters) {
+            P->cap_iters = 0;
+            HIP_CHECK(P->aux.alloc(aux_bytes));
+            P->cap_iters = iters;
+        }
+        if (!P->st.s) HIP_CHECK(hipStreamCreateWithFlags(&P->st.s, hipStreamNonBlocking));
+        L.B = &P->B;
+        L.aux = P->aux.as<double>();
+        L.s = P->st.s;
+        return 0;
+    }
+    if (int rc = L.own.alloc(n)) return rc;
+    HIP_CHECK(L.own_aux.alloc(aux_bytes));
+    HIP_CHECK(hipStreamCreateWithFlags(&L.own_st.s, hipStreamNonBlocking));
+    L.B = &L.own;
+    L.aux = L.own_aux.as<double>();
+    L.s = L.own_st.s;
     return 0;
 }
 
@@ -2990,11 +3064,13 @@ int cosims_run(const cosims_mol *M, const cosims_params *P, const cosims_run_cfg
     if (wg % 64 || wg > (L == 1 ? 256 : 1024) || wg % L)
         return fail(COSIMS_E_INVALID, "wg_size must be a multiple of 64 (at most 256 lane-mode, 1024 group-mode)");
     auto t0 = std::chrono::steady_clock::now();
-    Stream st;
-    HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
-    TrajBufs B;
-    int rc = B.alloc(n);
+    int dev = -1;
+    HIP_CHECK(hipGetDevice(&dev));
+    RunLease lease;
+    int rc = lease_run(dev, n, iters, lease);
     if (rc) return rc;
+    TrajBufs &B = *lease.B;
+    const hipStream_t s = lease.s;
     float sms = 0;
     if (c.rng == COSIMS_RNG_REFERENCE) {
         // the reference's stream, drawn on the host and uploaded
@@ -3009,26 +3085,26 @@ int cosims_run(const cosims_mol *M, const cosims_params *P, const cosims_run_cfg
         }
         int32_t cnt[64] = {0};
         cnt[0] = (int32_t)items.size();
-        HIP_CHECK(hipMemcpyAsync(B.p0.p, hp0.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, st.s));
-        HIP_CHECK(hipMemcpyAsync(B.v0.p, hv0.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, st.s));
-        HIP_CHECK(hipMemcpyAsync(B.g.p, hg.data(), sizeof(double) * n, hipMemcpyHostToDevice, st.s));
-        HIP_CHECK(hipMemcpyAsync(B.status.p, s8.data(), n, hipMemcpyHostToDevice, st.s));
+        HIP_CHECK(hipMemcpyAsync(B.p0.p, hp0.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(B.v0.p, hv0.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(B.g.p, hg.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(B.status.p, s8.data(), n, hipMemcpyHostToDevice, s));
         if (!items.empty())
             HIP_CHECK(hipMemcpyAsync(B.items[0].p, items.data(), sizeof(int32_t) * items.size(),
-                                     hipMemcpyHostToDevice, st.s));
-        HIP_CHECK(hipMemcpyAsync(B.counters.p, cnt, sizeof(cnt), hipMemcpyHostToDevice, st.s));
-        HIP_CHECK(hipStreamSynchronize(st.s));
+                                     hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(B.counters.p, cnt, sizeof(cnt), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipStreamSynchronize(s));
     } else if (c.rng == COSIMS_RNG_PHILOX) {
-        if ((rc = sample_into(M, P->seed, c.job, n, T, B, st.s, &sms))) return rc;
+        if ((rc = sample_into(M, P->seed, c.job, n, T, B, s, &sms))) return rc;
     } else {
         return fail(COSIMS_E_INVALID, "unknown rng");
     }
     int32_t hits = 0;
-    HIP_CHECK(hipMemcpyAsync(&hits, B.counters.p, sizeof(int32_t), hipMemcpyDeviceToHost, st.s));
-    HIP_CHECK(hipStreamSynchronize(st.s));
+    HIP_CHECK(hipMemcpyAsync(&hits, B.counters.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
     if (L == 1) {
         RoundStats R;
-        rc = run_rounds(M, prec, n, B, st.s, wg, c.blocks_per_cu, R);
+        rc = run_rounds(M, prec, n, B, s, wg, c.blocks_per_cu, R);
         if (rc) return rc;
         out->n_retry = R.retries;
         out->n_rounds = R.rounds;
@@ -3045,11 +3121,11 @@ int cosims_run(const cosims_mol *M, const cosims_params *P, const cosims_run_cfg
         if (kOrderWork && c.rng == COSIMS_RNG_PHILOX) {
             work = B.items[1].as<int32_t>();
             if ((rc = order_items(B.items[0].as<int32_t>(), B.items[1].as<int32_t>(), nullptr, nullptr, hits,
-                                  B.bin.as<uint8_t>(), B.order.as<int32_t>(), st.s)))
+                                  B.bin.as<uint8_t>(), B.order.as<int32_t>(), s)))
                 return rc;
         }
         GroupRun G;
-        rc = run_group(M, prec, hits, B, st.s, L, wg, c.blocks_per_cu, G, work);
+        rc = run_group(M, prec, hits, B, s, L, wg, c.blocks_per_cu, G, work);
         if (rc) return rc;
         out->n_rounds = 1;
         out->traj_kernel_ms[0] = G.ms;
@@ -3064,24 +3140,32 @@ int cosims_run(const cosims_mol *M, const cosims_params *P, const cosims_run_cfg
     out->lanes = L;
     out->wg_size = wg;
     out->sample_kernel_ms = sms;
-    DBuf red;
-    HIP_CHECK(red.alloc(sizeof(double) * 6 * iters));
+    // per-integral sums (k_reduce), then each integral's CCS and error with
+    // the reference's denominator (k_iter_ccs): only iters values come back
+    double *red = lease.aux, *om = red + kSums * iters, *er = om + iters;
     ReduceArgs RA{};
     RA.status = B.status.as<int8_t>(); RA.chi = B.chi.as<double>(); RA.ergy = B.ergy.as<double>();
-    RA.totsteps = B.tot.as<int32_t>(); RA.per_iter = T; RA.ccs_const = M->prep.ccs_const; RA.out = red.as<double>();
-    hipLaunchKernelGGL(k_reduce, dim3(iters), dim3(256), 0, st.s, RA);
+    RA.totsteps = B.tot.as<int32_t>(); RA.per_iter = T; RA.ccs_const = M->prep.ccs_const; RA.out = red;
+    hipLaunchKernelGGL(k_reduce, dim3(iters), dim3(256), 0, s, RA);
     HIP_CHECK(hipGetLastError());
-    std::vector<double> sums(6 * iters);
-    std::vector<int8_t> status(n);
-    HIP_CHECK(hipMemcpyAsync(sums.data(), red.p, sizeof(double) * 6 * iters, hipMemcpyDeviceToHost, st.s));
-    HIP_CHECK(hipMemcpyAsync(status.data(), B.status.p, n, hipMemcpyDeviceToHost, st.s));
-    if (c.chi) HIP_CHECK(hipMemcpyAsync(c.chi, B.chi.p, sizeof(double) * n, hipMemcpyDeviceToHost, st.s));
-    if (c.g) HIP_CHECK(hipMemcpyAsync(c.g, B.g.p, sizeof(double) * n, hipMemcpyDeviceToHost, st.s));
-    HIP_CHECK(hipStreamSynchronize(st.s));
+    const int64_t pr = P->print_rate;
+    const int upd = (int)(pr > 0 ? std::min<int64_t>(pr, INT32_MAX) : std::max<int64_t>(1, (int64_t)(T / 5.0)));
+    hipLaunchKernelGGL(k_iter_ccs, dim3(iters), dim3(256), 0, s, (const int8_t *)B.status.p, (const double *)red,
+                       (int)T, upd, om, er);
+    HIP_CHECK(hipGetLastError());
+    std::vector<double> sums(kSums * iters);
+    std::vector<int8_t> status(c.status ? n : 0);
+    HIP_CHECK(hipMemcpyAsync(sums.data(), red, sizeof(double) * kSums * iters, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(omega, om, sizeof(double) * iters, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(omega_err, er, sizeof(double) * iters, hipMemcpyDeviceToHost, s));
+    if (c.status) HIP_CHECK(hipMemcpyAsync(status.data(), B.status.p, n, hipMemcpyDeviceToHost, s));
+    if (c.chi) HIP_CHECK(hipMemcpyAsync(c.chi, B.chi.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    if (c.g) HIP_CHECK(hipMemcpyAsync(c.g, B.g.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
     if (c.status)
         for (int64_t i = 0; i < n; i++)
             c.status[i] = status[i] == ST_OK ? 1 : (status[i] == ST_MISS ? 0 : 2);
-    finish_ccs(status.data(), sums.data(), iters, T, P->print_rate, omega, omega_err, out);
+    final_stats(omega, omega_err, sums.data(), iters, T, out);
     out->n_traj = n;
     const int ft = M->prep.force_type;
     // clustered types: the member atoms their branches evaluated
