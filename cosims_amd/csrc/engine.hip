@@ -556,27 +556,72 @@ int device_init(Device &d)
     return 0;
 }
 
+// Device blocks of the short-lived buffers of molecule setup (boundary
+// marches) and of served requests: taken with hipMalloc (sizes rounded up to
+// a power of two) and handed back to this cache instead of hipFree, which
+// waits for the whole device (a setup or a retiring request beside a
+// persistent serving slice would otherwise hold its thread until the slice
+// ends).  Blocks are reused only after their owner has seen its last use
+// complete (a host synchronisation), so any stream may use them next; they
+// stay allocated until the process ends.
+struct DevCache {
+    std::mutex mu;
+    std::multimap<std::pair<int, size_t>, void *> blocks;  // (device, bytes) -> free block
+};
+DevCache &dev_cache()
+{
+    static DevCache *c = new DevCache();  // never destroyed: blocks outlive static destructors
+    return *c;
+}
+hipError_t dev_cache_get(size_t bytes, int *dev, size_t *cap, void **p)
+{
+    size_t sz = 256;
+    while (sz < bytes) sz <<= 1;
+    if (hipGetDevice(dev) != hipSuccess) *dev = 0;
+    DevCache &c = dev_cache();
+    {
+        std::lock_guard<std::mutex> g(c.mu);
+        auto it = c.blocks.find({*dev, sz});
+        if (it != c.blocks.end()) {
+            *p = it->second;
+            c.blocks.erase(it);
+            *cap = sz;
+            return hipSuccess;
+        }
+    }
+    const hipError_t e = hipMalloc(p, sz);
+    if (e == hipSuccess) *cap = sz;
+    return e;
+}
+void dev_cache_put(int dev, size_t cap, void *p)
+{
+    DevCache &c = dev_cache();
+    std::lock_guard<std::mutex> g(c.mu);
+    c.blocks.insert({{dev, cap}, p});
+}
+
 // RAII device buffer; alloc() replaces any previous allocation.  borrow()
 // points it at memory it does not own (a slice of the static scratch).
-// alloc_async() takes stream-ordered memory (hipMallocAsync on stream s),
-// given back with hipFreeAsync on the null stream: hipFree waits for the
-// whole device, so a buffer freed while persistent launches run on other
-// streams (molecule setup and request bookkeeping beside a serving slice)
-// would hold its thread until they end.  The owner frees it only after its
-// last use has completed.
+// alloc_cached() takes a block of the device cache (DevCache) and gives it
+// back there; the owner releases it only after its last use has completed.
 struct DBuf {
     void *p = nullptr;
-    bool owned = false, async = false;
+    bool owned = false, cached = false;
+    int dev = 0;
+    size_t cap = 0;
     DBuf() = default;
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
     ~DBuf() { reset(); }
     void reset()
     {
-        if (p && owned) (void)(async ? hipFreeAsync(p, nullptr) : hipFree(p));
+        if (p && owned) {
+            if (cached) dev_cache_put(dev, cap, p);
+            else (void)hipFree(p);
+        }
         p = nullptr;
         owned = false;
-        async = false;
+        cached = false;
     }
     hipError_t alloc(size_t bytes)
     {
@@ -586,11 +631,11 @@ struct DBuf {
         if (!owned) p = nullptr;
         return e;
     }
-    hipError_t alloc_async(size_t bytes, hipStream_t s)
+    hipError_t alloc_cached(size_t bytes)
     {
         reset();
-        const hipError_t e = hipMallocAsync(&p, bytes ? bytes : 16, s);
-        owned = async = e == hipSuccess;
+        const hipError_t e = dev_cache_get(bytes ? bytes : 16, &dev, &cap, &p);
+        owned = cached = e == hipSuccess;
         if (!owned) p = nullptr;
         return e;
     }
@@ -749,8 +794,8 @@ __global__ void __launch_bounds__(64) k_chain_rays(ChainArgs A)
 
 // Boundary marches on the GPU for prepare(): the molecule's atoms are
 // uploaded once, then the chain of launches runs with one wait at its end.
-// Its buffers are stream-ordered (DBuf::alloc_async): a setup running beside
-// a serving slice does not wait for the slice at every free.
+// Its buffers come from the device cache (DBuf::alloc_cached): a setup
+// running beside a serving slice does not wait for the slice at every free.
 struct GpuMarcher : BoundaryMarcher {
     DBuf atoms_buf;
     void *atoms = nullptr;
@@ -782,7 +827,7 @@ struct GpuMarcher : BoundaryMarcher {
         }
         Stream st;
         HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
-        HIP_CHECK(atoms_buf.alloc_async(sizeof(AtomF64) * p.n, st.s));
+        HIP_CHECK(atoms_buf.alloc_cached(sizeof(AtomF64) * p.n));
         atoms = atoms_buf.p;
         HIP_CHECK(hipMemcpyAsync(atoms, a.data(), sizeof(AtomF64) * p.n, hipMemcpyHostToDevice, st.s));
         HIP_CHECK(hipStreamSynchronize(st.s));
@@ -794,8 +839,8 @@ struct GpuMarcher : BoundaryMarcher {
         Stream st;
         HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
         DBuf dst, dh;
-        HIP_CHECK(dst.alloc_async(sizeof(ChainState), st.s));
-        HIP_CHECK(dh.alloc_async(sizeof(MarchHit) * kChainMaxRays, st.s));
+        HIP_CHECK(dst.alloc_cached(sizeof(ChainState)));
+        HIP_CHECK(dh.alloc_cached(sizeof(MarchHit) * kChainMaxRays));
         ChainArgs A{};
         A.m = mol_dev(p);
         A.bp = bp;
@@ -832,8 +877,8 @@ struct GpuMarcher : BoundaryMarcher {
         Stream st;
         HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
         DBuf dw, dh;
-        HIP_CHECK(dw.alloc_async(sizeof(March) * w.size(), st.s));
-        HIP_CHECK(dh.alloc_async(sizeof(MarchHit) * w.size(), st.s));
+        HIP_CHECK(dw.alloc_cached(sizeof(March) * w.size()));
+        HIP_CHECK(dh.alloc_cached(sizeof(MarchHit) * w.size()));
         HIP_CHECK(hipMemcpyAsync(dw.p, w.data(), sizeof(March) * w.size(), hipMemcpyHostToDevice, st.s));
         MarchArgs A{};
         A.m = mol_dev(p);
@@ -874,8 +919,9 @@ struct cosims_mol {
     mutable std::mutex tune_mu;
     mutable std::map<int, std::pair<int, int>> tuned;
     mutable std::atomic<bool> on_device{false};  // device copy made (ensure_device succeeded)
-    // stream-ordered (see DBuf::alloc_async): destroying a molecule does not
-    // wait for launches of other molecules
+    // stream-ordered on the null stream (allocated there, filled by a
+    // synchronous copy): destroying a molecule does not wait for launches of
+    // other molecules
     ~cosims_mol() { if (dmem) (void)hipFreeAsync(dmem, nullptr); }
 };
 
@@ -931,7 +977,8 @@ int upload(const cosims_mol *M)
            og1 = align_up(og0 + sizeof(float4) * ncl), og2 = align_up(og1 + sizeof(float4) * ncl),
            ocse = align_up(og2 + sizeof(float4) * ncl), ocxq = align_up(ocse + sizeof(int2) * ncl),
            ocab = align_up(ocxq + sizeof(float4) * (ncl > 0 ? n : 0)),
-           total = align_up(ocab + sizeof(float2) * (ncl > 0 ? n : 0));
+           ocdm = align_up(ocab + sizeof(float2) * (ncl > 0 ? n : 0)),
+           total = align_up(ocdm + sizeof(unsigned long long) * ((ncl + 63) / 64));
     std::vector<char> host(total, 0);
     auto *a64 = (AtomF64 *)(host.data() + o64);
     auto *a32 = (AtomF32 *)(host.data() + o32);
@@ -961,6 +1008,7 @@ int upload(const cosims_mol *M)
     auto *cse = (int2 *)(host.data() + ocse);
     auto *cxq = (float4 *)(host.data() + ocxq);
     auto *cab = (float2 *)(host.data() + ocab);
+    auto *cdm = (unsigned long long *)(host.data() + ocdm);
     float dpr_max = 0.0f;
     for (int c = 0; c < ncl; c++) {
         const ClusterF32 &h = h32[c];
@@ -968,6 +1016,7 @@ int upload(const cosims_mol *M)
         const bool dip = p.multipole_order >= 1 && !h.rep;
         g0[c] = make_float4(h.px, h.py, h.pz, h.mpr);
         g1[c] = make_float4(h.q, dip ? h.d0 : 0.0f, dip ? h.d1 : 0.0f, dip ? h.d2 : 0.0f);
+        if (g1[c].y != 0.0f || g1[c].z != 0.0f || g1[c].w != 0.0f) cdm[c >> 6] |= 1ull << (c & 63);
         float A, B;
         const int qi = c < n ? c : 0;
         lj_constants(p.sig6[qi], p.eps24m[qi], A, B);
@@ -1017,6 +1066,7 @@ int upload(const cosims_mol *M)
     m.h64 = (const ClusterF64 *)(d + oh64); m.h32 = (const ClusterF32 *)(d + oh32);
     m.cg0 = (const float4 *)(d + og0); m.cg1 = (const float4 *)(d + og1); m.cg2 = (const float4 *)(d + og2);
     m.cse = (const int2 *)(d + ocse); m.cxq = (const float4 *)(d + ocxq); m.cab = (const float2 *)(d + ocab);
+    m.cdm = (const unsigned long long *)(d + ocdm);
     m.cl_dpr_max = dpr_max;
     M->soa = (const double *)(d + osoa);
     M->npad = npad;
@@ -1624,6 +1674,18 @@ GrpFn group_kernel(int prec, int ft, int L)
     return nullptr;
 }
 
+// The LDS-header clustered kernel (mixed precision, 64 lanes).
+GrpFn cl_lds_kernel(int ft)
+{
+    switch (ft) {
+    case 1: return k_traj_cl_lds<1>;
+    case 2: return k_traj_cl_lds<2>;
+    case 3: return k_traj_cl_lds<3>;
+    case 5: return k_traj_cl_lds<5>;
+    default: return nullptr;
+    }
+}
+
 using LdsMultiFn = void (*)(LdsMultiArgs);
 
 template <int P, int FT>
@@ -1753,6 +1815,17 @@ int tile_atoms(const cosims_mol *M, int prec, int wg)
 size_t tile_lds_bytes(int prec, int tile)
 {
     return (size_t)tile * (prec == COSIMS_PREC_FP64 ? 6 * sizeof(double) : sizeof(float4) + sizeof(float2));
+}
+
+// A clustered-type molecule runs k_traj_cl_lds (cluster headers in LDS,
+// one 1024-thread workgroup a CU) in mixed precision at 64 lanes when its
+// multipole order is at most 1 and its headers fit LDS beside the rings of
+// 16 wavefronts; otherwise the table kernel (headers through L2).
+constexpr int kClLdsWg = 1024;
+bool cl_lds_mode(const cosims_mol *M, int prec, int L)
+{
+    return prec == COSIMS_PREC_MIXED && L == 64 && cl_lds_kernel(M->prep.force_type) != nullptr &&
+           M->md.order <= 1 && M->md.ncl > 0 && cl_lds_bytes(M->md.ncl, kClLdsWg) <= M->dev.lds_max;
 }
 
 // Launch arguments of the LDS-staged and tiled kernels for one molecule.
@@ -1908,6 +1981,15 @@ int run_group(const cosims_mol *M, int prec, int n_items, TrajBufs &B, hipStream
 {
     if (!items) items = B.items[0].as<int32_t>();
     const bool tiled = tile_mode(M, prec);
+    if (cl_lds_mode(M, prec, L)) {
+        GrpFn fn = cl_lds_kernel(M->prep.force_type);
+        GroupArgs G = group_args(M);
+        const size_t lds = cl_lds_bytes(M->md.ncl, kClLdsWg);
+        HIP_CHECK(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        G.io = traj_io(B, items, n_items);
+        const int64_t grid = group_grid((const void *)fn, kClLdsWg, lds, bpc, M->dev.cus, n_items, L);
+        return timed_group_launch(fn, G, grid, kClLdsWg, lds, s, B, n_items, out, timed);
+    }
     if (!tiled && !lds_mode(M, prec))
         return run_multi(M->rec_dev, nullptr, M->prep.force_type, prec, M->dev.cus, items, n_items, B, s, L, wg, bpc,
                          out, timed);

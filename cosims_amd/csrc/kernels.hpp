@@ -56,6 +56,9 @@ struct MolDev {
     const int2 *cse;
     const float4 *cxq;
     const float2 *cab;
+    // bit c % 64 of cdm[c / 64]: cluster c has a nonzero dipole in cg1 (for
+    // the branch counts of the far field)
+    const unsigned long long *cdm;
     float cl_dpr_max;     // largest (R_d + rho)^2 of cg2 (FP32)
 };
 

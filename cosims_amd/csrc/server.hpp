@@ -373,9 +373,9 @@ int issue_slice(cosims_server *S, Slice &X)
         SJob &J = *e.job;
         if (e.it0 + e.nb > c.max_blocks) return fail(COSIMS_E_INVALID, "server: block count beyond max_blocks");
         if (!J.dsums.p) {
-            if (J.dsums.alloc_async(sizeof(double) * kSums * c.max_blocks, X.s) != hipSuccess ||
-                J.dblk.alloc_async(sizeof(double) * 2 * c.max_blocks, X.s) != hipSuccess ||
-                J.dstate.alloc_async(sizeof(ReqDev), X.s) != hipSuccess)
+            if (J.dsums.alloc_cached(sizeof(double) * kSums * c.max_blocks) != hipSuccess ||
+                J.dblk.alloc_cached(sizeof(double) * 2 * c.max_blocks) != hipSuccess ||
+                J.dstate.alloc_cached(sizeof(ReqDev)) != hipSuccess)
                 return fail(COSIMS_E_NOMEM, "server: per-request device state");
             J.init = ReqDev{0, 0, c.min_blocks, c.max_blocks, J.target, 0.0, 0.0};
             HIP_CHECK(hipMemcpyAsync(J.dstate.p, &J.init, sizeof(ReqDev), hipMemcpyHostToDevice, X.s));

@@ -709,9 +709,38 @@ __device__ __forceinline__ void cl32_flush(const MolDev &m, const int *ring, int
     }
 }
 
-template <int FT>
-__device__ __forceinline__ void wave_clustered32(const MolDev &m, int ncl, int lane, float px, float py, float pz,
-                                                 int *lists, double &ox, double &oy, double &oz,
+// Where wave_clustered32 reads the cluster headers of the slice loop: the
+// molecule's SoA in global memory (through L2), or a copy of cg0, cg1 and
+// the (R_d + rho)^2 of cg2 staged in LDS by the workgroup.
+struct GlbHeaders {
+    const GLB f4n *G0, *G1, *G2;
+    __device__ explicit GlbHeaders(const MolDev &m)
+        : G0((const GLB f4n *)glb(m.cg0)), G1((const GLB f4n *)glb(m.cg1)), G2((const GLB f4n *)glb(m.cg2))
+    {
+    }
+    __device__ f4n g0(int c) const { return G0[c]; }
+    __device__ f4n g1(int c) const { return G1[c]; }
+    __device__ float dr2(int c) const { return G2[c].x; }
+    // (R_d + rho)^2 read only where the LJ cut-off decides; the far clusters
+    // with dipole terms counted from the headers
+    static constexpr bool kPrefetchDr2 = false;
+};
+struct LdsHeaders {
+    const f4n *G0, *G1;
+    const float *D;
+    __device__ f4n g0(int c) const { return G0[c]; }
+    __device__ f4n g1(int c) const { return G1[c]; }
+    __device__ float dr2(int c) const { return D[c]; }
+    // (R_d + rho)^2 read with the next slice's headers; the far clusters with
+    // dipole terms counted against the molecule's per-slice dipole bits (cdm)
+    static constexpr bool kPrefetchDr2 = true;
+};
+// LDS bytes of the staged headers: cg0, cg1 (16 B each) and (R_d + rho)^2.
+__host__ __device__ constexpr size_t cl_header_bytes(int ncl) { return (size_t)ncl * 36; }
+
+template <int FT, class HD>
+__device__ __forceinline__ void wave_clustered32(const MolDev &m, const HD &hd, int ncl, int lane, float px, float py,
+                                                 float pz, int *lists, double &ox, double &oy, double &oz,
                                                  const WaveCount &W)
 {
     unsigned n_hdr = 0, n_far = 0, n_fdip = 0;  // wave-uniform
@@ -723,24 +752,26 @@ __device__ __forceinline__ void wave_clustered32(const MolDev &m, int ncl, int l
     int *ex_ring = lists, *lj_ring = lists + kRing;
     int ex_head = 0, ex_n = 0, lj_head = 0, lj_n = 0;
     const unsigned long long below = (1ull << lane) - 1;
-    const GLB f4n *G0 = (const GLB f4n *)glb(m.cg0);
-    const GLB f4n *G1 = (const GLB f4n *)glb(m.cg1);
-    const GLB f4n *G2 = (const GLB f4n *)glb(m.cg2);
     const float dpr_max = m.cl_dpr_max;  // no cluster's (R_d + rho)^2 exceeds it
     // headers of the next slice are loaded while the current one is worked on
     f4n n0 = f4n{0.0f, 0.0f, 0.0f, 0.0f}, n1 = n0;
+    float n2 = 0.0f;
     if (ncl > 0) {
         const int c = min(lane, ncl - 1);
-        n0 = G0[c];
-        if (kFar) n1 = G1[c];
+        n0 = hd.g0(c);
+        if (kFar) n1 = hd.g1(c);
+        if constexpr (HD::kPrefetchDr2) n2 = hd.dr2(c);
     }
     for (int base = 0; base < ncl; base += 64) {
         const int c = base + lane;
-        const f4n h = n0, g = n1;
-        if (base + 64 < ncl) {
-            const int cn = min(c + 64, ncl - 1);
-            n0 = G0[cn];
-            if (kFar) n1 = G1[cn];
+        f4n h = n0, g = n1;
+        float d2 = n2;
+        if constexpr (!HD::kPrefetchDr2) {
+            if (base + 64 < ncl) {
+                const int cn = min(c + 64, ncl - 1);
+                n0 = hd.g0(cn);
+                if (kFar) n1 = hd.g1(cn);
+            }
         }
         int kex = 0, klj = 0;  // exact: 1 charges, 2 charges + LJ; LJ: 1 per-atom, 2 fixed parameters
         bool fr = false;
@@ -748,7 +779,7 @@ __device__ __forceinline__ void wave_clustered32(const MolDev &m, int ncl, int l
             const float dx = px - h.x, dy = py - h.y, dz = pz - h.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             // within (R_d + rho)^2 of this cluster (its LJ cut-off)
-            const bool disp = !(r2 > dpr_max) && !(r2 > G2[c].x);
+            const bool disp = !(r2 > dpr_max) && !(r2 > (HD::kPrefetchDr2 ? d2 : hd.dr2(c)));
             if constexpr (kFar) {
                 if (r2 > h.w) {
                     cl32_far(g, dx, dy, dz, r2, s);
@@ -770,7 +801,10 @@ __device__ __forceinline__ void wave_clustered32(const MolDev &m, int ncl, int l
         // the dipole is zeroed for a centre-replaced cluster and at order 0
         n_hdr += __popcll(__ballot(c < ncl));
         n_far += __popcll(__ballot(fr));
-        n_fdip += __popcll(__ballot(fr && (g.y != 0.0f || g.z != 0.0f || g.w != 0.0f)));
+        if constexpr (HD::kPrefetchDr2)  // one molecule a workgroup: a scalar load of the slice's dipole bits
+            n_fdip += __popcll(__ballot(fr) & m.cdm[base >> 6]);
+        else
+            n_fdip += __popcll(__ballot(fr && (g.y != 0.0f || g.z != 0.0f || g.w != 0.0f)));
         ex_n += __popcll(me);
         lj_n += __popcll(ml);
         wave_sync_lds();
@@ -783,6 +817,14 @@ __device__ __forceinline__ void wave_clustered32(const MolDev &m, int ncl, int l
             cl32_flush<false>(m, lj_ring, lj_head, 64, lane, px, py, pz, s, W);
             lj_head = (lj_head + 64) & (kRing - 1);
             lj_n -= 64;
+        }
+        if constexpr (HD::kPrefetchDr2) {  // LDS: the next slice's headers, read after this one is done
+            if (base + 64 < ncl) {
+                const int cn = min(c + 64, ncl - 1);
+                n0 = hd.g0(cn);
+                if (kFar) n1 = hd.g1(cn);
+                n2 = hd.dr2(cn);
+            }
         }
         // the next slice's appends reuse the flushed slots: every lane is past its reads
         wave_sync_lds();
@@ -917,7 +959,8 @@ struct TableSource {
             const int ncl = active ? r.m.ncl : 0;
             if constexpr (PREC == 1) {
                 if (r.m.order <= 1)
-                    wave_clustered32<FT>(r.m, ncl, j, (float)ex, (float)ey, (float)ez, lists, fx, fy, fz, W);
+                    wave_clustered32<FT>(r.m, GlbHeaders(r.m), ncl, j, (float)ex, (float)ey, (float)ez, lists, fx,
+                                         fy, fz, W);
                 else
                     wave_clustered<float, FT>(r.m, ncl, j, (float)ex, (float)ey, (float)ez, lists, fx, fy, fz, W);
             } else
@@ -935,6 +978,38 @@ struct TableSource {
     {
         const MolRec &r = rec();
         return grp_potential<L>(glb(r.soa), active ? r.npad : 0, r.m.charged, r.m.dipole_scalar, j, ex, ey, ez);
+    }
+};
+
+// Molecule source of k_traj_cl_lds: one molecule of a clustered force type
+// (multipole order 0 or 1), mixed precision, one group of 64 lanes a
+// wavefront; the cluster headers of the slice loop come from the
+// workgroup's LDS copy (LdsHeaders), the members of near clusters and the
+// potential's atoms from global memory.  Same arithmetic in the same order
+// as TableSource, so the two give the same bits.
+template <int FT>
+struct LdsClusterSource {
+    static constexpr bool kCoop = false;
+    static constexpr bool kMulti = false;
+    const GroupArgs &A;
+    int *lists;    // this wavefront's cluster rings
+    WaveCount W;   // this wavefront's branch counts
+    LdsHeaders H;
+    __device__ void bind(int) {}
+    __device__ void flush(const TrajIO &io) const { flush_counts(io.evals, W); }
+    __device__ int pairs() const { return 0; }
+    __device__ int atoms() const { return A.m.n; }
+    __device__ double mu() const { return A.m.mu; }
+    __device__ double dt0() const { return A.dt0; }
+    __device__ bool exits(double px, double py, double pz) const { return outside(A.m, px, py, pz); }
+    __device__ void force(bool active, int j, double ex, double ey, double ez, double &fx, double &fy,
+                          double &fz) const
+    {
+        wave_clustered32<FT>(A.m, H, active ? A.m.ncl : 0, j, (float)ex, (float)ey, (float)ez, lists, fx, fy, fz, W);
+    }
+    __device__ double potential(bool active, int j, double ex, double ey, double ez) const
+    {
+        return grp_potential<64>(glb(A.soa), active ? A.npad : 0, A.m.charged, A.m.dipole_scalar, j, ex, ey, ez);
     }
 };
 
@@ -1548,6 +1623,43 @@ __global__ void __launch_bounds__(1024) k_traj_multi(MultiArgs A)
     resolve_range(io, A.first, A.last);
     TableSource<PREC, FT, L> src{A, lists, W};
     group_loop<L>(io, src);
+}
+
+// A clustered-type molecule whose cluster headers fit LDS, mixed
+// precision, 64 lanes a trajectory (LdsClusterSource).  Dynamic LDS: each
+// wavefront's two cluster rings (kClRingInts) and branch counts, then the
+// headers (cl_header_bytes), staged once per workgroup.
+constexpr int kClRingInts = 2 * kRing;
+__host__ __device__ constexpr size_t cl_lds_bytes(int ncl, int wg)
+{
+    return (size_t)(wg / 64) * (kClRingInts + kCountInts) * sizeof(int) + cl_header_bytes(ncl);
+}
+
+template <int FT>
+__global__ void __launch_bounds__(1024) k_traj_cl_lds(GroupArgs A)
+{
+    extern __shared__ __attribute__((aligned(16))) int lds_cl[];
+    const int nw = blockDim.x >> 6, w = threadIdx.x >> 6;
+    int *lists = lds_cl + w * kClRingInts;
+    WaveCount W{(unsigned *)(lds_cl + nw * kClRingInts) + w * kCountInts};
+    W.clear();
+    const int ncl = A.m.ncl;
+    f4n *h0 = (f4n *)(lds_cl + nw * (kClRingInts + kCountInts));
+    f4n *h1 = h0 + ncl;
+    float *d2 = (float *)(h1 + ncl);
+    const GLB f4n *G0 = (const GLB f4n *)glb(A.m.cg0);
+    const GLB f4n *G1 = (const GLB f4n *)glb(A.m.cg1);
+    const GLB f4n *G2 = (const GLB f4n *)glb(A.m.cg2);
+    for (int i = threadIdx.x; i < ncl; i += blockDim.x) {
+        h0[i] = G0[i];
+        h1[i] = G1[i];
+        d2[i] = G2[i].x;
+    }
+    __syncthreads();
+    TrajIO io = A.io;
+    resolve_range(io, A.first, A.last);
+    LdsClusterSource<FT> src{A, lists, W, LdsHeaders{h0, h1, d2}};
+    group_loop<64>(io, src);
 }
 
 }  // namespace cosims

@@ -98,6 +98,23 @@ def test_batch_of_one_equals_run(ion128, prec):
     assert b["n_hit"] == single["n_hit"] and b["total_steps"] == single["total_steps"]
 
 
+@pytest.mark.parametrize("case,flags", CLUSTERED, ids=[f"{c}-{''.join(map(str, f))}" for c, f in CLUSTERED])
+def test_clustered_lds_headers_equal_table_kernel(case, flags):
+    """Mixed precision, 64 lanes: cosims_run of a clustered molecule stages
+    its cluster headers in LDS (k_traj_cl_lds, multipole order <= 1); a batch
+    of that one job reads them through L2 (table kernel, k_traj_multi).  Same
+    terms in the same order: the same bits, trajectory by trajectory.  (Order
+    2 runs the table kernel both ways.)"""
+    m = golden_engine_mol(golden(case), flags)
+    kw = dict(job=7, iters=2, traj=1500)
+    run = m.run(E.PREC_MIXED, lanes=64, wg_size=1024, keep=True, **kw)
+    (bat,), _ = E.run_batch([(m, kw)], prec=E.PREC_MIXED, lanes=64, wg_size=256)
+    assert run["lanes"] == bat["lanes"] == 64
+    assert np.array_equal(run["omega"], bat["omega"])
+    assert run["total_steps"] == bat["total_steps"]
+    assert run["wmean"] == bat["wmean"]
+
+
 @pytest.mark.parametrize("own", [False, True], ids=["shared", "own"])
 def test_batch_alone_equals_mix(ion128, ion2000, own):
     """Uneven molecules and two force types in one call: each job's result
