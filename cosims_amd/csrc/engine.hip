@@ -2276,6 +2276,14 @@ int autotune(const cosims_mol *M, int prec, int64_t n_hint, int &L_out, int &wg_
         wg_out = it->second.second;
         return 0;
     }
+    if (clustered_type(M->prep.force_type) && cl_lds_mode(M, prec, 64)) {
+        // the clustered types run 64 lanes; with the headers in LDS the
+        // workgroup is 1024 threads whatever is asked: nothing to time
+        L_out = 64;
+        wg_out = kClLdsWg;
+        M->tuned[key] = {L_out, wg_out};
+        return 0;
+    }
     Stream st;
     HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
     // stage 1 holds several rounds even at 4 lanes per trajectory (16 trajectories per wavefront)

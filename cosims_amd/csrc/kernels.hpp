@@ -192,17 +192,55 @@ struct ClAcc {
       sumX = 0, sumY = 0, sumZ = 0;
 };
 
+// 1 / sqrt(r2) and 1 / x in FP64 for the group kernels: v_rsq_f64 / v_rcp_f64
+// refined by two Newton steps (within a few ulp of the IEEE quotients).  The
+// group kernels sum in lane-split order and are not bit-identical to the
+// reference anyway; the lane kernel and cosims_accel keep the IEEE
+// divisions and square roots of the reference.
+__device__ __forceinline__ double rsqrt64(double r2)
+{
+    double y = __builtin_amdgcn_rsq(r2);
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        // y += y (1 - r2 y^2) / 2
+        const double e = __builtin_fma(-(r2 * y), y, 1.0);
+        y = __builtin_fma(0.5 * y, e, y);
+    }
+    return y;
+}
+__device__ __forceinline__ double rcp64(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        // y += y (1 - x y)
+        const double e = __builtin_fma(-x, y, 1.0);
+        y = __builtin_fma(y, e, y);
+    }
+    return y;
+}
+
 // exact charges (+ LJ) over a cluster's members, src/forceAccel.cpp:366-403 / :408-436
-template <typename T, bool LJ>
+// FAST (FP64 group kernels): 1 / r from rsqrt64 instead of the IEEE
+// quotients, the same terms otherwise.
+template <typename T, bool LJ, bool FAST = false>
 __device__ __forceinline__ void cl_exact(const MolDev &m, int s, int e, T px, T py, T pz, ClAcc<T> &A)
 {
     using O = Ops<T>;
+    constexpr bool kF = FAST && sizeof(T) == 8;
     const auto C = Layout<T>::catoms(m);
     for (int k = s; k < e; k++) {
         T charge = C[k].q;
         T d0 = px - C[k].x, d1 = py - C[k].y, d2 = pz - C[k].z;
         T r2 = d0 * d0 + d1 * d1 + d2 * d2;
-        T r_5 = O::div(T(1), r2 * r2 * O::sqrt_(r2));
+        T ir2 = 0, r_5;
+        if constexpr (kF) {
+            const T ir = rsqrt64(r2);
+            ir2 = ir * ir;
+            r_5 = ir2 * ir2 * ir;
+        } else {
+            r_5 = O::div(T(1), r2 * r2 * O::sqrt_(r2));
+        }
         T r_3 = r_5 * r2;
         r_3 *= charge;
         r_5 *= charge;
@@ -215,9 +253,9 @@ __device__ __forceinline__ void cl_exact(const MolDev &m, int s, int e, T px, T 
         A.sYZ -= T(3) * d1 * d2 * r_5;
         if (LJ) {
             T r6 = r2 * r2 * r2;
-            T t2 = O::div(C[k].s6, r6);
+            T t2 = kF ? C[k].s6 * (ir2 * ir2 * ir2) : O::div(C[k].s6, r6);
             T t1 = T(2) * t2 * t2;
-            T sc = O::div(C[k].e24 * (t1 - t2), r2);
+            T sc = kF ? C[k].e24 * (t1 - t2) * ir2 : O::div(C[k].e24 * (t1 - t2), r2);
             A.sumX += sc * d0; A.sumY += sc * d1; A.sumZ += sc * d2;
         }
     }
@@ -225,40 +263,49 @@ __device__ __forceinline__ void cl_exact(const MolDev &m, int s, int e, T px, T 
 
 // LJ over a cluster's members with per-atom or fixed parameters,
 // src/forceAccel.cpp:341-360 (fixed = data[cluster index]) and :563-582
-template <typename T, bool FIXED>
+template <typename T, bool FIXED, bool FAST = false>
 __device__ __forceinline__ void cl_lj(const MolDev &m, int s, int e, T fs6, T fe24, T px, T py, T pz, ClAcc<T> &A)
 {
     using O = Ops<T>;
+    constexpr bool kF = FAST && sizeof(T) == 8;
     const auto C = Layout<T>::catoms(m);
     for (int k = s; k < e; k++) {
         T d0 = px - C[k].x, d1 = py - C[k].y, d2 = pz - C[k].z;
         T r2 = d0 * d0 + d1 * d1 + d2 * d2;
-        T r6 = r2 * r2 * r2;
-        T t2 = O::div(FIXED ? fs6 : (T)C[k].s6, r6);
-        T t1 = T(2) * t2 * t2;
-        T sc = O::div((FIXED ? fe24 : (T)C[k].e24) * (t1 - t2), r2);
+        T t2, sc;
+        if constexpr (kF) {
+            const T ir2 = rcp64(r2);
+            t2 = (FIXED ? fs6 : (T)C[k].s6) * (ir2 * ir2 * ir2);
+            const T t1 = T(2) * t2 * t2;
+            sc = (FIXED ? fe24 : (T)C[k].e24) * (t1 - t2) * ir2;
+        } else {
+            T r6 = r2 * r2 * r2;
+            t2 = O::div(FIXED ? fs6 : (T)C[k].s6, r6);
+            T t1 = T(2) * t2 * t2;
+            sc = O::div((FIXED ? fe24 : (T)C[k].e24) * (t1 - t2), r2);
+        }
         A.sumX += sc * d0; A.sumY += sc * d1; A.sumZ += sc * d2;
     }
 }
 
 // The same two per-atom loops with the variant chosen at run time (used
 // where lanes handle clusters of different kinds side by side).
-template <typename T>
+template <typename T, bool FAST = false>
 __device__ __forceinline__ void cl_exact_rt(const MolDev &m, int s, int e, bool lj, T px, T py, T pz, ClAcc<T> &A)
 {
-    if (lj) cl_exact<T, true>(m, s, e, px, py, pz, A);
-    else cl_exact<T, false>(m, s, e, px, py, pz, A);
+    if (lj) cl_exact<T, true, FAST>(m, s, e, px, py, pz, A);
+    else cl_exact<T, false, FAST>(m, s, e, px, py, pz, A);
 }
-template <typename T>
+template <typename T, bool FAST = false>
 __device__ __forceinline__ void cl_lj_rt(const MolDev &m, int s, int e, bool fixed, T fs6, T fe24, T px, T py, T pz,
                                          ClAcc<T> &A)
 {
-    if (fixed) cl_lj<T, true>(m, s, e, fs6, fe24, px, py, pz, A);
-    else cl_lj<T, false>(m, s, e, T(0), T(0), px, py, pz, A);
+    if (fixed) cl_lj<T, true, FAST>(m, s, e, fs6, fe24, px, py, pz, A);
+    else cl_lj<T, false, FAST>(m, s, e, T(0), T(0), px, py, pz, A);
 }
 
 // far-field multipole of one cluster, src/forceAccel.cpp:258-337
-template <typename T>
+template <typename T, bool FAST = false>
 __device__ __forceinline__ void cl_far(const MolDev &m, const CAS ClusterT<T> &H, T dx0, T dx1, T dx2, T r2,
                                        ClAcc<T> &A)
 {
@@ -266,8 +313,14 @@ __device__ __forceinline__ void cl_far(const MolDev &m, const CAS ClusterT<T> &H
     const int order = m.order;
     const bool dip = order >= 1 && !H.rep;
     T xx = dx0 * dx0, yy = dx1 * dx1, zz = dx2 * dx2;
-    T r_2 = O::div(T(1), r2);
-    T r_1 = O::div(T(1), O::sqrt_(r2));
+    T r_2, r_1;
+    if constexpr (FAST && sizeof(T) == 8) {
+        r_1 = rsqrt64(r2);
+        r_2 = r_1 * r_1;
+    } else {
+        r_2 = O::div(T(1), r2);
+        r_1 = O::div(T(1), O::sqrt_(r2));
+    }
     T r_3 = r_2 * r_1;
     T r_5 = r_3 * r_2;
     T r_7 = 0, const1 = 0, const2 = 0, tM0 = 0, tM1 = 0, tM2 = 0;
@@ -327,7 +380,7 @@ __device__ __forceinline__ void cl_far(const MolDev &m, const CAS ClusterT<T> &H
 // multipoleOnly), :655-742 (type 3, dispersionOnly), :819-851 (type 5,
 // dispersionNoCharge).
 // With CNT the branches taken are counted into C.
-template <typename T, int FT, bool CNT = false>
+template <typename T, int FT, bool CNT = false, bool FAST = false>
 __device__ __forceinline__ void cl_one(const MolDev &m, const CAS ClusterT<T> &H, T px, T py, T pz, ClAcc<T> &A,
                                        ClCount &C)
 {
@@ -339,40 +392,40 @@ __device__ __forceinline__ void cl_one(const MolDev &m, const CAS ClusterT<T> &H
         T r2 = xx + yy + zz;
         const int s = H.start, e = H.end;
         if (r2 > H.mpr) {
-            cl_far<T>(m, H, dx0, dx1, dx2, r2, A);
+            cl_far<T, FAST>(m, H, dx0, dx1, dx2, r2, A);
             if (CNT) {
                 C.far++;
                 C.fdip += (m.order >= 1 && !H.rep) ? 1u : 0u;
             }
             if (FT == 1) {
                 if (!(r2 > H.dpr)) {
-                    cl_lj<T, true>(m, s, e, H.qs6, H.qe24, px, py, pz, A);
+                    cl_lj<T, true, FAST>(m, s, e, H.qs6, H.qe24, px, py, pz, A);
                     if (CNT) C.lj += na;
                 }
             } else {
-                cl_lj<T, false>(m, s, e, T(0), T(0), px, py, pz, A);
+                cl_lj<T, false, FAST>(m, s, e, T(0), T(0), px, py, pz, A);
                 if (CNT) C.lj += na;
             }
         } else if (FT == 2 || !(r2 > H.dpr)) {
-            cl_exact<T, true>(m, s, e, px, py, pz, A);
+            cl_exact<T, true, FAST>(m, s, e, px, py, pz, A);
             if (CNT) C.exlj += na;
         } else {
-            cl_exact<T, false>(m, s, e, px, py, pz, A);
+            cl_exact<T, false, FAST>(m, s, e, px, py, pz, A);
             if (CNT) C.exq += na;
         }
     } else {
         T r2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
         if constexpr (FT == 3) {
             if (!(r2 > H.dpr)) {
-                cl_exact<T, true>(m, H.start, H.end, px, py, pz, A);
+                cl_exact<T, true, FAST>(m, H.start, H.end, px, py, pz, A);
                 if (CNT) C.exlj += na;
             } else {
-                cl_exact<T, false>(m, H.start, H.end, px, py, pz, A);
+                cl_exact<T, false, FAST>(m, H.start, H.end, px, py, pz, A);
                 if (CNT) C.exq += na;
             }
         } else {
             if (!(r2 > H.dpr)) {
-                cl_lj<T, false>(m, H.start, H.end, T(0), T(0), px, py, pz, A);
+                cl_lj<T, false, FAST>(m, H.start, H.end, T(0), T(0), px, py, pz, A);
                 if (CNT) C.lj += na;
             }
         }
@@ -401,12 +454,12 @@ __device__ __forceinline__ void cl_epilogue(const MolDev &m, ClAcc<T> &A, T &ox,
 // Clustered force types over clusters first, first+stride, ...: stride 1 is
 // the reference's cluster order; the group kernel splits clusters over the
 // lanes of a group and sums the partial accumulators afterwards.
-template <typename T, int FT, bool CNT = false>
+template <typename T, int FT, bool CNT = false, bool FAST = false>
 __device__ __forceinline__ void cl_sum(const MolDev &m, int first, int stride, T px, T py, T pz, ClAcc<T> &A,
                                        ClCount &C)
 {
     const auto H = Layout<T>::heads(m);
-    for (int n = first; n < m.ncl; n += stride) cl_one<T, FT, CNT>(m, H[n], px, py, pz, A, C);
+    for (int n = first; n < m.ncl; n += stride) cl_one<T, FT, CNT, FAST>(m, H[n], px, py, pz, A, C);
 }
 
 template <typename T, int FT>

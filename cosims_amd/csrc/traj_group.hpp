@@ -375,18 +375,6 @@ __host__ __device__ inline void lj_constants(double s6, double e24, float &A, fl
 // Newton steps (within a few ulp of the quotients).  These kernels sum in
 // lane-split order and are not bit-identical to the reference anyway; the
 // lane kernel keeps the IEEE operations and the reference's order.
-__device__ __forceinline__ double rsqrt64(double r2)
-{
-    double y = __builtin_amdgcn_rsq(r2);
-#pragma unroll
-    for (int k = 0; k < 2; k++) {
-        // y += y (1 - r2 y^2) / 2
-        const double e = __builtin_fma(-(r2 * y), y, 1.0);
-        y = __builtin_fma(0.5 * y, e, y);
-    }
-    return y;
-}
-
 template <int FT>
 __device__ __forceinline__ void pair64(double ax, double ay, double az, double aq, double as6, double ae24,
                                        double px, double py, double pz, double (&s)[12])
@@ -454,7 +442,7 @@ __device__ __forceinline__ void grp_clustered(const MolDev &m, int j, T px, T py
 {
     ClAcc<T> A;
     ClCount C;  // not kept: branch counts come from the 64-lane kernels only
-    cl_sum<T, FT>(m, j, L, px, py, pz, A, C);
+    cl_sum<T, FT, false, true>(m, j, L, px, py, pz, A, C);
     A.sumX = gsum<L>(A.sumX); A.sumY = gsum<L>(A.sumY); A.sumZ = gsum<L>(A.sumZ);
     if constexpr (FT != 5) {
         A.fX = gsum<L>(A.fX); A.fY = gsum<L>(A.fY); A.fZ = gsum<L>(A.fZ);
@@ -503,7 +491,7 @@ __device__ __forceinline__ void wave_clustered(const MolDev &m, int ncl, int lan
                 const T r2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
                 if constexpr (FT == 1 || FT == 2) {
                     if (r2 > h.mpr) {
-                        cl_far<T>(m, h, dx0, dx1, dx2, r2, A);
+                        cl_far<T, true>(m, h, dx0, dx1, dx2, r2, A);
                         fr = true;
                         dp = m.order >= 1 && !h.rep;
                         if (FT == 2) klj = 1;
@@ -532,13 +520,13 @@ __device__ __forceinline__ void wave_clustered(const MolDev &m, int ncl, int lan
         for (int e = lane; e < nex; e += 64) {
             const int v = ex_list[e];
             const CAS ClusterT<T> &h = H[v >> 2];
-            cl_exact_rt<T>(m, h.start, h.end, (v & 3) == 2, px, py, pz, A);
+            cl_exact_rt<T, true>(m, h.start, h.end, (v & 3) == 2, px, py, pz, A);
             W.lane_add((v & 3) == 2 ? 3 : 4, (unsigned)(h.end - h.start));
         }
         for (int e = lane; e < nlj; e += 64) {
             const int v = lj_list[e];
             const CAS ClusterT<T> &h = H[v >> 2];
-            cl_lj_rt<T>(m, h.start, h.end, (v & 3) == 2, h.qs6, h.qe24, px, py, pz, A);
+            cl_lj_rt<T, true>(m, h.start, h.end, (v & 3) == 2, h.qs6, h.qe24, px, py, pz, A);
             W.lane_add(5, (unsigned)(h.end - h.start));
         }
         // the next chunk overwrites the lists: every lane is past its reads
