@@ -978,7 +978,9 @@ int upload(const cosims_mol *M)
            ocse = align_up(og2 + sizeof(float4) * ncl), ocxq = align_up(ocse + sizeof(int2) * ncl),
            ocab = align_up(ocxq + sizeof(float4) * (ncl > 0 ? n : 0)),
            ocdm = align_up(ocab + sizeof(float2) * (ncl > 0 ? n : 0)),
-           total = align_up(ocdm + sizeof(unsigned long long) * ((ncl + 63) / 64));
+           odg0 = align_up(ocdm + sizeof(unsigned long long) * ((ncl + 63) / 64)),
+           odg1 = align_up(odg0 + 4 * sizeof(double) * ncl), odg2 = align_up(odg1 + 4 * sizeof(double) * ncl),
+           total = align_up(odg2 + 4 * sizeof(double) * ncl);
     std::vector<char> host(total, 0);
     auto *a64 = (AtomF64 *)(host.data() + o64);
     auto *a32 = (AtomF32 *)(host.data() + o32);
@@ -1009,7 +1011,11 @@ int upload(const cosims_mol *M)
     auto *cxq = (float4 *)(host.data() + ocxq);
     auto *cab = (float2 *)(host.data() + ocab);
     auto *cdm = (unsigned long long *)(host.data() + ocdm);
+    auto *dg0 = (double *)(host.data() + odg0);
+    auto *dg1 = (double *)(host.data() + odg1);
+    auto *dg2 = (double *)(host.data() + odg2);
     float dpr_max = 0.0f;
+    double dpr_max64 = 0.0;
     for (int c = 0; c < ncl; c++) {
         const ClusterF32 &h = h32[c];
         dpr_max = std::max(dpr_max, h.dpr);
@@ -1017,6 +1023,12 @@ int upload(const cosims_mol *M)
         g0[c] = make_float4(h.px, h.py, h.pz, h.mpr);
         g1[c] = make_float4(h.q, dip ? h.d0 : 0.0f, dip ? h.d1 : 0.0f, dip ? h.d2 : 0.0f);
         if (g1[c].y != 0.0f || g1[c].z != 0.0f || g1[c].w != 0.0f) cdm[c >> 6] |= 1ull << (c & 63);
+        const ClusterF64 &d = h64[c];
+        dpr_max64 = std::max(dpr_max64, d.dpr);
+        double *e0 = dg0 + 4 * c, *e1 = dg1 + 4 * c, *e2 = dg2 + 4 * c;
+        e0[0] = d.px; e0[1] = d.py; e0[2] = d.pz; e0[3] = d.mpr;
+        e1[0] = d.q; e1[1] = dip ? d.d0 : 0.0; e1[2] = dip ? d.d1 : 0.0; e1[3] = dip ? d.d2 : 0.0;
+        e2[0] = d.dpr; e2[1] = d.qs6; e2[2] = d.qe24; e2[3] = 0.0;
         float A, B;
         const int qi = c < n ? c : 0;
         lj_constants(p.sig6[qi], p.eps24m[qi], A, B);
@@ -1067,6 +1079,8 @@ int upload(const cosims_mol *M)
     m.cg0 = (const float4 *)(d + og0); m.cg1 = (const float4 *)(d + og1); m.cg2 = (const float4 *)(d + og2);
     m.cse = (const int2 *)(d + ocse); m.cxq = (const float4 *)(d + ocxq); m.cab = (const float2 *)(d + ocab);
     m.cdm = (const unsigned long long *)(d + ocdm);
+    m.dg0 = (const double *)(d + odg0); m.dg1 = (const double *)(d + odg1); m.dg2 = (const double *)(d + odg2);
+    m.cl_dpr_max64 = dpr_max64;
     m.cl_dpr_max = dpr_max;
     M->soa = (const double *)(d + osoa);
     M->npad = npad;

@@ -59,6 +59,11 @@ struct MolDev {
     // bit c % 64 of cdm[c / 64]: cluster c has a nonzero dipole in cg1 (for
     // the branch counts of the far field)
     const unsigned long long *cdm;
+    // FP64 cluster SoA of the FP64 one-group-a-wavefront kernel, 4 doubles a
+    // cluster each: dg0 (x, y, z, (R_m + rho)^2), dg1 (charge, dipole; zero as
+    // in cg1), dg2 ((R_d + rho)^2, the band branch's sigma^6 and 24 eps/mu, 0)
+    const double *dg0, *dg1, *dg2;
+    double cl_dpr_max64;  // largest (R_d + rho)^2 of dg2
     float cl_dpr_max;     // largest (R_d + rho)^2 of cg2 (FP32)
 };
 

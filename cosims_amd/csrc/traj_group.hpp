@@ -844,6 +844,229 @@ __device__ __forceinline__ void wave_clustered32(const MolDev &m, const HD &hd, 
     ox = fx; oy = fy; oz = fz;
 }
 
+// FP64 counterparts of cl32_atom / cl32_lj / cl32_far and the slice loop of
+// wave_clustered32, for the FP64 group kernels of the clustered types with
+// one group per wavefront (multipole order 0 or 1): the same branches,
+// rings and regrouped far field (M_ab = sum w d_a d_b plus a trace term, the
+// antisymmetric part as three components), in FP64 with 1/r from rsqrt64 and
+// 1/r^2 from rcp64; headers from the FP64 SoA dg0, dg1, dg2, members from
+// the FP64 member records.  LJ in the reference's form t2 = sigma^6 r^-6,
+// (24 eps/mu)(2 t2^2 - t2) r^-2 (src/forceAccel.cpp:341-360, 366-403).
+typedef double d4n __attribute__((ext_vector_type(4)));
+
+template <bool LJ>
+__device__ __forceinline__ void cl64_atom(double ax, double ay, double az, double aq, double s6, double e24,
+                                          double px, double py, double pz, double (&s)[16])
+{
+    const double dx = px - ax, dy = py - ay, dz = pz - az;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double ir = rsqrt64(r2);
+    const double ir2 = ir * ir;
+    if (LJ) {
+        const double t2 = s6 * (ir2 * ir2 * ir2);
+        const double t1 = 2.0 * t2 * t2;
+        const double sc = e24 * (t1 - t2) * ir2;
+        s[0] += sc * dx; s[1] += sc * dy; s[2] += sc * dz;
+    }
+    const double qr3 = aq * ir2 * ir;
+    s[3] += qr3 * dx; s[4] += qr3 * dy; s[5] += qr3 * dz;
+    s[12] += qr3;
+    const double w = 3.0 * (qr3 * ir2);
+    const double u = w * dx, v = w * dy;
+    s[6] += u * dx; s[9] += u * dy; s[10] += u * dz;
+    s[7] += v * dy; s[11] += v * dz;
+    s[8] += (w * dz) * dz;
+}
+
+__device__ __forceinline__ void cl64_lj(double ax, double ay, double az, double s6, double e24, double px, double py,
+                                        double pz, double (&s)[16])
+{
+    const double dx = px - ax, dy = py - ay, dz = pz - az;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double ir2 = rcp64(r2);
+    const double t2 = s6 * (ir2 * ir2 * ir2);
+    const double t1 = 2.0 * t2 * t2;
+    const double sc = e24 * (t1 - t2) * ir2;
+    s[0] += sc * dx; s[1] += sc * dy; s[2] += sc * dz;
+}
+
+// far-field monopole + dipole of one cluster (src/forceAccel.cpp:258-316)
+__device__ __forceinline__ void cl64_far(d4n g, double dx, double dy, double dz, double r2, double (&s)[16])
+{
+    const double ir = rsqrt64(r2);
+    const double ir2 = ir * ir;
+    const double r3 = ir2 * ir, r5 = r3 * ir2, r7 = r5 * ir2;
+    const double c1 = 3.0 * (g.y * dx + g.z * dy + g.w * dz);
+    const double cR3 = g.x * r3, cR5 = g.x * r5;
+    const double al = c1 * r5 + cR3;
+    s[3] += al * dx - r3 * g.y; s[4] += al * dy - r3 * g.z; s[5] += al * dz - r3 * g.w;
+    s[12] += al;
+    const double w = 3.0 * cR5 + 5.0 * (c1 * r7);
+    const double u = w * dx, v = w * dy;
+    s[6] += u * dx; s[9] += u * dy; s[10] += u * dz;
+    s[7] += v * dy; s[11] += v * dz;
+    s[8] += (w * dz) * dz;
+    const double s5 = 3.0 * r5;
+    const double sx = s5 * dx, sy = s5 * dy, sz = s5 * dz;
+    s[13] += sx * g.z - sy * g.y;
+    s[14] += sx * g.w - sz * g.y;
+    s[15] += sy * g.w - sz * g.z;
+}
+
+// Members [k0, k1) of one cluster in cluster order, two at a time (MODE as
+// in cl32_members); the extra slot re-reads the last member and adds zero.
+template <int MODE>
+__device__ __forceinline__ void cl64_members(const MolDev &m, int k0, int k1, double fs6, double fe24, double px,
+                                             double py, double pz, double (&s)[16])
+{
+    // a member record (AtomF64, 64 B) as two vectors: x y z q | s6 e24 e4 pad
+    const GLB d4n *C = (const GLB d4n *)glb(m.c64);
+    for (int k = k0; k < k1; k += 2) {
+        d4n a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int kk = min(k + i, k1 - 1);
+            a[i] = C[2 * kk];
+            if constexpr (MODE != 0 && MODE != 3) b[i] = C[2 * kk + 1];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const bool on = k + i < k1;
+            const double q = on ? a[i].w : 0.0;
+            const double s6 = on ? (MODE == 3 ? fs6 : (MODE == 0 ? 0.0 : b[i].x)) : 0.0;
+            const double e24 = on ? (MODE == 3 ? fe24 : (MODE == 0 ? 0.0 : b[i].y)) : 0.0;
+            if constexpr (MODE == 0) cl64_atom<false>(a[i].x, a[i].y, a[i].z, q, s6, e24, px, py, pz, s);
+            else if constexpr (MODE == 1) cl64_atom<true>(a[i].x, a[i].y, a[i].z, q, s6, e24, px, py, pz, s);
+            else cl64_lj(a[i].x, a[i].y, a[i].z, s6, e24, px, py, pz, s);
+        }
+    }
+}
+
+template <bool EXACT>
+__device__ __forceinline__ void cl64_flush(const MolDev &m, const int *ring, int head, int cnt, int lane, double px,
+                                           double py, double pz, double (&s)[16], const WaveCount &W)
+{
+    if (lane >= cnt) return;
+    const int v = ring[(head + lane) & (kRing - 1)];
+    const int c = v >> 2, kind = v & 3;
+    const i2n se = ((const GLB i2n *)glb(m.cse))[c];
+    W.lane_add(!EXACT ? 5 : (kind == 2 ? 3 : 4), (unsigned)(se.y - se.x));
+    if constexpr (EXACT) {
+        if (kind == 2) cl64_members<1>(m, se.x, se.y, 0.0, 0.0, px, py, pz, s);
+        else cl64_members<0>(m, se.x, se.y, 0.0, 0.0, px, py, pz, s);
+    } else {
+        if (kind == 2) {
+            const d4n g2 = ((const GLB d4n *)glb(m.dg2))[c];
+            cl64_members<3>(m, se.x, se.y, g2.y, g2.z, px, py, pz, s);
+        } else {
+            cl64_members<2>(m, se.x, se.y, 0.0, 0.0, px, py, pz, s);
+        }
+    }
+}
+
+template <int FT>
+__device__ __forceinline__ void wave_clustered64(const MolDev &m, int ncl, int lane, double px, double py, double pz,
+                                                 int *lists, double &ox, double &oy, double &oz, const WaveCount &W)
+{
+    unsigned n_hdr = 0, n_far = 0, n_fdip = 0;  // wave-uniform
+    constexpr bool kFar = FT == 1 || FT == 2;
+    // s: LJ force 0-2, field 3-5, M xx yy zz xy xz yz 6-11, trace 12, antisymmetric xy xz yz 13-15
+    double s[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) s[k] = 0.0;
+    int *ex_ring = lists, *lj_ring = lists + kRing;
+    int ex_head = 0, ex_n = 0, lj_head = 0, lj_n = 0;
+    const unsigned long long below = (1ull << lane) - 1;
+    const GLB d4n *G0 = (const GLB d4n *)glb(m.dg0);
+    const GLB d4n *G1 = (const GLB d4n *)glb(m.dg1);
+    const GLB d4n *G2 = (const GLB d4n *)glb(m.dg2);
+    const double dpr_max = m.cl_dpr_max64;  // no cluster's (R_d + rho)^2 exceeds it
+    // headers of the next slice are loaded while the current one is worked on
+    d4n n0 = d4n{0.0, 0.0, 0.0, 0.0}, n1 = n0;
+    if (ncl > 0) {
+        const int c = min(lane, ncl - 1);
+        n0 = G0[c];
+        if (kFar) n1 = G1[c];
+    }
+    for (int base = 0; base < ncl; base += 64) {
+        const int c = base + lane;
+        const d4n h = n0, g = n1;
+        if (base + 64 < ncl) {
+            const int cn = min(c + 64, ncl - 1);
+            n0 = G0[cn];
+            if (kFar) n1 = G1[cn];
+        }
+        int kex = 0, klj = 0;  // exact: 1 charges, 2 charges + LJ; LJ: 1 per-atom, 2 fixed parameters
+        bool fr = false;
+        if (c < ncl) {
+            const double dx = px - h.x, dy = py - h.y, dz = pz - h.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            // within (R_d + rho)^2 of this cluster (its LJ cut-off)
+            const bool disp = !(r2 > dpr_max) && !(r2 > G2[c].x);
+            if constexpr (kFar) {
+                if (r2 > h.w) {
+                    cl64_far(g, dx, dy, dz, r2, s);
+                    fr = true;
+                    if (FT == 2) klj = 1;
+                    else if (disp) klj = 2;
+                } else {
+                    kex = (FT == 2 || disp) ? 2 : 1;
+                }
+            } else if constexpr (FT == 3) {
+                kex = disp ? 2 : 1;
+            } else {
+                if (disp) klj = 1;
+            }
+        }
+        const unsigned long long me = __ballot(kex != 0), ml = __ballot(klj != 0);
+        if (kex) ex_ring[(ex_head + ex_n + __popcll(me & below)) & (kRing - 1)] = 4 * c + kex;
+        if (klj) lj_ring[(lj_head + lj_n + __popcll(ml & below)) & (kRing - 1)] = 4 * c + klj;
+        n_hdr += __popcll(__ballot(c < ncl));
+        n_far += __popcll(__ballot(fr));
+        n_fdip += __popcll(__ballot(fr && (g.y != 0.0 || g.z != 0.0 || g.w != 0.0)));
+        ex_n += __popcll(me);
+        lj_n += __popcll(ml);
+        wave_sync_lds();
+        if (ex_n >= 64) {
+            cl64_flush<true>(m, ex_ring, ex_head, 64, lane, px, py, pz, s, W);
+            ex_head = (ex_head + 64) & (kRing - 1);
+            ex_n -= 64;
+        }
+        if (lj_n >= 64) {
+            cl64_flush<false>(m, lj_ring, lj_head, 64, lane, px, py, pz, s, W);
+            lj_head = (lj_head + 64) & (kRing - 1);
+            lj_n -= 64;
+        }
+        // the next slice's appends reuse the flushed slots: every lane is past its reads
+        wave_sync_lds();
+    }
+    if (ex_n) cl64_flush<true>(m, ex_ring, ex_head, ex_n, lane, px, py, pz, s, W);
+    if (lj_n) cl64_flush<false>(m, lj_ring, lj_head, lj_n, lane, px, py, pz, s, W);
+    W.wave_add(0, n_hdr);
+    W.wave_add(1, n_far);
+    W.wave_add(2, n_fdip);
+    wave_sync_lds();
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        if (FT == 5 && k >= 3) break;
+        if (FT == 3 && k >= 13) break;
+        s[k] = gsum<64>(s[k]);
+    }
+    double fx = s[0], fy = s[1], fz = s[2];
+    if constexpr (FT != 5) {
+        const double ds = m.dipole_scalar;
+        const double Ex = s[3], Ey = s[4], Ez = s[5], t = s[12];
+        const double Txx = s[6] - t, Tyy = s[7] - t, Tzz = s[8] - t;
+        const double Txy = s[9], Txz = s[10], Tyz = s[11];
+        double Axy = 0, Axz = 0, Ayz = 0;
+        if constexpr (kFar) { Axy = s[13]; Axz = s[14]; Ayz = s[15]; }
+        fx -= ds * (Txx * Ex + (Txy - Axy) * Ey + (Txz - Axz) * Ez);
+        fy -= ds * ((Txy + Axy) * Ex + Tyy * Ey + (Tyz - Ayz) * Ez);
+        fz -= ds * ((Txz + Axz) * Ex + (Tyz + Ayz) * Ey + Tzz * Ez);
+    }
+    ox = fx; oy = fy; oz = fz;
+}
+
 // getPotential across the group (src/forceAccel.cpp:943-982), FP64, over
 // the SoA X (x y z q s6 e24 e4, stride npad).
 template <int L, typename PD>
@@ -951,8 +1174,11 @@ struct TableSource {
                                          fy, fz, W);
                 else
                     wave_clustered<float, FT>(r.m, ncl, j, (float)ex, (float)ey, (float)ez, lists, fx, fy, fz, W);
-            } else
+            } else if (r.m.order <= 1) {
+                wave_clustered64<FT>(r.m, ncl, j, ex, ey, ez, lists, fx, fy, fz, W);
+            } else {
                 wave_clustered<double, FT>(r.m, ncl, j, ex, ey, ez, lists, fx, fy, fz, W);
+            }
         } else {
             MolDev m = r.m;
             if (!active) m.ncl = 0;
