@@ -738,7 +738,8 @@ struct ChainArgs {
     MolDev m;
     BoundaryPlan bp;
     double mk;
-    int64_t max_points;
+    int64_t max_points;    // rays
+    int64_t zwalk_points;  // the z-axis walk
     ChainState *st;
     MarchHit *hits;
 };
@@ -752,7 +753,7 @@ __global__ void __launch_bounds__(64) k_chain_zwalk(ChainArgs A)
     w.prev0[0] = 0; w.prev0[1] = 0; w.prev0[2] = A.bp.z0;
     w.first = 1;
     w.axis = 0;
-    march_wave(A.m, w, A.mk, A.max_points, A.st->z);
+    march_wave(A.m, w, A.mk, A.zwalk_points, A.st->z);
 }
 
 // ray_geometry (host) in the same IEEE operations
@@ -797,6 +798,11 @@ __global__ void __launch_bounds__(64) k_chain_rays(ChainArgs A)
 // Its buffers come from the device cache (DBuf::alloc_cached): a setup
 // running beside a serving slice does not wait for the slice at every free.
 struct GpuMarcher : BoundaryMarcher {
+    // One stream for the whole search: the atoms' upload, the marches and
+    // the read-backs are ordered on it, so the march kernels see the copy
+    // through the stream's own ordering (buffers come from the device cache
+    // and may have held another molecule's data, read on another stream).
+    Stream st;
     DBuf atoms_buf;
     void *atoms = nullptr;
     int boundary(const Prepared &p, double mk, const BoundaryPlan &bp, MarchHit &zhit, std::vector<MarchHit> &rays,
@@ -815,29 +821,31 @@ struct GpuMarcher : BoundaryMarcher {
         m.a64 = (const AtomF64 *)atoms;
         return m;
     }
+    int stream()
+    {
+        if (!st.s) HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+        return 0;
+    }
     int upload_atoms(const Prepared &p)
     {
         if (atoms) return 0;
         Device d;
         if (int rc = device_init(d)) return rc;
+        if (int rc = stream()) return rc;
         std::vector<AtomF64> a(p.n);
         for (int i = 0; i < p.n; i++) {
             a[i].x = p.xyz[3 * i]; a[i].y = p.xyz[3 * i + 1]; a[i].z = p.xyz[3 * i + 2];
             a[i].q = p.q[i]; a[i].s6 = p.sig6[i]; a[i].e24 = p.eps24m[i]; a[i].e4 = p.eps4[i]; a[i].pad = 0;
         }
-        Stream st;
-        HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
         HIP_CHECK(atoms_buf.alloc_cached(sizeof(AtomF64) * p.n));
         atoms = atoms_buf.p;
         HIP_CHECK(hipMemcpyAsync(atoms, a.data(), sizeof(AtomF64) * p.n, hipMemcpyHostToDevice, st.s));
-        HIP_CHECK(hipStreamSynchronize(st.s));
+        HIP_CHECK(hipStreamSynchronize(st.s));  // a[] is freed on return
         return 0;
     }
     int boundary_(const Prepared &p, double mk, const BoundaryPlan &bp, MarchHit &zhit, std::vector<MarchHit> &rays)
     {
         if (int rc = upload_atoms(p)) return rc;
-        Stream st;
-        HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
         DBuf dst, dh;
         HIP_CHECK(dst.alloc_cached(sizeof(ChainState)));
         HIP_CHECK(dh.alloc_cached(sizeof(MarchHit) * kChainMaxRays));
@@ -846,6 +854,7 @@ struct GpuMarcher : BoundaryMarcher {
         A.bp = bp;
         A.mk = mk;
         A.max_points = kMaxMarchPoints;
+        A.zwalk_points = zwalk_points(bp.z0);
         A.st = dst.as<ChainState>();
         A.hits = dh.as<MarchHit>();
         hipLaunchKernelGGL(k_chain_zwalk, dim3(1), dim3(64), 0, st.s, A);
@@ -862,6 +871,8 @@ struct GpuMarcher : BoundaryMarcher {
             rays.clear();
             return 0;
         }
+        if (!(S.g.levels >= 25 && S.g.levels < (1 << 20) && S.count == (S.g.levels + 1) * kBoundaryRot))
+            return fail(COSIMS_E_SETUP, "boundary search: inconsistent ray geometry");
         if (S.count <= kChainMaxRays) {
             rays.resize(S.count);
             return 0;
@@ -874,8 +885,6 @@ struct GpuMarcher : BoundaryMarcher {
     }
     int run_list(const Prepared &p, double mk, const std::vector<March> &w, std::vector<MarchHit> &out)
     {
-        Stream st;
-        HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
         DBuf dw, dh;
         HIP_CHECK(dw.alloc_cached(sizeof(March) * w.size()));
         HIP_CHECK(dh.alloc_cached(sizeof(MarchHit) * w.size()));
@@ -912,6 +921,8 @@ struct cosims_mol {
     mutable const double *soa = nullptr;  // 7 FP64 arrays of npad: x y z q s6 e24 e4
     mutable int npad = 0;
     mutable size_t dbytes = 0;            // device bytes of the molecule
+    mutable size_t dcap = 0;              // ... of its block of the device cache
+    mutable int ddev = 0;
     mutable MolRec rec{};                 // table record of this molecule (device pointers)
     mutable const MolRec *rec_dev = nullptr;  // the same record in device memory
     // autotuned launch shapes, keyed by precision and run-size bucket:
@@ -919,10 +930,10 @@ struct cosims_mol {
     mutable std::mutex tune_mu;
     mutable std::map<int, std::pair<int, int>> tuned;
     mutable std::atomic<bool> on_device{false};  // device copy made (ensure_device succeeded)
-    // stream-ordered on the null stream (allocated there, filled by a
-    // synchronous copy): destroying a molecule does not wait for launches of
-    // other molecules
-    ~cosims_mol() { if (dmem) (void)hipFreeAsync(dmem, nullptr); }
+    // a block of the device cache (filled by a synchronous copy): destroying
+    // a molecule does not wait for launches of other molecules, as hipFree
+    // would; its launches have completed by then
+    ~cosims_mol() { if (dmem) dev_cache_put(ddev, dcap, dmem); }
 };
 
 namespace {
@@ -1064,7 +1075,7 @@ int upload(const cosims_mol *M)
         lj_constants(soa[4 * npad + i], soa[5 * npad + i], A, B);
         se[i] = make_float2(A, B);
     }
-    HIP_CHECK(hipMallocAsync(&M->dmem, total, nullptr));
+    HIP_CHECK(dev_cache_get(total, &M->ddev, &M->dcap, &M->dmem));
     M->dbytes = total;
     char *d = (char *)M->dmem;
     MolDev &m = M->md;

@@ -370,7 +370,7 @@ int set_inertia(Prepared &m)
 // ------------------------------------------------------------ boundary --
 // One march on the host, the loop of setBoundry (src/impactParam.cpp:666-690)
 // and of its z-axis search (:616-631).
-MarchHit march_host(const Prepared &m, double mk, const March &w)
+MarchHit march_host(const Prepared &m, double mk, const March &w, int64_t max_points)
 {
     MarchHit h{};
     V3 p(w.start[0], w.start[1], w.start[2]);
@@ -380,7 +380,7 @@ MarchHit march_host(const Prepared &m, double mk, const March &w)
     V3 pv(w.prev0[0], w.prev0[1], w.prev0[2]);
     if (w.first) p = p + v;
     const double sx = w.start[0], sy = w.start[1];
-    for (int64_t k = w.first; k < kMaxMarchPoints; k++) {
+    for (int64_t k = w.first; k < max_points; k++) {
         const double q[3] = {p.x, p.y, p.z};
         const double en = potential_host(m, q);
         if ((en + mk) * sl < 0) {
@@ -409,7 +409,7 @@ void march_all(const Prepared &m, double mk, const std::vector<March> &w, std::v
     std::atomic_int next{0};
     for (int t = 0; t < nthr; t++)
         pool.emplace_back([&]() {
-            for (int k; (k = next.fetch_add(1)) < (int)w.size();) out[k] = march_host(m, mk, w[k]);
+            for (int k; (k = next.fetch_add(1)) < (int)w.size();) out[k] = march_host(m, mk, w[k], kMaxMarchPoints);
         });
     for (auto &th : pool) th.join();
 }
@@ -470,7 +470,7 @@ int boundary_points(const Prepared &m, double min_kin, int threads, BoundaryMarc
     } else {
         // z axis: pos.z -= 0.5 * slope, i.e. the step (0, 0, -0.5) * slope
         March zw{{0, 0, bp.z0}, {0, 0, -0.5}, {0, 0, bp.z0}, 1, 0};
-        zh = march_host(m, mk, zw);
+        zh = march_host(m, mk, zw, zwalk_points(bp.z0));
         if (zh.found == 1) {
             const RayGeometry g = ray_geometry(bp, zh);
             std::vector<March> rays((size_t)(g.levels + 1) * kBoundaryRot);

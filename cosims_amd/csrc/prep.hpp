@@ -103,6 +103,16 @@ struct MarchHit {
     double p[3], prev[3];
 };
 constexpr int64_t kMaxMarchPoints = int64_t(1) << 22;
+// The z-axis walk (src/impactParam.cpp:640-655 loops until it crosses) ends
+// once it is kZWalkBeyond A past the far side of the molecule, z < -(z0 +
+// kZWalkBeyond) with z0 = |maxZ| + 20: a walk that finds no crossing there
+// is a failed setup.  Every crossing of a real molecule lies within a few
+// hundred points of the start; without the bound a walk that never crosses
+// would run 2^22 points, one wavefront for tens of seconds on a molecule
+// of thousands of atoms.  Rays end at the z axis (<= posmag / 0.1 + 2
+// points) and keep kMaxMarchPoints.
+constexpr double kZWalkBeyond = 1.0e4;
+inline int64_t zwalk_points(double z0) { return (int64_t)((2 * z0 + kZWalkBeyond) / 0.5) + 2; }
 
 // Inputs of the boundary search fixed before any march: the z-axis walk's
 // start, and what the rays' geometry needs once zMax is known

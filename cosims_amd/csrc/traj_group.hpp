@@ -716,6 +716,7 @@ struct GlbHeaders {
 struct LdsHeaders {
     const f4n *G0, *G1;
     const float *D;
+    const unsigned long long *M;  // the molecule's per-slice dipole bits (cdm)
     __device__ f4n g0(int c) const { return G0[c]; }
     __device__ f4n g1(int c) const { return G1[c]; }
     __device__ float dr2(int c) const { return D[c]; }
@@ -723,8 +724,12 @@ struct LdsHeaders {
     // dipole terms counted against the molecule's per-slice dipole bits (cdm)
     static constexpr bool kPrefetchDr2 = true;
 };
-// LDS bytes of the staged headers: cg0, cg1 (16 B each) and (R_d + rho)^2.
-__host__ __device__ constexpr size_t cl_header_bytes(int ncl) { return (size_t)ncl * 36; }
+// LDS bytes of the staged headers: cg0, cg1 (16 B each), (R_d + rho)^2 and
+// the dipole bits (8 B a 64-cluster slice, 8-byte aligned after the floats).
+__host__ __device__ constexpr size_t cl_header_bytes(int ncl)
+{
+    return ((size_t)ncl * 36 + 7) / 8 * 8 + (size_t)((ncl + 63) / 64) * 8;
+}
 
 template <int FT, class HD>
 __device__ __forceinline__ void wave_clustered32(const MolDev &m, const HD &hd, int ncl, int lane, float px, float py,
@@ -766,8 +771,9 @@ __device__ __forceinline__ void wave_clustered32(const MolDev &m, const HD &hd, 
         if (c < ncl) {
             const float dx = px - h.x, dy = py - h.y, dz = pz - h.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            // within (R_d + rho)^2 of this cluster (its LJ cut-off)
-            const bool disp = !(r2 > dpr_max) && !(r2 > (HD::kPrefetchDr2 ? d2 : hd.dr2(c)));
+            // within (R_d + rho)^2 of this cluster (its LJ cut-off); dpr_max
+            // bounds every cluster's, so it only spares the read of dr2
+            const bool disp = HD::kPrefetchDr2 ? !(r2 > d2) : (!(r2 > dpr_max) && !(r2 > hd.dr2(c)));
             if constexpr (kFar) {
                 if (r2 > h.w) {
                     cl32_far(g, dx, dy, dz, r2, s);
@@ -789,8 +795,8 @@ __device__ __forceinline__ void wave_clustered32(const MolDev &m, const HD &hd, 
         // the dipole is zeroed for a centre-replaced cluster and at order 0
         n_hdr += __popcll(__ballot(c < ncl));
         n_far += __popcll(__ballot(fr));
-        if constexpr (HD::kPrefetchDr2)  // one molecule a workgroup: a scalar load of the slice's dipole bits
-            n_fdip += __popcll(__ballot(fr) & m.cdm[base >> 6]);
+        if constexpr (HD::kPrefetchDr2)  // one molecule a workgroup: the slice's dipole bits from LDS
+            n_fdip += __popcll(__ballot(fr) & hd.M[base >> 6]);
         else
             n_fdip += __popcll(__ballot(fr && (g.y != 0.0f || g.z != 0.0f || g.w != 0.0f)));
         ex_n += __popcll(me);
@@ -1861,6 +1867,7 @@ __global__ void __launch_bounds__(1024) k_traj_cl_lds(GroupArgs A)
     f4n *h0 = (f4n *)(lds_cl + nw * (kClRingInts + kCountInts));
     f4n *h1 = h0 + ncl;
     float *d2 = (float *)(h1 + ncl);
+    unsigned long long *dm = (unsigned long long *)((char *)h0 + ((size_t)ncl * 36 + 7) / 8 * 8);
     const GLB f4n *G0 = (const GLB f4n *)glb(A.m.cg0);
     const GLB f4n *G1 = (const GLB f4n *)glb(A.m.cg1);
     const GLB f4n *G2 = (const GLB f4n *)glb(A.m.cg2);
@@ -1869,10 +1876,11 @@ __global__ void __launch_bounds__(1024) k_traj_cl_lds(GroupArgs A)
         h1[i] = G1[i];
         d2[i] = G2[i].x;
     }
+    for (int i = threadIdx.x; i < (ncl + 63) / 64; i += blockDim.x) dm[i] = A.m.cdm[i];
     __syncthreads();
     TrajIO io = A.io;
     resolve_range(io, A.first, A.last);
-    LdsClusterSource<FT> src{A, lists, W, LdsHeaders{h0, h1, d2}};
+    LdsClusterSource<FT> src{A, lists, W, LdsHeaders{h0, h1, d2, dm}};
     group_loop<64>(io, src);
 }
 

@@ -178,3 +178,12 @@ def check_large_ion_setup(g, m, clusters=True):
         assert digest(cl["atoms"], np.int32) == g["pcm_sha256"]["atoms"]
         assert digest(cl["center"], np.float64) == g["pcm_sha256"]["center"]
         assert digest(cl["maxdist"], np.float64) == g["pcm_sha256"]["maxdist"]
+
+
+def no_crossing_molecule(n, seed=7):
+    """A neutral molecule whose Lennard-Jones well (epsilon 1e-9 kcal/mol) is
+    far shallower than 1 % of the slowest gas particle's kinetic energy: the
+    potential never falls below the boundary threshold and the z-axis walk
+    of impactParam::setBoundry (src/impactParam.cpp:640-655) never crosses."""
+    xyz = np.random.default_rng(seed).normal(scale=6.0, size=(n, 3))
+    return xyz, np.full(n, 12.0), np.full(n, 3.4), np.full(n, 1e-9)

@@ -10,7 +10,7 @@ import oracle
 from oracle import molio
 from cosims_amd import engine as E
 
-from conftest import CASES, GOLDEN, ROOT, golden, golden_engine_setup
+from conftest import CASES, GOLDEN, ROOT, golden, golden_engine_setup, no_crossing_molecule
 
 
 def test_library_exports_every_declared_symbol():
@@ -117,6 +117,19 @@ def test_errors_are_reported():
     p = E.default_params(charge=1, multipole=1, dispersion_cutoff=1)
     with pytest.raises(E.CosimsError, match="clusters"):
         E.Molecule.from_prepared(p, np.zeros((1, 3)), [1.0], [1.0], [1.0], [1.0], 3.9, (10, 10, 10))
+
+
+def test_boundary_walk_without_crossing_fails_within_its_bound():
+    """The z-axis walk stops 10^4 A past the far side of the molecule; a walk
+    that finds no crossing is a setup error after ~2 x 10^4 points, not after
+    the 2^22 points of the rays' limit (2,000 atoms: ~20 s of potential
+    evaluations on one core before the bound)."""
+    import time
+    p = E.default_params(temp=298.0, multipole=0, dispersion_cutoff=0, setup_gpu=0, threads=1)
+    t = time.perf_counter()
+    with pytest.raises(E.CosimsError, match="no potential crossing on the z axis"):
+        E.Molecule.create(p, *no_crossing_molecule(2000))
+    assert time.perf_counter() - t < 5.0
 
 
 def test_unknown_atom_type_is_rejected(tmp_path):
