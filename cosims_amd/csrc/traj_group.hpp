@@ -27,6 +27,7 @@
 #pragma once
 
 #include "kernels.hpp"
+#include <type_traits>
 
 #define GLB __attribute__((address_space(1)))
 
@@ -368,59 +369,76 @@ __host__ __device__ inline void lj_constants(double s6, double e24, float &A, fl
 
 // FP64 pair terms of the group kernels (src/forceAccel.cpp:764-797,
 // 865-880) of one atom (x, y, z, q, sigma^6, 24 eps/mu), added to this
-// lane's partial sums: LJ tx ty tz 0-2, field dX dY dZ 3-5, gradient sXX sYY
-// sZZ sXY sXZ sYZ 6-11.  The reference's four IEEE divisions and square root
-// a pair (sigma^6 / r^6, / r^2, 1 / r^3, / r^2) become products with
-// ir = 1 / sqrt(r^2) and ir2 = ir^2, ir from v_rsq_f64 refined by two
-// Newton steps (within a few ulp of the quotients).  These kernels sum in
-// lane-split order and are not bit-identical to the reference anyway; the
-// lane kernel keeps the IEEE operations and the reference's order.
+// lane's partial sums: LJ 0-2, field 3-5, and the field gradient's
+// independent parts as in pair32 (q x^2 / r^5, q y^2 / r^5, q xy / r^5,
+// q xz / r^5, q yz / r^5 in 6-10 and q / r^3 in 11: the zz part follows from
+// the trace in the epilogue).  The reference's four IEEE divisions and
+// square root a pair become products with ir = 1 / sqrt(r^2) (v_rsq_f64
+// refined by two Newton steps, within a few ulp of the quotients; 1 / r^2
+// from v_rcp_f64 the same way when no odd power is needed), the sums are
+// fused multiply-adds, and the LJ term is e24 t (2 t - 1) / r^2 with t =
+// sigma^6 / r^6: 39 FP64 operations a pair for force type 4 where the
+// reference's form takes 64.  These kernels sum in lane-split order and are
+// not bit-identical to the reference anyway; the lane kernel keeps the IEEE
+// operations and the reference's order.
+#pragma clang fp contract(fast)
 template <int FT>
 __device__ __forceinline__ void pair64(double ax, double ay, double az, double aq, double as6, double ae24,
                                        double px, double py, double pz, double (&s)[12])
 {
     const double xD = px - ax, yD = py - ay, zD = pz - az;
-    const double x2 = xD * xD, y2 = yD * yD, z2 = zD * zD;
-    const double r2 = x2 + y2 + z2;
-    const double ir = rsqrt64(r2);
-    const double ir2 = ir * ir;
-    const double t2 = as6 * (ir2 * ir2 * ir2);
-    const double t1 = 2 * t2 * t2;
-    const double sc = ae24 * (t1 - t2) * ir2;
+    const double r2 = xD * xD + yD * yD + zD * zD;
+    double ir = 0, ir2;
+    if constexpr (FT == 4) {
+        ir = rsqrt64(r2);
+        ir2 = ir * ir;
+    } else {
+        ir2 = rcp64(r2);
+    }
+    const double t = as6 * (ir2 * ir2 * ir2);
+    const double sc = ae24 * t * (2.0 * t - 1.0) * ir2;
     s[0] += sc * xD; s[1] += sc * yD; s[2] += sc * zD;
     if constexpr (FT == 4) {
-        const double r_3 = aq * (ir * ir2);
-        const double r_5 = r_3 * ir2;
-        s[3] += xD * r_3; s[4] += yD * r_3; s[5] += zD * r_3;
-        s[6] += (3.0 * x2 - r2) * r_5;
-        s[7] += (3.0 * y2 - r2) * r_5;
-        s[8] += (3.0 * z2 - r2) * r_5;
-        s[9] += 3 * xD * yD * r_5;
-        s[10] += 3 * xD * zD * r_5;
-        s[11] += 3 * yD * zD * r_5;
+        const double qr3 = aq * (ir * ir2);
+        s[3] += xD * qr3; s[4] += yD * qr3; s[5] += zD * qr3;
+        s[11] += qr3;
+        const double qr5 = qr3 * ir2;
+        const double u = qr5 * xD, v = qr5 * yD;
+        s[6] += u * xD; s[7] += v * yD;
+        s[8] += u * yD; s[9] += u * zD; s[10] += v * zD;
     }
 }
+#pragma clang fp contract(off)
 
-// Group sums of the FP64 partials and the epilogue (:800-802).
+// Group sums of the FP64 partials and the epilogue (:800-802): the gradient
+// tensor T_ab = sum q (3 a b - r^2 delta_ab) / r^5 from the partials, its
+// zz part from the trace (T_xx + T_yy + T_zz = 0).
 template <int FT, int L>
 __device__ __forceinline__ void force64_epilogue(double (&s)[12], double ds, double &ox, double &oy, double &oz)
 {
     double tx = gsum<L>(s[0]), ty = gsum<L>(s[1]), tz = gsum<L>(s[2]);
     if constexpr (FT == 4) {
         const double dX = gsum<L>(s[3]), dY = gsum<L>(s[4]), dZ = gsum<L>(s[5]);
-        const double sXX = gsum<L>(s[6]), sYY = gsum<L>(s[7]), sZZ = gsum<L>(s[8]);
-        const double sXY = gsum<L>(s[9]), sXZ = gsum<L>(s[10]), sYZ = gsum<L>(s[11]);
-        tx -= ds * (sXX * dX + sXY * dY + sXZ * dZ);
-        ty -= ds * (sXY * dX + sYY * dY + sYZ * dZ);
-        tz -= ds * (sXZ * dX + sYZ * dY + sZZ * dZ);
+        const double Sxx = gsum<L>(s[6]), Syy = gsum<L>(s[7]), t = gsum<L>(s[11]);
+        const double Sxy = gsum<L>(s[8]), Sxz = gsum<L>(s[9]), Syz = gsum<L>(s[10]);
+        const double Txx = 3.0 * Sxx - t, Tyy = 3.0 * Syy - t, Tzz = 2.0 * t - 3.0 * Sxx - 3.0 * Syy;
+        const double Txy = 3.0 * Sxy, Txz = 3.0 * Sxz, Tyz = 3.0 * Syz;
+        tx -= ds * (Txx * dX + Txy * dY + Txz * dZ);
+        ty -= ds * (Txy * dX + Tyy * dY + Tyz * dZ);
+        tz -= ds * (Txz * dX + Tyz * dY + Tzz * dZ);
     }
     ox = tx; oy = ty; oz = tz;
 }
 
-// FP64 pair terms over atoms j, j+L, ... < npad of the SoA X (x y z q s6 e24
-// with stride npad), then the group sums and the epilogue: pair64 and
-// force64_epilogue, the same operations as the tiled kernel's, so the LDS,
-// table and tiled kernels give the same bits.
+// FP64 pair terms over atoms j, j + L, ... < npad of the SoA X (x y z q s6
+// e24 with stride npad), then the group sums and the epilogue: pair64 and
+// force64_epilogue, the same pairs in the same order as the tiled kernel's,
+// so the LDS, table and tiled kernels give the same bits.  From LDS (X a
+// generic pointer): npad is a multiple of 64, so every lane makes the same
+// npad / L passes, and the next atom is read into registers of its own one
+// pair ahead, so the reads of the unrolled pairs go out together.  From
+// global memory (the table kernel), and in the tiled kernel, whose other
+// live state leaves no room for the prefetch registers, the plain loop.
 template <int FT, int L, typename PD>
 __device__ __forceinline__ void grp_force64(PD X, int npad, int j, double px, double py, double pz, double ds,
                                             double &ox, double &oy, double &oz)
@@ -429,7 +447,20 @@ __device__ __forceinline__ void grp_force64(PD X, int npad, int j, double px, do
     double s[12];
 #pragma unroll
     for (int k = 0; k < 12; k++) s[k] = 0.0;
-    for (int k = j; k < npad; k += L) pair64<FT>(X[k], Y[k], Z[k], Q[k], S6[k], E24[k], px, py, pz, s);
+    const int cnt = npad / L;
+    if constexpr (!std::is_same<PD, const double *>::value) {
+        for (int k = j; k < npad; k += L) pair64<FT>(X[k], Y[k], Z[k], Q[k], S6[k], E24[k], px, py, pz, s);
+    } else if (cnt > 0) {
+        double ax = X[j], ay = Y[j], az = Z[j], aq = Q[j], as = S6[j], ae = E24[j];
+#pragma unroll 4
+        for (int i = 1; i < cnt; i++) {
+            const int k = j + i * L;
+            const double bx = X[k], by = Y[k], bz = Z[k], bq = Q[k], bs = S6[k], be = E24[k];
+            pair64<FT>(ax, ay, az, aq, as, ae, px, py, pz, s);
+            ax = bx; ay = by; az = bz; aq = bq; as = bs; ae = be;
+        }
+        pair64<FT>(ax, ay, az, aq, as, ae, px, py, pz, s);
+    }
     force64_epilogue<FT, L>(s, ds, ox, oy, oz);
 }
 
