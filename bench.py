@@ -546,7 +546,9 @@ def main():
         out["fp64_value"] = n_traj / (fp64_s)
         out["mixed_over_fp64"] = value / out["fp64_value"]
         out["fp64_kernel_ms"] = fp64["traj_kernel_ms"][0]
-        out["fp64_launch"] = {"lanes_per_trajectory": fp64["lanes"], "wg_size": fp64["wg_size"]}
+        out["fp64_launch"] = {"lanes_per_trajectory": fp64["lanes"], "wg_size": fp64["wg_size"],
+                              "kernel": f"k_traj_grp<fp64,type4,{fp64['lanes']}>",
+                              **launch_roofline(fp64, PEAK_FP64)}
     threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
     if a.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(m, a.cpu_seconds, threads)
