@@ -799,22 +799,27 @@ __device__ __forceinline__ void wave_clustered32(const MolDev &m, const HD &hd, 
         }
         int kex = 0, klj = 0;  // exact: 1 charges, 2 charges + LJ; LJ: 1 per-atom, 2 fixed parameters
         bool fr = false;
-        if (c < ncl) {
+        if constexpr (kFar) {
+            // nearly every cluster is far: every lane adds far-field terms, a
+            // near cluster's (and a lane past the last cluster's) with zero
+            // moments and r^2 = 1, so that each of its terms is exactly +-0
+            // and leaves the sums' bits as they were; no branch around them
+            const bool valid = c < ncl;
+            const float dx = px - h.x, dy = py - h.y, dz = pz - h.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const bool disp = HD::kPrefetchDr2 ? !(r2 > d2) : (!(r2 > dpr_max) && !(r2 > hd.dr2(min(c, ncl - 1))));
+            fr = valid && r2 > h.w;
+            const f4n gz = fr ? g : f4n{0.0f, 0.0f, 0.0f, 0.0f};
+            cl32_far(gz, dx, dy, dz, fr ? r2 : 1.0f, s);
+            if (fr) klj = FT == 2 ? 1 : (disp ? 2 : 0);
+            else if (valid) kex = (FT == 2 || disp) ? 2 : 1;
+        } else if (c < ncl) {
             const float dx = px - h.x, dy = py - h.y, dz = pz - h.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             // within (R_d + rho)^2 of this cluster (its LJ cut-off); dpr_max
             // bounds every cluster's, so it only spares the read of dr2
             const bool disp = HD::kPrefetchDr2 ? !(r2 > d2) : (!(r2 > dpr_max) && !(r2 > hd.dr2(c)));
-            if constexpr (kFar) {
-                if (r2 > h.w) {
-                    cl32_far(g, dx, dy, dz, r2, s);
-                    fr = true;
-                    if (FT == 2) klj = 1;
-                    else if (disp) klj = 2;
-                } else {
-                    kex = (FT == 2 || disp) ? 2 : 1;
-                }
-            } else if constexpr (FT == 3) {
+            if constexpr (FT == 3) {
                 kex = disp ? 2 : 1;
             } else {
                 if (disp) klj = 1;
@@ -1035,21 +1040,25 @@ __device__ __forceinline__ void wave_clustered64(const MolDev &m, int ncl, int l
         }
         int kex = 0, klj = 0;  // exact: 1 charges, 2 charges + LJ; LJ: 1 per-atom, 2 fixed parameters
         bool fr = false;
-        if (c < ncl) {
+        if constexpr (kFar) {
+            // as in wave_clustered32: every lane adds far-field terms, a near
+            // cluster's (and a lane past the last cluster's) with zero moments
+            // and r^2 = 1, each term exactly +-0; no branch around them
+            const bool valid = c < ncl;
+            const double dx = px - h.x, dy = py - h.y, dz = pz - h.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const bool disp = !(r2 > dpr_max) && !(r2 > G2[min(c, ncl - 1)].x);
+            fr = valid && r2 > h.w;
+            const d4n gz = fr ? g : d4n{0.0, 0.0, 0.0, 0.0};
+            cl64_far(gz, dx, dy, dz, fr ? r2 : 1.0, s);
+            if (fr) klj = FT == 2 ? 1 : (disp ? 2 : 0);
+            else if (valid) kex = (FT == 2 || disp) ? 2 : 1;
+        } else if (c < ncl) {
             const double dx = px - h.x, dy = py - h.y, dz = pz - h.z;
             const double r2 = dx * dx + dy * dy + dz * dz;
             // within (R_d + rho)^2 of this cluster (its LJ cut-off)
             const bool disp = !(r2 > dpr_max) && !(r2 > G2[c].x);
-            if constexpr (kFar) {
-                if (r2 > h.w) {
-                    cl64_far(g, dx, dy, dz, r2, s);
-                    fr = true;
-                    if (FT == 2) klj = 1;
-                    else if (disp) klj = 2;
-                } else {
-                    kex = (FT == 2 || disp) ? 2 : 1;
-                }
-            } else if constexpr (FT == 3) {
+            if constexpr (FT == 3) {
                 kex = disp ? 2 : 1;
             } else {
                 if (disp) klj = 1;
