@@ -1027,6 +1027,7 @@ int upload(const cosims_mol *M)
     auto *dg2 = (double *)(host.data() + odg2);
     float dpr_max = 0.0f;
     double dpr_max64 = 0.0;
+    int ndip = 0, ndip64 = 0;
     for (int c = 0; c < ncl; c++) {
         const ClusterF32 &h = h32[c];
         dpr_max = std::max(dpr_max, h.dpr);
@@ -1039,6 +1040,8 @@ int upload(const cosims_mol *M)
         double *e0 = dg0 + 4 * c, *e1 = dg1 + 4 * c, *e2 = dg2 + 4 * c;
         e0[0] = d.px; e0[1] = d.py; e0[2] = d.pz; e0[3] = d.mpr;
         e1[0] = d.q; e1[1] = dip ? d.d0 : 0.0; e1[2] = dip ? d.d1 : 0.0; e1[3] = dip ? d.d2 : 0.0;
+        ndip += (cdm[c >> 6] >> (c & 63)) & 1;
+        ndip64 += e1[1] != 0.0 || e1[2] != 0.0 || e1[3] != 0.0;
         e2[0] = d.dpr; e2[1] = d.qs6; e2[2] = d.qe24; e2[3] = 0.0;
         float A, B;
         const int qi = c < n ? c : 0;
@@ -1093,6 +1096,8 @@ int upload(const cosims_mol *M)
     m.dg0 = (const double *)(d + odg0); m.dg1 = (const double *)(d + odg1); m.dg2 = (const double *)(d + odg2);
     m.cl_dpr_max64 = dpr_max64;
     m.cl_dpr_max = dpr_max;
+    m.cl_ndip = ndip;
+    m.cl_ndip64 = ndip64;
     M->soa = (const double *)(d + osoa);
     M->npad = npad;
     MolRec &r = M->rec;

@@ -65,6 +65,7 @@ struct MolDev {
     const double *dg0, *dg1, *dg2;
     double cl_dpr_max64;  // largest (R_d + rho)^2 of dg2
     float cl_dpr_max;     // largest (R_d + rho)^2 of cg2 (FP32)
+    int cl_ndip, cl_ndip64;  // clusters with a nonzero dipole in cg1 (the bits of cdm), in dg1
 };
 
 // Outside the boundary ellipsoid (src/diffEqSolver.cpp:99, 174): the

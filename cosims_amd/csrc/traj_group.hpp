@@ -733,15 +733,18 @@ __device__ __forceinline__ void cl32_flush(const MolDev &m, const int *ring, int
 // the (R_d + rho)^2 of cg2 staged in LDS by the workgroup.
 struct GlbHeaders {
     const GLB f4n *G0, *G1, *G2;
+    const GLB unsigned long long *M;
     __device__ explicit GlbHeaders(const MolDev &m)
-        : G0((const GLB f4n *)glb(m.cg0)), G1((const GLB f4n *)glb(m.cg1)), G2((const GLB f4n *)glb(m.cg2))
+        : G0((const GLB f4n *)glb(m.cg0)), G1((const GLB f4n *)glb(m.cg1)), G2((const GLB f4n *)glb(m.cg2)),
+          M((const GLB unsigned long long *)glb(m.cdm))
     {
     }
     __device__ f4n g0(int c) const { return G0[c]; }
     __device__ f4n g1(int c) const { return G1[c]; }
     __device__ float dr2(int c) const { return G2[c].x; }
-    // (R_d + rho)^2 read only where the LJ cut-off decides; the far clusters
-    // with dipole terms counted from the headers
+    // dipole bits of 64-cluster slice k (cdm)
+    __device__ unsigned long long dip(int k) const { return M[k]; }
+    // (R_d + rho)^2 read only where the LJ cut-off decides
     static constexpr bool kPrefetchDr2 = false;
 };
 struct LdsHeaders {
@@ -751,8 +754,8 @@ struct LdsHeaders {
     __device__ f4n g0(int c) const { return G0[c]; }
     __device__ f4n g1(int c) const { return G1[c]; }
     __device__ float dr2(int c) const { return D[c]; }
-    // (R_d + rho)^2 read with the next slice's headers; the far clusters with
-    // dipole terms counted against the molecule's per-slice dipole bits (cdm)
+    __device__ unsigned long long dip(int k) const { return M[k]; }
+    // (R_d + rho)^2 read with the next slice's headers
     static constexpr bool kPrefetchDr2 = true;
 };
 // LDS bytes of the staged headers: cg0, cg1 (16 B each), (R_d + rho)^2 and
@@ -767,7 +770,14 @@ __device__ __forceinline__ void wave_clustered32(const MolDev &m, const HD &hd, 
                                                  float pz, int *lists, double &ox, double &oy, double &oz,
                                                  const WaveCount &W)
 {
-    unsigned n_hdr = 0, n_far = 0, n_fdip = 0;  // wave-uniform
+    // one trajectory a wavefront: the cluster count (0 for an idle group) is
+    // the same on every lane; read as a scalar, the slice loop and its ring
+    // counters stay scalar
+    ncl = __builtin_amdgcn_readfirstlane(ncl);
+    // branch counts: every cluster is examined once an evaluation and nearly
+    // every one is far, so the slices count the exceptions, the clusters
+    // that are not far and those of them with a dipole (wave-uniform)
+    unsigned n_near = 0, n_near_dip = 0;
     constexpr bool kFar = FT == 1 || FT == 2;
     // s: LJ force 0-2, field 3-5, M xx yy zz xy xz yz 6-11, trace 12, antisymmetric xy xz yz 13-15
     float s[16];
@@ -828,13 +838,14 @@ __device__ __forceinline__ void wave_clustered32(const MolDev &m, const HD &hd, 
         const unsigned long long me = __ballot(kex != 0), ml = __ballot(klj != 0);
         if (kex) ex_ring[(ex_head + ex_n + __popcll(me & below)) & (kRing - 1)] = 4 * c + kex;
         if (klj) lj_ring[(lj_head + lj_n + __popcll(ml & below)) & (kRing - 1)] = 4 * c + klj;
-        // the dipole is zeroed for a centre-replaced cluster and at order 0
-        n_hdr += __popcll(__ballot(c < ncl));
-        n_far += __popcll(__ballot(fr));
-        if constexpr (HD::kPrefetchDr2)  // one molecule a workgroup: the slice's dipole bits from LDS
-            n_fdip += __popcll(__ballot(fr) & hd.M[base >> 6]);
-        else
-            n_fdip += __popcll(__ballot(fr && (g.y != 0.0f || g.z != 0.0f || g.w != 0.0f)));
+        if constexpr (kFar) {
+            // the dipole is zeroed for a centre-replaced cluster and at order 0
+            const unsigned long long nf = __ballot(c < ncl && !fr);
+            if (nf) {
+                n_near += __popcll(nf);
+                n_near_dip += __popcll(nf & hd.dip(base >> 6));
+            }
+        }
         ex_n += __popcll(me);
         lj_n += __popcll(ml);
         wave_sync_lds();
@@ -861,9 +872,9 @@ __device__ __forceinline__ void wave_clustered32(const MolDev &m, const HD &hd, 
     }
     if (ex_n) cl32_flush<true>(m, ex_ring, ex_head, ex_n, lane, px, py, pz, s, W);
     if (lj_n) cl32_flush<false>(m, lj_ring, lj_head, lj_n, lane, px, py, pz, s, W);
-    W.wave_add(0, n_hdr);
-    W.wave_add(1, n_far);
-    W.wave_add(2, n_fdip);
+    W.wave_add(0, ncl);
+    W.wave_add(1, kFar ? ncl - n_near : 0);
+    W.wave_add(2, kFar && ncl > 0 ? m.cl_ndip - n_near_dip : 0);
     wave_sync_lds();
 #pragma unroll
     for (int k = 0; k < 16; k++) {
@@ -1010,7 +1021,8 @@ template <int FT>
 __device__ __forceinline__ void wave_clustered64(const MolDev &m, int ncl, int lane, double px, double py, double pz,
                                                  int *lists, double &ox, double &oy, double &oz, const WaveCount &W)
 {
-    unsigned n_hdr = 0, n_far = 0, n_fdip = 0;  // wave-uniform
+    ncl = __builtin_amdgcn_readfirstlane(ncl);  // wave-uniform, as in wave_clustered32
+    unsigned n_near = 0, n_near_dip = 0;  // wave-uniform; the exceptions, as in wave_clustered32
     constexpr bool kFar = FT == 1 || FT == 2;
     // s: LJ force 0-2, field 3-5, M xx yy zz xy xz yz 6-11, trace 12, antisymmetric xy xz yz 13-15
     double s[16];
@@ -1067,9 +1079,14 @@ __device__ __forceinline__ void wave_clustered64(const MolDev &m, int ncl, int l
         const unsigned long long me = __ballot(kex != 0), ml = __ballot(klj != 0);
         if (kex) ex_ring[(ex_head + ex_n + __popcll(me & below)) & (kRing - 1)] = 4 * c + kex;
         if (klj) lj_ring[(lj_head + lj_n + __popcll(ml & below)) & (kRing - 1)] = 4 * c + klj;
-        n_hdr += __popcll(__ballot(c < ncl));
-        n_far += __popcll(__ballot(fr));
-        n_fdip += __popcll(__ballot(fr && (g.y != 0.0 || g.z != 0.0 || g.w != 0.0)));
+        if constexpr (kFar) {
+            const bool near = c < ncl && !fr;
+            const unsigned long long nf = __ballot(near);
+            if (nf) {
+                n_near += __popcll(nf);
+                n_near_dip += __popcll(__ballot(near && (g.y != 0.0 || g.z != 0.0 || g.w != 0.0)));
+            }
+        }
         ex_n += __popcll(me);
         lj_n += __popcll(ml);
         wave_sync_lds();
@@ -1088,9 +1105,9 @@ __device__ __forceinline__ void wave_clustered64(const MolDev &m, int ncl, int l
     }
     if (ex_n) cl64_flush<true>(m, ex_ring, ex_head, ex_n, lane, px, py, pz, s, W);
     if (lj_n) cl64_flush<false>(m, lj_ring, lj_head, lj_n, lane, px, py, pz, s, W);
-    W.wave_add(0, n_hdr);
-    W.wave_add(1, n_far);
-    W.wave_add(2, n_fdip);
+    W.wave_add(0, ncl);
+    W.wave_add(1, kFar ? ncl - n_near : 0);
+    W.wave_add(2, kFar && ncl > 0 ? m.cl_ndip64 - n_near_dip : 0);
     wave_sync_lds();
 #pragma unroll
     for (int k = 0; k < 16; k++) {
