@@ -807,8 +807,9 @@ __device__ __forceinline__ void wave_clustered32(const MolDev &m, const HD &hd, 
                 if (kFar) n1 = hd.g1(cn);
             }
         }
-        int kex = 0, klj = 0;  // exact: 1 charges, 2 charges + LJ; LJ: 1 per-atom, 2 fixed parameters
-        bool fr = false;
+        // ring entries 4 c + kind; exact: 1 charges, 2 charges + LJ; LJ: 1 per-atom, 2 fixed parameters
+        int kex = 0, klj = 0;
+        bool fr = false, near = false, ljc = false, disp = false;
         if constexpr (kFar) {
             // nearly every cluster is far: every lane adds far-field terms, a
             // near cluster's (and a lane past the last cluster's) with zero
@@ -817,33 +818,43 @@ __device__ __forceinline__ void wave_clustered32(const MolDev &m, const HD &hd, 
             const bool valid = c < ncl;
             const float dx = px - h.x, dy = py - h.y, dz = pz - h.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            const bool disp = HD::kPrefetchDr2 ? !(r2 > d2) : (!(r2 > dpr_max) && !(r2 > hd.dr2(min(c, ncl - 1))));
+            disp = HD::kPrefetchDr2 ? !(r2 > d2) : (!(r2 > dpr_max) && !(r2 > hd.dr2(min(c, ncl - 1))));
             fr = valid && r2 > h.w;
+            near = valid && !fr;
+            ljc = FT == 2 ? fr : (fr && disp);
             const f4n gz = fr ? g : f4n{0.0f, 0.0f, 0.0f, 0.0f};
             cl32_far(gz, dx, dy, dz, fr ? r2 : 1.0f, s);
-            if (fr) klj = FT == 2 ? 1 : (disp ? 2 : 0);
-            else if (valid) kex = (FT == 2 || disp) ? 2 : 1;
         } else if (c < ncl) {
             const float dx = px - h.x, dy = py - h.y, dz = pz - h.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             // within (R_d + rho)^2 of this cluster (its LJ cut-off); dpr_max
             // bounds every cluster's, so it only spares the read of dr2
-            const bool disp = HD::kPrefetchDr2 ? !(r2 > d2) : (!(r2 > dpr_max) && !(r2 > hd.dr2(c)));
+            disp = HD::kPrefetchDr2 ? !(r2 > d2) : (!(r2 > dpr_max) && !(r2 > hd.dr2(c)));
             if constexpr (FT == 3) {
                 kex = disp ? 2 : 1;
             } else {
                 if (disp) klj = 1;
             }
+            near = kex != 0;
+            ljc = klj != 0;
         }
-        const unsigned long long me = __ballot(kex != 0), ml = __ballot(klj != 0);
-        if (kex) ex_ring[(ex_head + ex_n + __popcll(me & below)) & (kRing - 1)] = 4 * c + kex;
-        if (klj) lj_ring[(lj_head + lj_n + __popcll(ml & below)) & (kRing - 1)] = 4 * c + klj;
+        // the ballots come straight from the conditions; a ring entry's kind
+        // is formed only on the (rare) lanes that append
+        const unsigned long long me = __ballot(near), ml = __ballot(ljc);
+        if (near) {
+            if constexpr (kFar) kex = (FT == 2 || disp) ? 2 : 1;
+            ex_ring[(ex_head + ex_n + __popcll(me & below)) & (kRing - 1)] = 4 * c + kex;
+        }
+        if (ljc) {
+            if constexpr (kFar) klj = FT == 2 ? 1 : 2;
+            lj_ring[(lj_head + lj_n + __popcll(ml & below)) & (kRing - 1)] = 4 * c + klj;
+        }
         if constexpr (kFar) {
-            // the dipole is zeroed for a centre-replaced cluster and at order 0
-            const unsigned long long nf = __ballot(c < ncl && !fr);
-            if (nf) {
-                n_near += __popcll(nf);
-                n_near_dip += __popcll(nf & hd.dip(base >> 6));
+            // the clusters that are not far (me), and those with a dipole (the
+            // dipole is zeroed for a centre-replaced cluster and at order 0)
+            if (me) {
+                n_near += __popcll(me);
+                n_near_dip += __popcll(me & hd.dip(base >> 6));
             }
         }
         ex_n += __popcll(me);
