@@ -1364,9 +1364,25 @@ struct TileSource {
                 }
                 __syncthreads();
                 if (wave) {
-                    for (int k = j; k < cnt; k += L)
-                        pair64<FT>(X[k], X[T + k], X[2 * T + k], X[3 * T + k], X[4 * T + k], X[5 * T + k], ex, ey,
-                                   ez, s);
+                    // cnt and T are multiples of 64: every lane makes cnt / L
+                    // passes; at 32 and 64 lanes two pairs a pass (their atoms
+                    // read two at a time), with fewer lanes (more state live)
+                    // one, which needs no registers beyond 128
+                    const int passes = cnt / L;
+                    if constexpr (L >= 32) {
+#pragma unroll 2
+                        for (int i = 0; i < passes; i++) {
+                            const int k = j + i * L;
+                            pair64<FT>(X[k], X[T + k], X[2 * T + k], X[3 * T + k], X[4 * T + k], X[5 * T + k], ex,
+                                       ey, ez, s);
+                        }
+                    } else {
+                        for (int i = 0; i < passes; i++) {
+                            const int k = j + i * L;
+                            pair64<FT>(X[k], X[T + k], X[2 * T + k], X[3 * T + k], X[4 * T + k], X[5 * T + k], ex,
+                                       ey, ez, s);
+                        }
+                    }
                 }
             }
             if (wave) force64_epilogue<FT, L>(s, A.m.dipole_scalar, fx, fy, fz);
