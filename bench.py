@@ -406,6 +406,9 @@ def main():
                     help="requests of a stream with the reference's default (clustered) force field (0: none)")
     ap.add_argument("--serve-clustered-rate", type=float, default=1.0)
     ap.add_argument("--serve-target", type=float, default=0.005)
+    ap.add_argument("--serve-warmup", type=int, default=40,
+                    help="requests of an unmeasured stream (20 req/s, its own seed) before the measured ones: a "
+                         "process's first stream runs ~20 %% slower while the device cache fills (0: none)")
     ap.add_argument("--serve-fp64-atoms", type=int, default=1000,
                     help="rerun served requests of up to this many atoms in FP64 over the same blocks (0: none)")
     ap.add_argument("--serve-class-wg", type=str, default="0,0,0,1024,1024",
@@ -557,6 +560,10 @@ def main():
         cwg = [int(x) for x in a.serve_class_wg.split(",")]
         cl = [int(x) for x in a.serve_class_lanes.split(",")]
         own = bool(a.serve_own_launches)
+        if a.serve_warmup > 0:
+            wu = serving(E, synth, a.serve_warmup, 20.0, a.serve_target, seed=7000, class_wg=cwg, class_lanes=cl,
+                         own_launches=own)
+            log(f"serving warm-up: {a.serve_warmup} requests, {wu['trajectories_per_s']:.0f} traj/s (not reported)")
         sv = serving(E, synth, a.serve_requests, a.serve_rate, a.serve_target, fp64_max_atoms=a.serve_fp64_atoms,
                      class_wg=cwg, class_lanes=cl, own_launches=own)
         out["serving"] = sv

@@ -331,15 +331,44 @@ std::vector<SliceEntry> plan_slice(cosims_server *S, const std::vector<std::shar
 }
 
 // Per pipeline place: the descriptors of a slice's per-block CCS and
-// stopping-rule kernels and the staging record the host reads back.
+// stopping-rule kernels and the staging record the host reads back (into
+// pinned memory: the read-back is queued with the slice, and a copy into
+// pageable memory could hold the issuing thread until it lands).
 struct SliceDev {
     DBuf bd, sd, stage, head;
     size_t bd_cap = 0, sd_cap = 0, stage_cap = 0, head_cap = 0;
     std::vector<BlockDesc> bdh;
     std::vector<StopDesc> sdh;
-    std::vector<double> stage_h;
-    std::vector<int32_t> head_h;
+    double *stage_h = nullptr;
+    int32_t *head_h = nullptr;
+    size_t stage_h_cap = 0, head_h_cap = 0, nd = 0;
+    hipEvent_t stop_ev = nullptr;  // recorded after this place's k_stop
+    SliceDev() = default;
+    SliceDev(const SliceDev &) = delete;
+    SliceDev &operator=(const SliceDev &) = delete;
+    ~SliceDev()
+    {
+        if (stage_h) (void)hipHostFree(stage_h);
+        if (head_h) (void)hipHostFree(head_h);
+        if (stop_ev) (void)hipEventDestroy(stop_ev);
+    }
 };
+
+// Pinned host capacity for n elements; grown rarely (at least 4096
+// elements, doubling), since freeing pinned memory waits for the device.
+template <class T>
+int reserve_pinned(T *&p, size_t &cap, size_t n)
+{
+    if (cap >= n) return 0;
+    size_t c = std::max<size_t>(4096, cap);
+    while (c < n) c *= 2;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    HIP_CHECK(hipHostMalloc((void **)&p, sizeof(T) * c, hipHostMallocDefault));
+    cap = c;
+    return 0;
+}
 
 // A slice in flight: its plan, device buffers and stream.
 struct Slice {
@@ -363,7 +392,7 @@ int reserve_dev(DBuf &b, size_t &cap, size_t n)
 
 // Issues one slice (sampling, launches, per-block reduction into each
 // request's device sums) without waiting for the launches.
-int issue_slice(cosims_server *S, Slice &X)
+int issue_slice(cosims_server *S, Slice &X, hipEvent_t &last_stop)
 {
     const auto &c = S->cfg;
     BatchPlan &P = *X.P;
@@ -432,27 +461,24 @@ int issue_slice(cosims_server *S, Slice &X)
     HIP_CHECK(hipMemcpyAsync(D.bd.p, D.bdh.data(), sizeof(BlockDesc) * D.bdh.size(), hipMemcpyHostToDevice, X.s));
     hipLaunchKernelGGL(k_block_ccs, dim3((unsigned)D.bdh.size()), dim3(256), 0, X.s, D.bd.as<BlockDesc>());
     HIP_CHECK(hipGetLastError());
-    return 0;
-}
-
-// Waits for a slice, then applies the stopping rule on the device (k_stop:
-// every request of the slice, its blocks in block order; blocks of finished
-// requests are ignored) and reads back one staging record: the blocks each
-// request took, their CCS, errors and sums, and whether it has stopped.
-int collect_slice(cosims_server *S, Slice &X)
-{
-    const auto &c = S->cfg;
-    BatchPlan &P = *X.P;
-    SliceDev &D = *X.D;
-    GroupRun R;
-    if (int rc = launch_finish(P, X.s, R)) return rc;
+    // The stopping rule (k_stop: every request of the slice, its blocks in
+    // block order; blocks of finished requests are ignored) and the
+    // read-back of its staging record follow in the slice's stream, so they
+    // start as the slice's workgroups drain; launched when the host collects
+    // the slice, they waited ~40 ms for a CU slot behind the next slices'
+    // persistent workgroups.  Each slice's k_stop runs after the previous
+    // slice's (a request's blocks are taken in block order).
     const size_t ne = X.e.size();
     D.sdh.resize(ne);
     size_t nd = 0;  // staging doubles
     for (size_t i = 0; i < ne; i++) nd += 1 + (size_t)(2 + kSums) * X.e[i].nb;
-    if (int rc = reserve_dev<double>(D.stage, D.stage_cap, std::max<size_t>(1, nd))) return rc;
+    nd = std::max<size_t>(1, nd);
+    if (int rc = reserve_dev<double>(D.stage, D.stage_cap, nd)) return rc;
     if (int rc = reserve_dev<int32_t>(D.head, D.head_cap, 3 * std::max<size_t>(1, ne))) return rc;
     if (int rc = reserve_dev<StopDesc>(D.sd, D.sd_cap, std::max<size_t>(1, ne))) return rc;
+    if (int rc = reserve_pinned<double>(D.stage_h, D.stage_h_cap, nd)) return rc;
+    if (int rc = reserve_pinned<int32_t>(D.head_h, D.head_h_cap, 3 * std::max<size_t>(1, ne))) return rc;
+    D.nd = nd;
     size_t off = 0;
     for (size_t i = 0; i < ne; i++) {
         SJob &J = *X.e[i].job;
@@ -461,20 +487,36 @@ int collect_slice(cosims_server *S, Slice &X)
         off += 1 + (size_t)(2 + kSums) * X.e[i].nb;
     }
     HIP_CHECK(hipMemcpyAsync(D.sd.p, D.sdh.data(), sizeof(StopDesc) * ne, hipMemcpyHostToDevice, X.s));
+    if (last_stop) HIP_CHECK(hipStreamWaitEvent(X.s, last_stop, 0));
     hipLaunchKernelGGL(k_stop, dim3((unsigned)((ne + 63) / 64)), dim3(64), 0, X.s, D.sd.as<StopDesc>(), (int)ne);
     HIP_CHECK(hipGetLastError());
-    D.stage_h.resize(std::max<size_t>(1, nd));
-    D.head_h.resize(3 * ne);
-    HIP_CHECK(hipMemcpyAsync(D.stage_h.data(), D.stage.p, sizeof(double) * nd, hipMemcpyDeviceToHost, X.s));
-    HIP_CHECK(hipMemcpyAsync(D.head_h.data(), D.head.p, sizeof(int32_t) * 3 * ne, hipMemcpyDeviceToHost, X.s));
-    HIP_CHECK(hipStreamSynchronize(X.s));
-    off = 0;
+    if (!D.stop_ev) HIP_CHECK(hipEventCreateWithFlags(&D.stop_ev, hipEventDisableTiming));
+    HIP_CHECK(hipEventRecord(D.stop_ev, X.s));
+    last_stop = D.stop_ev;
+    HIP_CHECK(hipMemcpyAsync(D.stage_h, D.stage.p, sizeof(double) * nd, hipMemcpyDeviceToHost, X.s));
+    HIP_CHECK(hipMemcpyAsync(D.head_h, D.head.p, sizeof(int32_t) * 3 * ne, hipMemcpyDeviceToHost, X.s));
+    return 0;
+}
+
+// Waits for a slice (its launches, the per-block CCS, the stopping rule and
+// the read-back of the staging record, all queued at issue) and takes the
+// blocks each request took, their CCS, errors and sums, and whether it has
+// stopped.
+int collect_slice(cosims_server *S, Slice &X)
+{
+    const auto &c = S->cfg;
+    BatchPlan &P = *X.P;
+    SliceDev &D = *X.D;
+    GroupRun R;
+    if (int rc = launch_finish(P, X.s, R)) return rc;  // waits for the stream
+    const size_t ne = X.e.size();
+    size_t off = 0;
     double units = 0;  // force-loop work of the slice (integration steps x atoms, or x clusters)
     for (size_t i = 0; i < ne; i++) {
         SJob &J = *X.e[i].job;
         const auto &B = P.jobs[i];
-        const int32_t *h = D.head_h.data() + 3 * i;
-        const double *rec = D.stage_h.data() + off;
+        const int32_t *h = D.head_h + 3 * i;
+        const double *rec = D.stage_h + off;
         units += rec[0] * B.units;
         if (h[2] < 0 || h[0] != J.used) return fail(COSIMS_E_INVALID, "server: blocks out of order");
         for (int k = h[0]; k < h[1]; k++) {
@@ -542,7 +584,8 @@ void scheduler(cosims_server *S)
     bool ok = true;
     for (auto &x : streams) ok &= hipStreamCreateWithFlags(&x.s, hipStreamNonBlocking) == hipSuccess;
     Slice fly[kMaxDepth];
-    int head = 0, inflight = 0;  // fly[head] is the oldest slice
+    hipEvent_t last_stop = nullptr;  // after the k_stop of the last slice issued
+    int head = 0, inflight = 0;      // fly[head] is the oldest slice
     double last_end = 0;         // busy time: union of the slices' issue-to-collect spans
     for (;;) {
         std::vector<std::shared_ptr<SJob>> act;
@@ -577,7 +620,7 @@ void scheduler(cosims_server *S)
             X.D = &devs[q];
             X.s = streams[q].s;
             X.t_issue = S->now_ms();
-            rc = issue_slice(S, X);
+            rc = issue_slice(S, X, last_stop);
             inflight++;
         }
         Slice &X = fly[head];
@@ -671,7 +714,7 @@ int cosims_server_create(const cosims_server_cfg *cfg, cosims_server **out)
     if (c.max_active <= 0) c.max_active = 64;
     if (c.block_traj <= 0) c.block_traj = 10000;
     if (c.slice_traj <= 0) c.slice_traj = 1 << 20;
-    if (c.slice_ms <= 0) c.slice_ms = 400;
+    if (c.slice_ms <= 0) c.slice_ms = 150;
     if (c.policy != COSIMS_SERVE_SHORTEST_FIRST && c.policy != COSIMS_SERVE_EQUAL_SHARE)
         return fail(COSIMS_E_INVALID, "unknown server policy");
     if (c.min_blocks <= 0) c.min_blocks = 3;

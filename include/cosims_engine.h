@@ -372,7 +372,7 @@ typedef struct cosims_server_cfg {
     int32_t setup_threads;    /* host setup workers; 0 => 8 */
     int64_t block_traj;       /* trajectories per block; 0 => 10000 */
     int64_t slice_traj;       /* trajectories per slice at most; 0 => 2^20 */
-    double slice_ms;          /* kernel time per slice to aim for; 0 => 400 */
+    double slice_ms;          /* kernel time per slice to aim for; 0 => 150 */
     int32_t policy;           /* COSIMS_SERVE_* */
     int64_t min_slice_traj;   /* shortest-first slices are topped up to this many; 0 => 65536 */
     int32_t pipeline_depth;   /* slices in flight at once (1..8); 0 => 3 */
