@@ -382,6 +382,61 @@ def copy_probe(nbytes=256 << 20, reps=3):
     log(f"copied {nbytes} bytes x{reps}")
 
 
+def serving_block(E, synth, a):
+    """The serving streams of the bench line (warm-up, the configuration's
+    stream, the sustained and the clustered streams): the keys they add."""
+    out = {"served_latency_ms": None}
+    cwg = [int(x) for x in a.serve_class_wg.split(",")]
+    cl = [int(x) for x in a.serve_class_lanes.split(",")]
+    own = bool(a.serve_own_launches)
+    if a.serve_warmup > 0:
+        wu = serving(E, synth, a.serve_warmup, 20.0, a.serve_target, seed=7000, class_wg=cwg, class_lanes=cl,
+                     own_launches=own)
+        log(f"serving warm-up: {a.serve_warmup} requests, {wu['trajectories_per_s']:.0f} traj/s (not reported)")
+    sv = serving(E, synth, a.serve_requests, a.serve_rate, a.serve_target, fp64_max_atoms=a.serve_fp64_atoms,
+                 class_wg=cwg, class_lanes=cl, own_launches=own)
+    out["serving"] = sv
+    # the config's stream (20 req/s, several times what one GPU serves: its percentiles measure
+    # the backlog) and a stream at a rate the GPU sustains
+    out["served_latency_ms"] = {"p50": sv["latency_ms"]["p50"], "p99": sv["latency_ms"]["p99"],
+                                "offered_rate": a.serve_rate, "requests": a.serve_requests}
+    if a.serve_sustained_rate > 0 and a.serve_sustained_requests > 0:
+        su = serving(E, synth, a.serve_sustained_requests, a.serve_sustained_rate, a.serve_target, seed=6000,
+                     class_wg=cwg, class_lanes=cl, own_launches=own)
+        out["serving_sustained"] = su
+        out["served_latency_ms"]["sustained"] = {"p50": su["latency_ms"]["p50"], "p99": su["latency_ms"]["p99"],
+                                                 "offered_rate": a.serve_sustained_rate,
+                                                 "requests": a.serve_sustained_requests}
+    if a.serve_compare:
+        eq = serving(E, synth, a.serve_requests, a.serve_rate, a.serve_target, policy=E.SERVE_EQUAL_SHARE,
+                     class_wg=cwg, class_lanes=cl, own_launches=own)
+        out["serving_equal_share"] = {k: eq[k] for k in ("latency_ms", "served_rate", "trajectories_per_s",
+                                                         "makespan_s", "slices", "trajectories_issued")}
+    if a.serve_clustered_requests > 0:
+        # requests with the reference's default cut-offs: PCM clustering in each request's setup
+        # and the clustered force type in its trajectories
+        cu = serving(E, synth, a.serve_clustered_requests, a.serve_clustered_rate, a.serve_target, seed=7000,
+                     fp64_max_atoms=a.serve_fp64_atoms, fp64_max_requests=8, clustered=True, class_wg=cwg,
+                     class_lanes=cl, own_launches=own)
+        out["serving_clustered"] = cu
+        out["served_latency_ms"]["clustered"] = {"p50": cu["latency_ms"]["p50"], "p99": cu["latency_ms"]["p99"],
+                                                 "offered_rate": a.serve_clustered_rate,
+                                                 "requests": a.serve_clustered_requests}
+    return out
+
+
+def serving_child(a):
+    """The serving block in a process of its own, as a server is deployed (a
+    process that only serves: the streams ran 10 % slower after the headline
+    legs in the same process); its JSON line is merged into this one."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__)] + sys.argv[1:] + ["--serve-child"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
+    if r.returncode != 0:
+        raise SystemExit(f"serving child failed with status {r.returncode}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -430,6 +485,10 @@ def main():
     ap.add_argument("--ceiling-probe", action="store_true",
                     help="only one mixed run of the headline workload at --lanes/--wg and the force-loop ceiling "
                          "probe at the same shape (for rocprofv3 counter passes over both kernels)")
+    ap.add_argument("--serve-in-process", action="store_true",
+                    help="run the serving streams in this process after the headline legs (default: in a child "
+                         "process that only serves)")
+    ap.add_argument("--serve-child", action="store_true", help=argparse.SUPPRESS)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); trajectories shard "
@@ -437,6 +496,12 @@ def main():
         sys.exit(2)
     if a.copy_probe:
         copy_probe()
+        return
+    if a.serve_child:  # the serving block alone, its keys as one JSON line (see serving_child)
+        from cosims_amd import engine as E
+        from cosims_amd import synth
+        E.lib()
+        print(json.dumps(serving_block(E, synth, a)), flush=True)
         return
     if a.ceiling_probe:
         from cosims_amd import engine as E
@@ -555,44 +620,10 @@ def main():
     threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
     if a.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(m, a.cpu_seconds, threads)
-    out["served_latency_ms"] = None
     if a.serve_requests > 0:
-        cwg = [int(x) for x in a.serve_class_wg.split(",")]
-        cl = [int(x) for x in a.serve_class_lanes.split(",")]
-        own = bool(a.serve_own_launches)
-        if a.serve_warmup > 0:
-            wu = serving(E, synth, a.serve_warmup, 20.0, a.serve_target, seed=7000, class_wg=cwg, class_lanes=cl,
-                         own_launches=own)
-            log(f"serving warm-up: {a.serve_warmup} requests, {wu['trajectories_per_s']:.0f} traj/s (not reported)")
-        sv = serving(E, synth, a.serve_requests, a.serve_rate, a.serve_target, fp64_max_atoms=a.serve_fp64_atoms,
-                     class_wg=cwg, class_lanes=cl, own_launches=own)
-        out["serving"] = sv
-        # the config's stream (20 req/s, several times what one GPU serves: its percentiles measure
-        # the backlog) and a stream at a rate the GPU sustains
-        out["served_latency_ms"] = {"p50": sv["latency_ms"]["p50"], "p99": sv["latency_ms"]["p99"],
-                                    "offered_rate": a.serve_rate, "requests": a.serve_requests}
-        if a.serve_sustained_rate > 0 and a.serve_sustained_requests > 0:
-            su = serving(E, synth, a.serve_sustained_requests, a.serve_sustained_rate, a.serve_target, seed=6000,
-                         class_wg=cwg, class_lanes=cl, own_launches=own)
-            out["serving_sustained"] = su
-            out["served_latency_ms"]["sustained"] = {"p50": su["latency_ms"]["p50"], "p99": su["latency_ms"]["p99"],
-                                                     "offered_rate": a.serve_sustained_rate,
-                                                     "requests": a.serve_sustained_requests}
-        if a.serve_compare:
-            eq = serving(E, synth, a.serve_requests, a.serve_rate, a.serve_target, policy=E.SERVE_EQUAL_SHARE,
-                         class_wg=cwg, class_lanes=cl, own_launches=own)
-            out["serving_equal_share"] = {k: eq[k] for k in ("latency_ms", "served_rate", "trajectories_per_s",
-                                                             "makespan_s", "slices", "trajectories_issued")}
-        if a.serve_clustered_requests > 0:
-            # requests with the reference's default cut-offs: PCM clustering in each request's setup
-            # and the clustered force type in its trajectories
-            cu = serving(E, synth, a.serve_clustered_requests, a.serve_clustered_rate, a.serve_target, seed=7000,
-                         fp64_max_atoms=a.serve_fp64_atoms, fp64_max_requests=8, clustered=True, class_wg=cwg,
-                         class_lanes=cl, own_launches=own)
-            out["serving_clustered"] = cu
-            out["served_latency_ms"]["clustered"] = {"p50": cu["latency_ms"]["p50"], "p99": cu["latency_ms"]["p99"],
-                                                     "offered_rate": a.serve_clustered_rate,
-                                                     "requests": a.serve_clustered_requests}
+        out.update(serving_block(E, synth, a) if a.serve_in_process else serving_child(a))
+    else:
+        out["served_latency_ms"] = None
     print(json.dumps(out), flush=True)
 
 
