@@ -437,8 +437,10 @@ __device__ __forceinline__ void force64_epilogue(double (&s)[12], double ds, dou
 // generic pointer): npad is a multiple of 64, so every lane makes the same
 // npad / L passes, and the next atom is read into registers of its own one
 // pair ahead, so the reads of the unrolled pairs go out together.  From
-// global memory (the table kernel), and in the tiled kernel, whose other
-// live state leaves no room for the prefetch registers, the plain loop.
+// global memory (the table kernel), the plain loop; the tiled kernel, whose
+// other live state leaves no room for the prefetch registers, has a loop of
+// its own (two pairs a pass at 32 and 64 lanes, no registers held across
+// passes).
 template <int FT, int L, typename PD>
 __device__ __forceinline__ void grp_force64(PD X, int npad, int j, double px, double py, double pz, double ds,
                                             double &ox, double &oy, double &oz)
