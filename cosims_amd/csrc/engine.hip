@@ -756,36 +756,45 @@ __global__ void __launch_bounds__(64) k_chain_zwalk(ChainArgs A)
     march_wave(A.m, w, A.mk, A.zwalk_points, A.st->z);
 }
 
-// ray_geometry (host) in the same IEEE operations
-__global__ void k_chain_geometry(ChainArgs A)
+// ray_geometry (host) in the same IEEE operations, from the z-axis walk's
+// crossing; count = 0 when the walk found none
+__device__ RayGeometry chain_geometry(const ChainArgs &A, const MarchHit &z, int &count)
 {
-    if (threadIdx.x != 0) return;
-    ChainState &S = *A.st;
-    S.count = 0;
-    if (S.z.found != 1) return;
-    RayGeometry g;
-    double zMax = (S.z.p[2] < S.z.prev[2]) ? S.z.prev[2] : S.z.p[2];
+    RayGeometry g{};
+    count = 0;
+    if (z.found != 1) return g;
+    double zMax = (z.p[2] < z.prev[2]) ? z.prev[2] : z.p[2];
     zMax *= 1.3;
     g.zmax = zMax;
     g.posmag = A.bp.posmag0 + fabs(zMax - A.bp.maxz);
     const int cz = (int)ceil(zMax);
     g.levels = (int)ceil((double)(25 < cz ? cz : 25));
     g.zinc = 2 * zMax / (g.levels);
-    S.g = g;
-    S.count = (g.levels + 1) * kBoundaryRot;
+    count = (g.levels + 1) * kBoundaryRot;
+    return g;
 }
 
+// Every block derives the ray geometry from the walk's crossing itself (a
+// few scalar operations; no launch of its own, which under a serving load
+// would wait for a CU slot like every launch); block 0 publishes it.
 __global__ void __launch_bounds__(64) k_chain_rays(ChainArgs A)
 {
-    const ChainState &S = *A.st;
+    ChainState &S = *A.st;
+    const MarchHit z = S.z;
+    int count;
+    const RayGeometry g = chain_geometry(A, z, count);
     const int r = blockIdx.x;
-    if (S.count > kChainMaxRays || r >= S.count) return;
+    if (r == 0 && threadIdx.x == 0) {
+        S.g = g;
+        S.count = count;
+    }
+    if (count > kChainMaxRays || r >= count) return;
     const int i = r / kBoundaryRot, n = r % kBoundaryRot;
     // ray_march (host) in the same IEEE operations
     March w;
-    w.start[0] = S.g.posmag * A.bp.cos_t[n];
-    w.start[1] = S.g.posmag * A.bp.sin_t[n];
-    w.start[2] = -S.g.zmax + i * S.g.zinc;
+    w.start[0] = g.posmag * A.bp.cos_t[n];
+    w.start[1] = g.posmag * A.bp.sin_t[n];
+    w.start[2] = -g.zmax + i * g.zinc;
     w.dir[0] = A.bp.dir_x[n]; w.dir[1] = A.bp.dir_y[n]; w.dir[2] = 0.0;
     w.prev0[0] = 0; w.prev0[1] = 0; w.prev0[2] = 0;
     w.first = 0;
@@ -846,26 +855,29 @@ struct GpuMarcher : BoundaryMarcher {
     int boundary_(const Prepared &p, double mk, const BoundaryPlan &bp, MarchHit &zhit, std::vector<MarchHit> &rays)
     {
         if (int rc = upload_atoms(p)) return rc;
-        DBuf dst, dh;
-        HIP_CHECK(dst.alloc_cached(sizeof(ChainState)));
-        HIP_CHECK(dh.alloc_cached(sizeof(MarchHit) * kChainMaxRays));
+        // the chain state and the hits in one buffer, read back by one copy
+        // (each launch and copy behind a serving load waits for a CU slot)
+        constexpr size_t kStateBytes = (sizeof(ChainState) + 255) / 256 * 256;
+        DBuf buf;
+        HIP_CHECK(buf.alloc_cached(kStateBytes + sizeof(MarchHit) * kChainMaxRays));
         ChainArgs A{};
         A.m = mol_dev(p);
         A.bp = bp;
         A.mk = mk;
         A.max_points = kMaxMarchPoints;
         A.zwalk_points = zwalk_points(bp.z0);
-        A.st = dst.as<ChainState>();
-        A.hits = dh.as<MarchHit>();
+        A.st = buf.as<ChainState>();
+        A.hits = (MarchHit *)(buf.as<char>() + kStateBytes);
         hipLaunchKernelGGL(k_chain_zwalk, dim3(1), dim3(64), 0, st.s, A);
-        hipLaunchKernelGGL(k_chain_geometry, dim3(1), dim3(64), 0, st.s, A);
         hipLaunchKernelGGL(k_chain_rays, dim3(kChainMaxRays), dim3(64), 0, st.s, A);
         HIP_CHECK(hipGetLastError());
-        ChainState S{};
-        HIP_CHECK(hipMemcpyAsync(&S, dst.p, sizeof(S), hipMemcpyDeviceToHost, st.s));
-        rays.resize(kChainMaxRays);
-        HIP_CHECK(hipMemcpyAsync(rays.data(), dh.p, sizeof(MarchHit) * kChainMaxRays, hipMemcpyDeviceToHost, st.s));
+        std::vector<char> host(kStateBytes + sizeof(MarchHit) * kChainMaxRays);
+        HIP_CHECK(hipMemcpyAsync(host.data(), buf.p, host.size(), hipMemcpyDeviceToHost, st.s));
         HIP_CHECK(hipStreamSynchronize(st.s));
+        ChainState S;
+        std::memcpy(&S, host.data(), sizeof(S));
+        rays.resize(kChainMaxRays);
+        std::memcpy(rays.data(), host.data() + kStateBytes, sizeof(MarchHit) * kChainMaxRays);
         zhit = S.z;
         if (S.z.found != 1) {
             rays.clear();
